@@ -1,0 +1,117 @@
+"""ctypes binding of libams.so (include/ams.h).
+
+The library is the only compute path: if it is missing or fails to load this
+module raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libams.so"
+
+AMS_OK = 0
+
+
+class AmsArray(C.Structure):
+    _fields_ = [
+        ("n0", C.c_int32),
+        ("n1", C.c_int32),
+        ("pos", C.POINTER(C.c_float)),
+        ("dir", C.c_float * 3),
+        ("bptype", C.c_int32),
+        ("radius", C.c_float),
+    ]
+
+
+class AmsTraceOpts(C.Structure):
+    _fields_ = [
+        ("wave_len", C.c_double),
+        ("max_reflection", C.c_int32),
+        ("gridsize", C.c_int32),
+        ("fib_n", C.c_int64),
+        ("per_element", C.c_int32),
+        ("flags", C.c_int32),
+        ("tasks_per_unit", C.c_int32),
+        ("stream", C.c_void_p),
+        ("keys_out", C.POINTER(C.c_uint64)),
+        ("keys_cap", C.c_int64),
+        ("seed_keys", C.POINTER(C.c_uint64)),
+        ("n_seed", C.c_int64),
+    ]
+
+
+STAT_INT_FIELDS = [
+    "rays", "records_raw", "records_unique", "paths", "tasks", "rounds", "chains", "successes", "overflow",
+    "fwd_queries", "fwd_nodes", "fwd_tris", "fwd_spheres", "bwd_queries", "bwd_nodes", "bwd_tris",
+]
+STAT_DBL_FIELDS = ["t_discover_ms", "t_evaluate_ms", "t_total_ms"]
+
+
+class AmsTraceStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in STAT_INT_FIELDS] + [(n, C.c_double) for n in STAT_DBL_FIELDS]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n in STAT_INT_FIELDS + STAT_DBL_FIELDS}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libams.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m automs_amd.build` "
+            "(the HIP library is the only compute path)")
+    L = C.CDLL(str(LIB_PATH))
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int32)
+    L.ams_scene_create.argtypes = [fp, C.c_int32, ip, C.c_int32, fp, fp, C.POINTER(C.c_void_p)]
+    L.ams_scene_create.restype = C.c_int
+    L.ams_scene_destroy.argtypes = [C.c_void_p]
+    L.ams_scene_destroy.restype = C.c_int
+    L.ams_scene_plane_ids.argtypes = [C.c_void_p, ip]
+    L.ams_scene_plane_ids.restype = C.c_int
+    L.ams_trace_channel.argtypes = [C.c_void_p, C.POINTER(AmsArray), C.POINTER(AmsArray),
+                                    C.POINTER(AmsTraceOpts), C.c_void_p, C.POINTER(AmsTraceStats)]
+    L.ams_trace_channel.restype = C.c_int
+    L.ams_last_error.argtypes = []
+    L.ams_last_error.restype = C.c_char_p
+    L.ams_version.restype = C.c_int
+    L.ams_mem_outstanding.restype = C.c_int64
+    _bind_contract(L)
+    _lib = L
+    return L
+
+
+def _bind_contract(L):
+    """Declare the contraction entry points when the library exports them."""
+    vp = C.c_void_p
+    sigs = {
+        "ams_contract_fwd": [vp, vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp],
+        "ams_contract_bwd": [vp, vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp],
+        "ams_objective_fwd": [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, vp, vp, vp, vp],
+        "ams_objective_bwd": [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, vp, vp, vp, vp],
+        "ams_adam_step": [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, vp],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name, None)
+        if f is not None:
+            f.argtypes = args
+            f.restype = C.c_int
+
+
+def check(rc: int, what: str) -> None:
+    if rc != AMS_OK:
+        msg = lib().ams_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+
+
+EXPORTED = [
+    "ams_scene_create", "ams_scene_destroy", "ams_scene_plane_ids", "ams_trace_channel",
+    "ams_last_error", "ams_version", "ams_mem_outstanding",
+]

@@ -1,0 +1,18 @@
+// ams_bvh.h -- host-side binned-SAH BVH builder (FP64 bounds, FP32 output
+// rounded outward) producing 32-byte threaded nodes in depth-first preorder.
+#pragma once
+#include <vector>
+
+#include "ams_common.h"
+
+struct AmsBvhBuild {
+  std::vector<AmsNode> nodes;  // depth-first preorder with skip links
+  std::vector<int> order;      // leaf slot -> primitive index
+};
+
+// lo/hi: per-primitive bounds [n][3] in FP64.  leaf_max <= 7.
+void ams_bvh_build(int n, const double *lo, const double *hi, int leaf_max, AmsBvhBuild *out);
+
+// Plane ids: for each triangle, the smallest triangle index lying on the same
+// plane (unit normal and offset equal after canonical sign, to 1e-6 / 1e-5 m).
+void ams_plane_ids(int n_tri, const float *tri, int *plane_out, double *pl_n_out, double *pl_d_out);

@@ -1,0 +1,159 @@
+// ams_capi.cpp -- extern "C" entry points: argument validation, error text,
+// scene lifetime and allocation bookkeeping.  Compute lives in the .hip files.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/ams.h"
+#include "ams_internal.h"
+
+namespace {
+thread_local std::string g_err;
+std::mutex g_mem_mu;
+std::unordered_map<void *, size_t> g_mem;  // live device allocations
+int64_t g_mem_bytes = 0;
+}  // namespace
+
+namespace ams {
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+void *dmalloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  g_mem[p] = bytes;
+  g_mem_bytes += (int64_t)bytes;
+  return p;
+}
+
+void dfree(void *p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_mem.find(p);
+    if (it != g_mem.end()) {
+      g_mem_bytes -= (int64_t)it->second;
+      g_mem.erase(it);
+    }
+  }
+  (void)hipFree(p);
+}
+
+int64_t outstanding() {
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  return g_mem_bytes;
+}
+
+}  // namespace ams
+
+extern "C" {
+
+const char *ams_last_error(void) { return g_err.c_str(); }
+
+int ams_version(void) { return 1; }
+
+int64_t ams_mem_outstanding(void) { return ams::outstanding(); }
+
+int ams_scene_create(const float *tri, int32_t n_tri, const int32_t *mat, int32_t n_mat, const float *mat_prop,
+                     const float *refl_phase, ams_scene **out) {
+  if (!out) {
+    ams::set_error("ams_scene_create: out is NULL");
+    return AMS_ERR_ARG;
+  }
+  *out = nullptr;
+  if (n_tri < 0 || n_tri > 0xFFFFE || (n_tri > 0 && (!tri || !mat)) || n_mat < 0 ||
+      (n_mat > 0 && !mat_prop)) {
+    ams::set_error("ams_scene_create: bad triangle / material arguments (n_tri must be < 1048575)");
+    return AMS_ERR_ARG;
+  }
+  for (int i = 0; i < n_tri; i++) {
+    if (mat[i] < 0 || mat[i] >= n_mat) {
+      ams::set_error("ams_scene_create: material index out of range at triangle " + std::to_string(i));
+      return AMS_ERR_ARG;
+    }
+  }
+  ams_scene *s = new ams_scene();
+  int rc = ams::upload_scene(s, n_tri, tri, mat, n_mat, mat_prop, refl_phase);
+  if (rc != AMS_OK) {
+    ams_scene_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return AMS_OK;
+}
+
+int ams_scene_destroy(ams_scene *s) {
+  if (!s) return AMS_OK;
+  s->scratch.release();
+  ams::dfree(s->d_tnodes);
+  ams::dfree(s->d_tris);
+  ams::dfree(s->d_aux);
+  ams::dfree(s->d_planes);
+  ams::dfree(s->d_mat_prop);
+  ams::dfree(s->d_refl_phase);
+  delete s;
+  return AMS_OK;
+}
+
+int ams_scene_plane_ids(const ams_scene *s, int32_t *out) {
+  if (!s || !out) {
+    ams::set_error("ams_scene_plane_ids: NULL argument");
+    return AMS_ERR_ARG;
+  }
+  for (int i = 0; i < s->n_tri; i++) out[i] = s->plane[i];
+  return AMS_OK;
+}
+
+static bool check_array(const ams_array *a, const char *what) {
+  if (!a || a->n0 <= 0 || a->n1 <= 0 || !a->pos) {
+    ams::set_error(std::string("ams_trace_channel: bad ") + what + " array");
+    return false;
+  }
+  if (a->bptype < 0 || a->bptype > 3) {
+    // optixPathTracer.cu:110-135 has no default case (undefined result)
+    ams::set_error(std::string("ams_trace_channel: ") + what + " beam pattern type must be 0..3");
+    return false;
+  }
+  if (!(a->radius >= 0.0f)) {
+    ams::set_error(std::string("ams_trace_channel: ") + what + " radius must be >= 0");
+    return false;
+  }
+  return true;
+}
+
+int ams_trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const ams_trace_opts *o,
+                      float *H_out, ams_trace_stats *stats) {
+  if (!s || !o || !H_out) {
+    ams::set_error("ams_trace_channel: NULL scene, options or output");
+    return AMS_ERR_ARG;
+  }
+  if (!check_array(tx, "tx") || !check_array(rx, "rx")) return AMS_ERR_ARG;
+  if (o->max_reflection < 1 || o->max_reflection > AMS_MAX_HIT) {
+    ams::set_error("ams_trace_channel: max_reflection must be in 1..5");
+    return AMS_ERR_ARG;
+  }
+  if (o->gridsize < 1 || o->fib_n < 1 || !(o->wave_len > 0)) {
+    ams::set_error("ams_trace_channel: gridsize, fib_n and wave_len must be positive");
+    return AMS_ERR_ARG;
+  }
+  int n1g = (rx->n1 - 1) / o->gridsize + 1, n0g = (rx->n0 - 1) / o->gridsize + 1;
+  if ((long long)n1g * n0g >= (1LL << 20)) {
+    ams::set_error("ams_trace_channel: more than 2^20 Rx tiles");
+    return AMS_ERR_ARG;
+  }
+  long long items = (long long)o->gridsize * o->gridsize * tx->n0 * tx->n1;
+  if (items > (1LL << 30)) {
+    ams::set_error("ams_trace_channel: gridsize^2 * n_tx too large");
+    return AMS_ERR_ARG;
+  }
+  return ams::trace_channel(s, tx, rx, o, H_out, stats);
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// ams_internal.h -- library-private state: the scene handle, a named scratch
+// pool reused across calls, device allocation bookkeeping and error text.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ams.h"
+#include "ams_common.h"
+
+namespace ams {
+
+void set_error(const std::string &msg);
+
+// Device allocations with a pointer -> size record, so that every byte the
+// library holds can be reported (the role of cudaMemManager.h:23-81).
+void *dmalloc(size_t bytes);
+void dfree(void *p);
+int64_t outstanding();
+
+template <typename T>
+int dev_upload(T **dst, const void *src, size_t bytes) {
+  *dst = (T *)dmalloc(bytes > 0 ? bytes : 16);
+  if (!*dst) return AMS_ERR_ALLOC;
+  if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("hipMemcpy upload failed");
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+// Named device buffers that only grow; reused by later calls on the handle.
+struct Scratch {
+  std::map<std::string, std::pair<void *, size_t>> bufs;
+  void *get(const char *name, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    auto &b = bufs[name];
+    if (b.second >= bytes) return b.first;
+    if (b.first) dfree(b.first);
+    size_t want = bytes + bytes / 4;
+    b.first = dmalloc(want);
+    b.second = b.first ? want : 0;
+    if (!b.first) set_error(std::string("device allocation failed: ") + name);
+    return b.first;
+  }
+  void release() {
+    for (auto &kv : bufs)
+      if (kv.second.first) dfree(kv.second.first);
+    bufs.clear();
+  }
+};
+
+}  // namespace ams
+
+struct ams_scene {
+  int n_tri = 0, n_mat = 0;
+  std::vector<int> plane;
+  AmsNode *d_tnodes = nullptr;
+  int n_tnodes = 0;
+  AmsTri *d_tris = nullptr;
+  AmsTriAux *d_aux = nullptr;
+  AmsPlane *d_planes = nullptr;
+  float *d_mat_prop = nullptr;
+  float *d_refl_phase = nullptr;
+  ams::Scratch scratch;
+};
+
+namespace ams {
+int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, int32_t n_mat,
+                 const float *mat_prop, const float *refl_phase);
+int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const ams_trace_opts *o,
+                  float *H_out, ams_trace_stats *stats);
+}  // namespace ams

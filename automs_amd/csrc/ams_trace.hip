@@ -1,0 +1,1066 @@
+// ams_trace.hip -- gfx950 channel tracer: forward path discovery, GPU dedup
+// of (path, tile) records, backward image-method evaluation with register
+// accumulation per (Tx, Rx) lane, and the neighbour-expansion rounds.
+//
+// Reference behaviour restated (file:line in the reference tree):
+//   raygen stage 1 / closest-hit forward  optixPathTracer.cu:142-190, :285-331
+//   raygen stage 2 / closest-hit backward optixPathTracer.cu:191-273, :332-417
+//   sphere intersection                   optixPathTracer.cu:419-444
+//   beam patterns                         optixPathTracer.cu:107-136
+//   record compaction, dedup, rounds      optixPathTracer.cpp:93-360
+//   torch op entry                        optixPathTracer.cpp:781-861
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/ams.h"
+#include "ams_bvh.h"
+#include "ams_common.h"
+#include "ams_internal.h"
+
+// ------------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ float sel3(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 norm3(float3 a) { return a * rsqrtf(dot3(a, a)); }
+
+// Per-ray state of the watertight triangle test (Woop, Benthin, Wald,
+// "Watertight Ray/Triangle Intersection", JCGT 2013).
+struct Ray {
+  float3 o, d, inv;
+  int kx, ky, kz;
+  float sx, sy, sz;
+};
+
+__device__ __forceinline__ void ray_setup(Ray &r, float3 o, float3 d) {
+  r.o = o;
+  r.d = d;
+  r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  if (sel3(d, kz) < 0.0f) {
+    int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  float dz = sel3(d, kz);
+  r.sx = sel3(d, kx) / dz;
+  r.sy = sel3(d, ky) / dz;
+  r.sz = 1.0f / dz;
+}
+
+// Watertight test; returns true and t if the hit lies in [tmin, tmax].
+__device__ __forceinline__ bool tri_test(const Ray &r, float4 a, float4 b, float4 c, float tmin, float tmax,
+                                         float &t_out) {
+  float3 A = f3(a.x, a.y, a.z) - r.o, B = f3(b.x, b.y, b.z) - r.o, C = f3(c.x, c.y, c.z) - r.o;
+  float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
+  float Ax = sel3(A, r.kx) - r.sx * Akz, Ay = sel3(A, r.ky) - r.sy * Akz;
+  float Bx = sel3(B, r.kx) - r.sx * Bkz, By = sel3(B, r.ky) - r.sy * Bkz;
+  float Cx = sel3(C, r.kx) - r.sx * Ckz, Cy = sel3(C, r.ky) - r.sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  float det = U + V + W;
+  if (det == 0.0f) return false;
+  float T = U * (r.sz * Akz) + V * (r.sz * Bkz) + W * (r.sz * Ckz);
+  if (det < 0.0f) {
+    det = -det;
+    T = -T;
+  }
+  if (T < tmin * det || T > tmax * det) return false;
+  t_out = T / det;
+  return t_out >= tmin && t_out <= tmax;
+}
+
+__device__ __forceinline__ bool box_test(const Ray &r, float4 lo, float4 hi, float tmin, float tmax) {
+  float tx0 = (lo.x - r.o.x) * r.inv.x, tx1 = (hi.x - r.o.x) * r.inv.x;
+  float ty0 = (lo.y - r.o.y) * r.inv.y, ty1 = (hi.y - r.o.y) * r.inv.y;
+  float tz0 = (lo.z - r.o.z) * r.inv.z, tz1 = (hi.z - r.o.z) * r.inv.z;
+  float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+  float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+  // conservative margin of 2 gamma(3) on the far distance (Ize 2013)
+  return t0 <= t1 * 1.0000004f;
+}
+
+struct Counters {
+  unsigned long long nodes = 0, tris = 0, sph = 0, queries = 0;
+};
+
+// Closest scene triangle in [tmin, tbest], triangles of plane skip_pl are
+// ignored (a straight ray meets a plane once).  Ties on t go to the smaller
+// original triangle index.  Returns the leaf-order slot or -1.
+__device__ __forceinline__ int closest_tri(const AmsNode *__restrict__ nodes, int n_nodes,
+                                           const AmsTri *__restrict__ tris, const Ray &r, float tmin,
+                                           float &tbest, int skip_pl, int &orig_out, Counters &ct) {
+  int hit = -1, horig = 0x7fffffff;
+  int i = 0;
+  while (i < n_nodes) {
+    const float4 *np = reinterpret_cast<const float4 *>(nodes + i);
+    float4 lo = np[0], hi = np[1];
+    ct.nodes++;
+    int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
+    if (box_test(r, lo, hi, tmin, tbest)) {
+      if (info == 0) {
+        i++;
+        continue;
+      }
+      int first = info >> 3, cnt = info & 7;
+      for (int k = 0; k < cnt; k++) {
+        const AmsTri tr = tris[first + k];
+        ct.tris++;
+        if (__float_as_int(tr.b.w) == skip_pl) continue;
+        float t;
+        if (tri_test(r, tr.a, tr.b, tr.c, tmin, tbest, t)) {
+          int orig = __float_as_int(tr.a.w);
+          if (t < tbest || orig < horig) {
+            tbest = t;
+            hit = first + k;
+            horig = orig;
+          }
+        }
+      }
+    }
+    i = miss;
+  }
+  orig_out = horig;
+  return hit;
+}
+
+// Closest reception sphere whose closest approach b lies in [tmin, tmax],
+// with miss distance below the acceptance radius and the origin outside the
+// sphere (optixPathTracer.cu:436-444).  Reported at t = b.
+__device__ __forceinline__ int closest_sphere(const AmsNode *__restrict__ nodes, int n_nodes,
+                                              const float4 *__restrict__ sph, const Ray &r, float tmin,
+                                              float &tbest, float acc2, float r2, Counters &ct) {
+  int hit = -1, horig = 0x7fffffff;
+  int i = 0;
+  while (i < n_nodes) {
+    const float4 *np = reinterpret_cast<const float4 *>(nodes + i);
+    float4 lo = np[0], hi = np[1];
+    ct.nodes++;
+    int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
+    if (box_test(r, lo, hi, tmin, tbest)) {
+      if (info == 0) {
+        i++;
+        continue;
+      }
+      int first = info >> 3, cnt = info & 7;
+      for (int k = 0; k < cnt; k++) {
+        float4 s = sph[first + k];
+        ct.sph++;
+        float3 O = f3(s.x, s.y, s.z) - r.o;
+        float b = dot3(O, r.d);
+        float3 P = O - r.d * b;
+        float perp2 = dot3(P, P);
+        float c = dot3(O, O);
+        int orig = __float_as_int(s.w);
+        if (perp2 < acc2 && b >= tmin && b <= tbest && c > r2) {
+          if (b < tbest || orig < horig) {
+            tbest = b;
+            hit = orig;
+            horig = orig;
+          }
+        }
+      }
+    }
+    i = miss;
+  }
+  return hit;
+}
+
+__device__ __forceinline__ int theta_bin(float3 d, float3 N) {
+  // optixPathTracer.cu:304-306
+  float c = dot3(d, N) * rsqrtf(dot3(N, N) * dot3(d, d));
+  float th = acosf(fminf(1.0f, fabsf(c)));
+  int id = (int)((th / (float)(M_PI / 2) * 36.0f + 1.0f) / 2.0f);
+  return id > 18 ? 18 : id;
+}
+
+// optixPathTracer.cu:107-136
+__device__ __forceinline__ float beam_pattern(float3 a, float3 b, int type) {
+  if (type == 0) return 1.0f;
+  float c = dot3(a, b) * rsqrtf(dot3(a, a) * dot3(b, b));
+  switch (type) {
+  case 1:
+    return c < 0.0f ? 0.0f : c * 2.4495f;
+  case 2: {
+    float th = acosf(fminf(1.0f, fmaxf(-1.0f, c)));
+    if (th > (float)(M_PI * 2 / 3)) return 0.1f;
+    if (th > (float)(M_PI * 5 / 12))
+      return powf(10.0f, ((float)(M_PI * 2 / 3) - th) / (float)M_PI * 4.0f * 0.41300368f - 1.0f) * 2.4495f;
+    return c * 2.4495f;
+  }
+  case 3:
+    return c < 0.0f ? 0.0f : 1.4142f;
+  default:
+    return 0.0f;  // rejected on the host before launch
+  }
+}
+
+__device__ __forceinline__ void key_set_plane(uint64_t &hi, uint64_t &lo, int k, uint32_t p) {
+  switch (k) {
+  case 0:
+    hi = (hi & ~(0xFFFFFull << 36)) | ((uint64_t)p << 36);
+    break;
+  case 1:
+    hi = (hi & ~(0xFFFFFull << 16)) | ((uint64_t)p << 16);
+    break;
+  case 2:
+    hi = (hi & ~0xFFFFull) | ((uint64_t)p >> 4);
+    lo = (lo & ~(0xFull << 60)) | ((uint64_t)(p & 0xF) << 60);
+    break;
+  case 3:
+    lo = (lo & ~(0xFFFFFull << 40)) | ((uint64_t)p << 40);
+    break;
+  default:
+    lo = (lo & ~(0xFFFFFull << 20)) | ((uint64_t)p << 20);
+    break;
+  }
+}
+
+__device__ __forceinline__ uint32_t key_get_plane(uint64_t hi, uint64_t lo, int k) {
+  switch (k) {
+  case 0:
+    return (uint32_t)((hi >> 36) & 0xFFFFF);
+  case 1:
+    return (uint32_t)((hi >> 16) & 0xFFFFF);
+  case 2:
+    return (uint32_t)(((hi & 0xFFFF) << 4) | (lo >> 60));
+  case 3:
+    return (uint32_t)((lo >> 40) & 0xFFFFF);
+  default:
+    return (uint32_t)((lo >> 20) & 0xFFFFF);
+  }
+}
+
+// Wave-aggregated append slot: one atomic per wave for the active lanes.
+__device__ __forceinline__ unsigned long long wave_append(unsigned long long *counter) {
+  unsigned long long mask = __ballot(1);
+  int lane = __lane_id();
+  int leader = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const Counters &c) {
+  unsigned long long v[4] = {c.nodes, c.tris, c.sph, c.queries};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    unsigned long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (__lane_id() == 0 && x) atomicAdd(dst + k, x);
+  }
+}
+
+// ------------------------------------------------------------------------
+// forward discovery
+// ------------------------------------------------------------------------
+struct DiscoverArgs {
+  const AmsNode *tnodes;
+  int n_tnodes;
+  const AmsTri *tris;
+  const AmsTriAux *aux;
+  const float *mat_prop;
+  const AmsNode *snodes;
+  int n_snodes;
+  const float4 *sph;
+  const float *src;
+  int n_src;
+  int n1_dst, gs;
+  float acc2, r2;
+  int K;
+  long long fib_n;
+  int per_element;
+  long long ray_begin, ray_end;
+  uint64_t *out_hi, *out_lo;
+  unsigned long long *out_count;
+  unsigned long long cap;
+  unsigned long long *counters;  // nodes, tris, spheres, queries, records(raw)
+};
+
+struct Branch {
+  float3 o, d;
+  int ttl, b, nref, last_pl;
+  uint64_t khi, klo;
+};
+
+constexpr int kDedup = 4;
+
+__global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
+  long long r = a.ray_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  Counters ct;
+  bool active = r < a.ray_end;
+  unsigned long long nrec = 0;
+  if (active) {
+    long long e = a.per_element ? r / a.fib_n : r % a.n_src;
+    long long k = a.per_element ? r % a.fib_n : r;
+    // Fibonacci direction in FP64, the azimuth range-reduced in cycles
+    const double g = 0.38196601125010515179541316563436;
+    double z = 1.0 - (2.0 * (double)k + 1.0) / (double)a.fib_n;
+    double rr = sqrt(fmax(0.0, 1.0 - z * z));
+    double x = (double)k * g;
+    double f = x - floor(x);
+    double sn, cs;
+    sincos(2.0 * M_PI * f, &sn, &cs);
+    Branch st[AMS_MAX_HIT];
+    int sp = 0;
+    Branch cur;
+    cur.o = f3(a.src[3 * e], a.src[3 * e + 1], a.src[3 * e + 2]);
+    cur.d = f3((float)(rr * cs), (float)(rr * sn), (float)z);
+    cur.ttl = a.K + 1;
+    cur.b = 0;
+    cur.nref = 0;
+    cur.last_pl = -1;
+    cur.khi = (0xFFFFFull << 36) | (0xFFFFFull << 16) | 0xFFFFull;
+    cur.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
+    uint64_t seen_hi[kDedup], seen_lo[kDedup];
+    int n_seen = 0;
+    bool more = true;
+    while (more) {
+      while (cur.ttl > 0) {
+        Ray ray;
+        ray_setup(ray, cur.o, cur.d);
+        float tt = INFINITY;
+        int torig;
+        int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+        float ts = tt;
+        int rx = closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
+        ct.queries++;
+        if (rx >= 0 && ts < tt) {
+          // Rx hit: record (key, tile) and continue from the closest approach
+          int h = a.K + 1 - cur.ttl;
+          uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
+          uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
+          uint64_t lo = (cur.klo & ~0xFFFFFull) | tile;
+          bool dup = false;
+          for (int q = 0; q < kDedup; q++)
+            if (q < n_seen && seen_hi[q] == hi && seen_lo[q] == lo) dup = true;
+          if (!dup) {
+            seen_hi[n_seen % kDedup] = hi;
+            seen_lo[n_seen % kDedup] = lo;
+            n_seen++;
+            unsigned long long idx = wave_append(a.out_count);
+            nrec++;
+            if (idx < a.cap) {
+              a.out_hi[idx] = hi;
+              a.out_lo[idx] = lo;
+            }
+          }
+          cur.o = cur.o + cur.d * ts;
+          cur.last_pl = -1;
+          continue;
+        }
+        if (slot < 0) break;  // miss, optixPathTracer.cu:276-279
+        const AmsTri tr = a.tris[slot];
+        int pl = __float_as_int(tr.b.w);
+        cur.o = cur.o + cur.d * tt;
+        cur.last_pl = pl;
+        if (cur.ttl > 1) {
+          // optixPathTracer.cu:294-328
+          float4 nm = a.aux[torig].n_maxedge;
+          float3 N0 = f3(nm.x, nm.y, nm.z);
+          float3 N = dot3(N0, cur.d) > 0.0f ? N0 * -1.0f : N0;  // faceforward(N0, -d, N0)
+          int m = __float_as_int(tr.c.w);
+          int th = theta_bin(cur.d, N);
+          bool pen = a.mat_prop[th + 19 * m] < 0.0f;
+          cur.ttl--;
+          int h = a.K + 1 - cur.ttl;
+          if (pen && sp < AMS_MAX_HIT) st[sp++] = cur;  // transmitted parent continues later
+          cur.b |= 1 << (h - 1);
+          cur.d = norm3(cur.d - N * (2.0f * dot3(cur.d, N)));
+          if (cur.nref < AMS_MAX_HIT) key_set_plane(cur.khi, cur.klo, cur.nref, (uint32_t)pl);
+          cur.nref++;
+        }
+        // ttl == 1: scene is pass-through (optixPathTracer.cu:294)
+      }
+      if (sp > 0) cur = st[--sp];
+      else more = false;
+    }
+  }
+  wave_add_counters(a.counters, ct);
+  unsigned long long x = nrec;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+  if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
+}
+
+// ------------------------------------------------------------------------
+// backward evaluation
+// ------------------------------------------------------------------------
+struct EvalArgs {
+  const AmsNode *tnodes;
+  int n_tnodes;
+  const AmsTri *tris;
+  const AmsTriAux *aux;
+  const float *mat_prop;
+  const float *refl_phase;
+  const AmsPlane *planes;
+  const float *src;
+  int n_src;
+  const float *dst;
+  int n0_dst, n1_dst, n_dst, gs, n1g, flags;
+  double wave_len_d;
+  float wave_len;
+  float src_r2, src_acc2;
+  float3 src_dir, dst_dir;
+  int src_bptype, dst_bptype;
+  const uint64_t *path_hi, *path_lo;
+  const int *task_path;       // tasks of this round, grouped by tile
+  const int *unit_tile;       // work unit -> tile
+  const int *unit_begin;      // work unit -> first task
+  const int *unit_end;        // work unit -> end task
+  int items;                  // gs*gs*n_src
+  double *H;                  // [n_src][n_dst][2]
+  unsigned char *task_ok;
+  unsigned long long *counters;  // nodes, tris, spheres, queries, chains, successes
+};
+
+__global__ void __launch_bounds__(512) k_evaluate(EvalArgs a) {
+  int unit = blockIdx.y;
+  int item = blockIdx.x * blockDim.x + threadIdx.x;
+  int g = a.unit_tile[unit];
+  int tb = a.unit_begin[unit], te = a.unit_end[unit];
+  int q = item / a.n_src, i = item % a.n_src;
+  int gi = g / a.n1g, gj = g % a.n1g;
+  int di = gi * a.gs + q / a.gs, dj = gj * a.gs + q % a.gs;
+  int j = di * a.n1_dst + dj;
+  // optixPathTracer.cu:211-220 (flags bit 0 keeps the reference's row wrap)
+  bool active = item < a.items && j < a.n_dst && (dj < a.n1_dst || (a.flags & 1));
+  Counters ct;
+  unsigned long long nchain = 0, nsucc = 0;
+  double acc_re = 0.0, acc_im = 0.0;
+  double3 si = make_double3(0, 0, 0), rj = make_double3(0, 0, 0);
+  if (active) {
+    si = make_double3(a.src[3 * i], a.src[3 * i + 1], a.src[3 * i + 2]);
+    rj = make_double3(a.dst[3 * j], a.dst[3 * j + 1], a.dst[3 * j + 2]);
+  }
+  for (int t = tb; t < te; t++) {
+    bool ok = false;
+    if (active) {
+      int p = a.task_path[t];
+      uint64_t khi = a.path_hi[p], klo = a.path_lo[p];
+      int h = (int)(khi >> 61), bm = (int)((khi >> 56) & 31);
+      int nref = __popc(bm & ((1 << h) - 1));
+      // image of the Tx element across the reflection planes (FP64)
+      double3 I = si;
+      for (int k = 0; k < nref; k++) {
+        const AmsPlane P = a.planes[key_get_plane(khi, klo, k)];
+        double s = 2.0 * (I.x * P.n[0] + I.y * P.n[1] + I.z * P.n[2] - P.d);
+        I.x -= s * P.n[0];
+        I.y -= s * P.n[1];
+        I.z -= s * P.n[2];
+      }
+      double ex = I.x - rj.x, ey = I.y - rj.y, ez = I.z - rj.z;
+      double L = sqrt(ex * ex + ey * ey + ez * ez);
+      float3 d = f3((float)(ex / L), (float)(ey / L), (float)(ez / L));
+      float3 o = f3((float)rj.x, (float)rj.y, (float)rj.z);
+      float3 s3 = f3((float)si.x, (float)si.y, (float)si.z);
+      float A = beam_pattern(d, a.dst_dir, a.dst_bptype);
+      float D = 0.0f, Dres = 0.0f, far = 1e6f, psi = 0.0f;
+      int ttl = h + 1, ri = 0, last_pl = -1;
+      nchain++;
+      while (ttl > 0) {
+        Ray ray;
+        ray_setup(ray, o, d);
+        float tt = INFINITY;
+        int torig;
+        int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
+        ct.queries++;
+        // own Tx sphere (other Tx spheres are pass-through: optixPathTracer.cu:396)
+        float3 O = s3 - o;
+        float bs = dot3(O, d);
+        float3 Pp = O - d * bs;
+        bool sh = dot3(Pp, Pp) < a.src_acc2 && bs >= AMS_TMIN && dot3(O, O) > a.src_r2;
+        if (sh && bs < tt) {
+          if (ttl == 1) {
+            // optixPathTracer.cu:398-406
+            D += bs;
+            A *= beam_pattern(d * -1.0f, a.src_dir, a.src_bptype) * a.wave_len /
+                 (4.0f * (float)M_PI * (Dres + D));
+            double cyc = L / a.wave_len_d;
+            float ang = (float)(2.0 * M_PI * (cyc - floor(cyc))) - psi;
+            float sn, cs;
+            sincosf(ang, &sn, &cs);
+            acc_re += (double)(A * cs);
+            acc_im += (double)(-A * sn);
+            ok = true;
+          }
+          break;
+        }
+        if (slot < 0) break;  // miss
+        const AmsTri tr = a.tris[slot];
+        int pl = __float_as_int(tr.b.w);
+        o = o + d * tt;
+        D += tt;
+        last_pl = pl;
+        if (ttl == 1) break;  // blocked, optixPathTracer.cu:348-349
+        float4 nm = a.aux[torig].n_maxedge;
+        float3 N0 = f3(nm.x, nm.y, nm.z);
+        float3 N = dot3(N0, d) > 0.0f ? N0 * -1.0f : N0;
+        int m = __float_as_int(tr.c.w);
+        int th = theta_bin(d, N);
+        float loss = fabsf(a.mat_prop[th + 19 * m]);
+        ttl--;
+        int k = h + 1 - ttl;
+        bool refl = (bm >> (k - 1)) & 1;
+        if (refl) {
+          if (ri >= nref || (uint32_t)pl != key_get_plane(khi, klo, nref - 1 - ri)) break;
+          ri++;
+        }
+        // optixPathTracer.cu:369-391
+        float fspl = 1.0f;
+        if (D > far) {
+          fspl = a.wave_len / (4.0f * (float)M_PI * Dres);
+          Dres = 0.0f;
+        }
+        A *= loss * fspl;
+        Dres += D;
+        float fk = 2.0f * nm.w * nm.w / a.wave_len;
+        far = D > fk ? fk : 1e6f;
+        D = 0.0f;
+        if (refl) {
+          d = norm3(d - N * (2.0f * dot3(d, N)));
+          psi += a.refl_phase[m];
+        }
+      }
+    }
+    if (ok) nsucc++;
+    unsigned long long any = __ballot(ok);
+    if (any && __lane_id() == __ffsll((long long)any) - 1) a.task_ok[t] = 1;
+  }
+  if (active && (acc_re != 0.0 || acc_im != 0.0)) {
+    double *hp = a.H + 2 * ((size_t)i * a.n_dst + j);
+    atomicAdd(hp, acc_re);
+    atomicAdd(hp + 1, acc_im);
+  }
+  wave_add_counters(a.counters, ct);
+  unsigned long long v[2] = {nchain, nsucc};
+  for (int k = 0; k < 2; k++) {
+    unsigned long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (__lane_id() == 0 && x) atomicAdd(a.counters + 4 + k, x);
+  }
+}
+
+__global__ void k_add_h(const double *__restrict__ H64, float *__restrict__ out, size_t n) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] += (float)H64[k];
+}
+
+__global__ void k_unique_flags(const uint64_t *__restrict__ hi, const uint64_t *__restrict__ lo, size_t n,
+                               unsigned char *__restrict__ flag) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) flag[k] = (k == 0) || hi[k] != hi[k - 1] || lo[k] != lo[k - 1];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+#define HIPCHK(x)                                                               \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      ams::set_error(std::string(#x) + ": " + hipGetErrorString(e_));          \
+      return AMS_ERR_HIP;                                                       \
+    }                                                                           \
+  } while (0)
+
+namespace ams {
+
+static inline float bits_f(int x) {
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, int32_t n_mat,
+                 const float *mat_prop, const float *refl_phase) {
+  s->n_tri = n_tri;
+  s->n_mat = n_mat;
+  std::vector<int> plane(n_tri > 0 ? n_tri : 1);
+  std::vector<double> pl_n(3 * (size_t)(n_tri > 0 ? n_tri : 1)), pl_d(n_tri > 0 ? n_tri : 1);
+  if (n_tri > 0) ams_plane_ids(n_tri, tri, plane.data(), pl_n.data(), pl_d.data());
+  s->plane.assign(plane.begin(), plane.begin() + n_tri);
+  std::vector<double> lo(3 * (size_t)n_tri), hi(3 * (size_t)n_tri);
+  std::vector<AmsTriAux> aux(n_tri > 0 ? n_tri : 1);
+  std::vector<AmsPlane> planes(n_tri > 0 ? n_tri : 1);
+  for (int i = 0; i < n_tri; i++) {
+    const float *t = tri + 9 * (size_t)i;
+    for (int k = 0; k < 3; k++) {
+      lo[3 * i + k] = std::min((double)t[k], std::min((double)t[3 + k], (double)t[6 + k]));
+      hi[3 * i + k] = std::max((double)t[k], std::max((double)t[3 + k], (double)t[6 + k]));
+    }
+    // FP32 normal and longest edge as the kernels of the reference compute them
+    float e1[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+    float e2[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+    float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (l > 0) {
+      c[0] /= l;
+      c[1] /= l;
+      c[2] /= l;
+    } else {
+      c[0] = 0;
+      c[1] = 0;
+      c[2] = 1;
+    }
+    auto el = [&](int a0, int b0) {
+      float dx = t[3 * a0] - t[3 * b0], dy = t[3 * a0 + 1] - t[3 * b0 + 1], dz = t[3 * a0 + 2] - t[3 * b0 + 2];
+      return std::sqrt(dx * dx + dy * dy + dz * dz);
+    };
+    float me = std::max(std::max(el(0, 1), el(1, 2)), el(2, 0));
+    aux[i].n_maxedge = make_float4(c[0], c[1], c[2], me);
+    planes[i].n[0] = pl_n[3 * i];
+    planes[i].n[1] = pl_n[3 * i + 1];
+    planes[i].n[2] = pl_n[3 * i + 2];
+    planes[i].d = pl_d[i];
+  }
+  AmsBvhBuild B;
+  ams_bvh_build(n_tri, lo.data(), hi.data(), AMS_LEAF_MAX, &B);
+  std::vector<AmsTri> tris(n_tri > 0 ? n_tri : 1);
+  for (int s2 = 0; s2 < n_tri; s2++) {
+    int i = B.order[s2];
+    const float *t = tri + 9 * (size_t)i;
+    AmsTri r;
+    r.a = make_float4(t[0], t[1], t[2], bits_f(i));
+    r.b = make_float4(t[3], t[4], t[5], bits_f(plane[i]));
+    r.c = make_float4(t[6], t[7], t[8], bits_f(mat[i]));
+    tris[s2] = r;
+  }
+  if (B.nodes.empty()) {
+    AmsNode n;  // an empty tree: one leaf that no ray enters
+    for (int k = 0; k < 3; k++) {
+      n.lo[k] = 1.0f;
+      n.hi[k] = -1.0f;
+    }
+    n.miss = 1;
+    n.info = 0;
+    B.nodes.push_back(n);
+  }
+  s->n_tnodes = (int)B.nodes.size();
+  std::vector<float> mp(19 * (size_t)std::max(n_mat, 1), 0.0f), rp(std::max(n_mat, 1), 0.0f);
+  for (int k = 0; k < 19 * n_mat; k++) mp[k] = mat_prop[k];
+  for (int k = 0; k < n_mat; k++) rp[k] = refl_phase ? refl_phase[k] : 0.0f;
+  if (dev_upload(&s->d_tnodes, B.nodes.data(), sizeof(AmsNode) * B.nodes.size()) ||
+      dev_upload(&s->d_tris, tris.data(), sizeof(AmsTri) * tris.size()) ||
+      dev_upload(&s->d_aux, aux.data(), sizeof(AmsTriAux) * aux.size()) ||
+      dev_upload(&s->d_planes, planes.data(), sizeof(AmsPlane) * planes.size()) ||
+      dev_upload(&s->d_mat_prop, mp.data(), sizeof(float) * mp.size()) ||
+      dev_upload(&s->d_refl_phase, rp.data(), sizeof(float) * rp.size()))
+    return AMS_ERR_HIP;
+  return AMS_OK;
+}
+
+// (hi, lo) sort + unique in place on device; returns the unique count.
+static int sort_unique(Scratch &sc, uint64_t *hi, uint64_t *lo, size_t n, hipStream_t st, size_t *n_out) {
+  if (n == 0) {
+    *n_out = 0;
+    return AMS_OK;
+  }
+  uint64_t *hi2 = (uint64_t *)sc.get("su_hi2", n * 8), *lo2 = (uint64_t *)sc.get("su_lo2", n * 8);
+  unsigned char *flag = (unsigned char *)sc.get("su_flag", n);
+  size_t *cnt = (size_t *)sc.get("su_cnt", 16);
+  if (!hi2 || !lo2 || !flag || !cnt) return AMS_ERR_ALLOC;
+  size_t tb = 0, tb2 = 0, tb3 = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lo, lo2, hi, hi2, (int)n, 0, 64, st));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, hi2, hi, lo2, lo, (int)n, 0, 64, st));
+  HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tb3, hi2, flag, hi, cnt, (int)n, st));
+  size_t tbm = std::max(tb, std::max(tb2, tb3));
+  void *tmp = sc.get("su_tmp", tbm);
+  if (!tmp) return AMS_ERR_ALLOC;
+  // pass 1: by lo (values hi); pass 2: stable by hi (values lo)
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lo, lo2, hi, hi2, (int)n, 0, 64, st));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb2, hi2, hi, lo2, lo, (int)n, 0, 64, st));
+  k_unique_flags<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(hi, lo, n, flag);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipcub::DeviceSelect::Flagged(tmp, tb3, hi, flag, hi2, cnt, (int)n, st));
+  HIPCHK(hipcub::DeviceSelect::Flagged(tmp, tb3, lo, flag, lo2, cnt + 1, (int)n, st));
+  size_t hc[2];
+  HIPCHK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpyAsync(hi, hi2, hc[0] * 8, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(lo, lo2, hc[0] * 8, hipMemcpyDeviceToDevice, st));
+  *n_out = hc[0];
+  return AMS_OK;
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const ams_trace_opts *o, float *H_out,
+                  ams_trace_stats *stats) {
+  auto t_start = std::chrono::steady_clock::now();
+  hipStream_t st = (hipStream_t)o->stream;
+  const int n_src = tx->n0 * tx->n1, n_dst = rx->n0 * rx->n1;
+  const int gs = o->gridsize, K = o->max_reflection;
+  const int n1g = (rx->n1 - 1) / gs + 1, n0g = (rx->n0 - 1) / gs + 1;
+  const long long n_grid = (long long)n0g * n1g;
+  Scratch &sc = s->scratch;
+  ams_trace_stats S;
+  memset(&S, 0, sizeof S);
+
+  // Tx / Rx positions and the Rx sphere hierarchy
+  float *d_src = (float *)sc.get("src", sizeof(float) * 3 * n_src);
+  float *d_dst = (float *)sc.get("dst", sizeof(float) * 3 * n_dst);
+  if (!d_src || !d_dst) return AMS_ERR_ALLOC;
+  HIPCHK(hipMemcpyAsync(d_src, tx->pos, sizeof(float) * 3 * n_src, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_dst, rx->pos, sizeof(float) * 3 * n_dst, hipMemcpyHostToDevice, st));
+  const float dst_acc = std::max(rx->radius, AMS_ACC_MIN);
+  std::vector<double> slo(3 * (size_t)n_dst), shi(3 * (size_t)n_dst);
+  for (int j = 0; j < n_dst; j++)
+    for (int k = 0; k < 3; k++) {
+      slo[3 * j + k] = (double)rx->pos[3 * j + k] - dst_acc;
+      shi[3 * j + k] = (double)rx->pos[3 * j + k] + dst_acc;
+    }
+  AmsBvhBuild SB;
+  ams_bvh_build(n_dst, slo.data(), shi.data(), AMS_LEAF_MAX, &SB);
+  std::vector<float4> sph(n_dst);
+  for (int q = 0; q < n_dst; q++) {
+    int j = SB.order[q];
+    sph[q] = make_float4(rx->pos[3 * j], rx->pos[3 * j + 1], rx->pos[3 * j + 2], bits_f(j));
+  }
+  AmsNode *d_snodes = (AmsNode *)sc.get("snodes", sizeof(AmsNode) * SB.nodes.size());
+  float4 *d_sph = (float4 *)sc.get("sph", sizeof(float4) * sph.size());
+  unsigned long long *d_ctr = (unsigned long long *)sc.get("ctr", sizeof(unsigned long long) * 16);
+  if (!d_snodes || !d_sph || !d_ctr) return AMS_ERR_ALLOC;
+  HIPCHK(hipMemcpyAsync(d_snodes, SB.nodes.data(), sizeof(AmsNode) * SB.nodes.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_sph, sph.data(), sizeof(float4) * sph.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
+
+  // ---------------- forward discovery in chunks of rays -----------------
+  const long long n_rays = o->per_element ? (long long)n_src * o->fib_n : o->fib_n;
+  const long long chunk = std::min<long long>(n_rays, 1LL << 22);
+  const unsigned long long cap = (unsigned long long)chunk * 6 + 1024;
+  uint64_t *c_hi = (uint64_t *)sc.get("c_hi", cap * 8), *c_lo = (uint64_t *)sc.get("c_lo", cap * 8);
+  unsigned long long *d_cnt = (unsigned long long *)sc.get("c_cnt", 8);
+  if (!c_hi || !c_lo || !d_cnt) return AMS_ERR_ALLOC;
+  std::vector<uint64_t> acc_hi, acc_lo;  // running unique set (host)
+  DiscoverArgs da;
+  da.tnodes = s->d_tnodes;
+  da.n_tnodes = s->n_tnodes;
+  da.tris = s->d_tris;
+  da.aux = s->d_aux;
+  da.mat_prop = s->d_mat_prop;
+  da.snodes = d_snodes;
+  da.n_snodes = (int)SB.nodes.size();
+  da.sph = d_sph;
+  da.src = d_src;
+  da.n_src = n_src;
+  da.n1_dst = rx->n1;
+  da.gs = gs;
+  da.acc2 = dst_acc * dst_acc;
+  da.r2 = rx->radius * rx->radius;
+  da.K = K;
+  da.fib_n = o->fib_n;
+  da.per_element = o->per_element;
+  da.out_hi = c_hi;
+  da.out_lo = c_lo;
+  da.out_count = d_cnt;
+  da.cap = cap;
+  da.counters = d_ctr;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float disc_ms = 0.0f;
+  size_t n_unique_total = 0;
+  std::vector<uint64_t> uh, ul;
+  if (o->seed_keys) {
+    uh.assign(o->n_seed, 0);
+    ul.assign(o->n_seed, 0);
+    for (long long k = 0; k < o->n_seed; k++) {
+      uh[k] = o->seed_keys[2 * k];
+      ul[k] = o->seed_keys[2 * k + 1];
+      if ((ul[k] & 0xFFFFF) >= (uint64_t)n_grid) {
+        set_error("seed key tile out of range");
+        return AMS_ERR_ARG;
+      }
+      int h = (int)(uh[k] >> 61);
+      if (h > K) {
+        set_error("seed key hit count exceeds max_reflection");
+        return AMS_ERR_ARG;
+      }
+      uint32_t pl[5], tile;
+      int hh, bb;
+      ams_key_unpack(uh[k], ul[k], &hh, &bb, pl, &tile);
+      int nref = __builtin_popcount((unsigned)bb & ((1u << hh) - 1u));
+      for (int q = 0; q < nref; q++) {
+        if (pl[q] >= (uint32_t)s->n_tri) {
+          set_error("seed key: a reflection has no valid plane id");
+          return AMS_ERR_ARG;
+        }
+      }
+    }
+  }
+  for (long long rb = o->seed_keys ? n_rays : 0; rb < n_rays;) {
+    long long re = std::min(n_rays, rb + chunk);
+    HIPCHK(hipMemsetAsync(d_cnt, 0, 8, st));
+    da.ray_begin = rb;
+    da.ray_end = re;
+    unsigned blocks = (unsigned)((re - rb + 255) / 256);
+    HIPCHK(hipEventRecord(e0, st));
+    k_discover<<<blocks, 256, 0, st>>>(da);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, st));
+    unsigned long long hcnt = 0;
+    HIPCHK(hipMemcpyAsync(&hcnt, d_cnt, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    disc_ms += ms;
+    if (hcnt > cap) {
+      set_error("forward records exceed the per-chunk buffer");
+      S.overflow += (long long)(hcnt - cap);
+      return AMS_ERR_OVERFLOW;
+    }
+    size_t nu = 0;
+    int rc = sort_unique(sc, c_hi, c_lo, (size_t)hcnt, st, &nu);
+    if (rc) return rc;
+    size_t old = uh.size();
+    uh.resize(old + nu);
+    ul.resize(old + nu);
+    HIPCHK(hipMemcpyAsync(uh.data() + old, c_hi, nu * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ul.data() + old, c_lo, nu * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    S.records_raw += (long long)hcnt;
+    rb = re;
+  }
+  // merge chunk-unique sets on the host: sort (hi, lo) and unique
+  {
+    std::vector<size_t> ix(uh.size());
+    for (size_t k = 0; k < ix.size(); k++) ix[k] = k;
+    std::sort(ix.begin(), ix.end(), [&](size_t a0, size_t b0) {
+      return uh[a0] != uh[b0] ? uh[a0] < uh[b0] : ul[a0] < ul[b0];
+    });
+    acc_hi.reserve(ix.size());
+    acc_lo.reserve(ix.size());
+    for (size_t k : ix) {
+      if (!acc_hi.empty() && acc_hi.back() == uh[k] && acc_lo.back() == ul[k]) continue;
+      acc_hi.push_back(uh[k]);
+      acc_lo.push_back(ul[k]);
+    }
+    n_unique_total = acc_hi.size();
+  }
+  S.rays = n_rays;
+  S.records_unique = (long long)n_unique_total;
+  S.t_discover_ms = disc_ms;
+  unsigned long long hctr[16];
+  HIPCHK(hipMemcpyAsync(hctr, d_ctr, sizeof hctr, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  S.fwd_nodes = (long long)hctr[0];
+  S.fwd_tris = (long long)hctr[1];
+  S.fwd_spheres = (long long)hctr[2];
+  S.fwd_queries = (long long)hctr[3];
+
+  // ---------------- path list and round-1 tasks (host) -------------------
+  std::vector<uint64_t> p_hi, p_lo;
+  std::vector<int> rec_path(n_unique_total);
+  for (size_t k = 0; k < n_unique_total; k++) {
+    uint64_t h = acc_hi[k], l = acc_lo[k] & ~0xFFFFFull;
+    if (p_hi.empty() || p_hi.back() != h || p_lo.back() != l) {
+      p_hi.push_back(h);
+      p_lo.push_back(l);
+    }
+    rec_path[k] = (int)p_hi.size() - 1;
+  }
+  const int n_paths = (int)p_hi.size();
+  S.paths = n_paths;
+  double *d_H = (double *)sc.get("H64", sizeof(double) * 2 * (size_t)n_src * n_dst);
+  if (!d_H) return AMS_ERR_ALLOC;
+  HIPCHK(hipMemsetAsync(d_H, 0, sizeof(double) * 2 * (size_t)n_src * n_dst, st));
+  HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
+  if (n_paths > 0) {
+    uint64_t *d_phi = (uint64_t *)sc.get("p_hi", 8 * (size_t)n_paths), *d_plo = (uint64_t *)sc.get("p_lo", 8 * (size_t)n_paths);
+    if (!d_phi || !d_plo) return AMS_ERR_ALLOC;
+    HIPCHK(hipMemcpyAsync(d_phi, p_hi.data(), 8 * (size_t)n_paths, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_plo, p_lo.data(), 8 * (size_t)n_paths, hipMemcpyHostToDevice, st));
+    // tasks grouped per tile (stable), optixPathTracer.cpp:287-330
+    std::vector<int> tp, tt;
+    {
+      std::vector<long long> cnt(n_grid + 1, 0);
+      for (size_t k = 0; k < n_unique_total; k++) cnt[(acc_lo[k] & 0xFFFFF) + 1]++;
+      for (long long g = 0; g < n_grid; g++) cnt[g + 1] += cnt[g];
+      tp.resize(n_unique_total);
+      tt.resize(n_unique_total);
+      for (size_t k = 0; k < n_unique_total; k++) {
+        long long g = (long long)(acc_lo[k] & 0xFFFFF);
+        long long pos = cnt[g]++;
+        tp[pos] = rec_path[k];
+        tt[pos] = (int)g;
+      }
+    }
+    std::unordered_set<uint64_t> traced;
+    traced.reserve(tp.size() * 4 + 16);
+    for (size_t k = 0; k < tp.size(); k++) traced.insert(((uint64_t)tp[k] << 24) | (uint64_t)tt[k]);
+
+    EvalArgs ea;
+    ea.tnodes = s->d_tnodes;
+    ea.n_tnodes = s->n_tnodes;
+    ea.tris = s->d_tris;
+    ea.aux = s->d_aux;
+    ea.mat_prop = s->d_mat_prop;
+    ea.refl_phase = s->d_refl_phase;
+    ea.planes = s->d_planes;
+    ea.src = d_src;
+    ea.n_src = n_src;
+    ea.dst = d_dst;
+    ea.n0_dst = rx->n0;
+    ea.n1_dst = rx->n1;
+    ea.n_dst = n_dst;
+    ea.gs = gs;
+    ea.n1g = n1g;
+    ea.flags = o->flags;
+    ea.wave_len_d = o->wave_len;
+    ea.wave_len = (float)o->wave_len;
+    ea.src_r2 = tx->radius * tx->radius;
+    float sacc = std::max(tx->radius, AMS_ACC_MIN);
+    ea.src_acc2 = sacc * sacc;
+    ea.src_dir = make_float3(tx->dir[0], tx->dir[1], tx->dir[2]);
+    ea.dst_dir = make_float3(rx->dir[0], rx->dir[1], rx->dir[2]);
+    ea.src_bptype = tx->bptype;
+    ea.dst_bptype = rx->bptype;
+    ea.path_hi = d_phi;
+    ea.path_lo = d_plo;
+    ea.items = gs * gs * n_src;
+    ea.H = d_H;
+    ea.counters = d_ctr;
+    const int bpt = (ea.items + 511) / 512;                       // blocks per tile slice
+    const int block = ((ea.items + bpt - 1) / bpt + 63) / 64 * 64;  // multiple of the wave size
+    const int slice = o->tasks_per_unit > 0 ? o->tasks_per_unit : 16;
+    float eval_ms = 0.0f;
+    while (!tp.empty()) {
+      S.rounds++;
+      size_t nt = tp.size();
+      S.tasks += (long long)nt;
+      // work units: (tile, slice of tasks)
+      std::vector<int> ut, ub, ue;
+      for (size_t k = 0; k < nt;) {
+        size_t e = k;
+        while (e < nt && tt[e] == tt[k] && e - k < (size_t)slice) e++;
+        ut.push_back(tt[k]);
+        ub.push_back((int)k);
+        ue.push_back((int)e);
+        k = e;
+      }
+      int nu = (int)ut.size();
+      int *d_tp = (int *)sc.get("t_path", 4 * nt);
+      unsigned char *d_ok = (unsigned char *)sc.get("t_ok", nt);
+      int *d_u = (int *)sc.get("units", 12 * (size_t)nu);
+      if (!d_tp || !d_ok || !d_u) return AMS_ERR_ALLOC;
+      HIPCHK(hipMemcpyAsync(d_tp, tp.data(), 4 * nt, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(d_u, ut.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(d_u + nu, ub.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(d_u + 2 * nu, ue.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemsetAsync(d_ok, 0, nt, st));
+      ea.task_path = d_tp;
+      ea.unit_tile = d_u;
+      ea.unit_begin = d_u + nu;
+      ea.unit_end = d_u + 2 * nu;
+      ea.task_ok = d_ok;
+      HIPCHK(hipEventRecord(e0, st));
+      for (int u0 = 0; u0 < nu; u0 += 65535) {
+        int un = std::min(65535, nu - u0);
+        EvalArgs eb = ea;
+        eb.unit_tile += u0;
+        eb.unit_begin += u0;
+        eb.unit_end += u0;
+        k_evaluate<<<dim3(bpt, un), block, 0, st>>>(eb);
+        HIPCHK(hipGetLastError());
+      }
+      HIPCHK(hipEventRecord(e1, st));
+      std::vector<unsigned char> ok(nt);
+      HIPCHK(hipMemcpyAsync(ok.data(), d_ok, nt, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      eval_ms += ms;
+      // neighbour expansion, optixPathTracer.cpp:264-284 (linear bounds only)
+      std::vector<int> np_, nt_;
+      for (size_t k = 0; k < nt; k++) {
+        if (!ok[k]) continue;
+        long long g = tt[k];
+        long long nb[4] = {g - 1, g + 1, g - n1g, g + n1g};
+        for (int q = 0; q < 4; q++) {
+          if (nb[q] < 0 || nb[q] >= n_grid) continue;
+          if (traced.insert(((uint64_t)tp[k] << 24) | (uint64_t)nb[q]).second) {
+            np_.push_back(tp[k]);
+            nt_.push_back((int)nb[q]);
+          }
+        }
+      }
+      if (o->flags & 2) {
+        np_.clear();
+        nt_.clear();
+      }
+      std::vector<long long> cnt(n_grid + 1, 0);
+      for (size_t k = 0; k < nt_.size(); k++) cnt[nt_[k] + 1]++;
+      for (long long g = 0; g < n_grid; g++) cnt[g + 1] += cnt[g];
+      tp.assign(np_.size(), 0);
+      tt.assign(np_.size(), 0);
+      for (size_t k = 0; k < np_.size(); k++) {
+        long long pos = cnt[nt_[k]]++;
+        tp[pos] = np_[k];
+        tt[pos] = nt_[k];
+      }
+    }
+    S.t_evaluate_ms = eval_ms;
+  }
+  // accumulate into the caller's FP32 result, optixPathTracer.cu:259-264
+  size_t nH = 2 * (size_t)n_src * n_dst;
+  if (nH > 0) {
+    k_add_h<<<(unsigned)((nH + 255) / 256), 256, 0, st>>>(d_H, H_out, nH);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(hctr, d_ctr, sizeof hctr, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  S.bwd_nodes = (long long)hctr[0];
+  S.bwd_tris = (long long)hctr[1];
+  S.bwd_queries = (long long)hctr[3];
+  S.chains = (long long)hctr[4];
+  S.successes = (long long)hctr[5];
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  S.t_total_ms = ms_since(t_start);
+  if (stats) *stats = S;
+  if (o->keys_out && o->keys_cap > 0) {
+    size_t c = std::min<size_t>((size_t)o->keys_cap, n_unique_total);
+    for (size_t k = 0; k < c; k++) {
+      o->keys_out[2 * k] = acc_hi[k];
+      o->keys_out[2 * k + 1] = acc_lo[k];
+    }
+  }
+  return AMS_OK;
+}
+
+}  // namespace ams

@@ -1,0 +1,119 @@
+"""Host API of the channel tracer (the counterpart of go.py's build_channel /
+build_channels and of the torch op torch.ops.pathtrace.torch_launch_pathtrace,
+optixPathTracer.cpp:781-870).  Compute runs in libams.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .scenes import ArraySpec, Scene, Workload
+
+
+class TraceScene:
+    """A scene handle: triangles, materials and the BVH built once on the
+    host and kept in device memory across traces."""
+
+    def __init__(self, scene: Scene):
+        L = _lib.lib()
+        self.scene = scene
+        tri = np.ascontiguousarray(scene.tri, dtype=np.float32).reshape(-1)
+        mat = np.ascontiguousarray(scene.mat, dtype=np.int32)
+        mp = np.ascontiguousarray(scene.mat_prop, dtype=np.float32).reshape(-1)
+        rp = None if scene.refl_phase is None else np.ascontiguousarray(scene.refl_phase, dtype=np.float32)
+        h = C.c_void_p()
+        fp = C.POINTER(C.c_float)
+        _lib.check(L.ams_scene_create(
+            tri.ctypes.data_as(fp), int(scene.n_tri), mat.ctypes.data_as(C.POINTER(C.c_int32)),
+            int(scene.mat_prop.shape[0]), mp.ctypes.data_as(fp),
+            rp.ctypes.data_as(fp) if rp is not None else None, C.byref(h)), "ams_scene_create")
+        self.handle = h
+
+    def plane_ids(self) -> np.ndarray:
+        out = np.zeros(self.scene.n_tri, np.int32)
+        _lib.check(_lib.lib().ams_scene_plane_ids(self.handle, out.ctypes.data_as(C.POINTER(C.c_int32))),
+                   "ams_scene_plane_ids")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().ams_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _array(spec: ArraySpec, keep: list) -> _lib.AmsArray:
+    pos = np.ascontiguousarray(spec.pos, dtype=np.float32)
+    keep.append(pos)
+    a = _lib.AmsArray()
+    a.n0, a.n1 = int(pos.shape[0]), int(pos.shape[1])
+    a.pos = pos.ctypes.data_as(C.POINTER(C.c_float))
+    d = np.asarray(spec.dir, dtype=np.float32).reshape(3)
+    a.dir[0], a.dir[1], a.dir[2] = float(d[0]), float(d[1]), float(d[2])
+    a.bptype = int(spec.bptype)
+    a.radius = float(spec.radius)
+    return a
+
+
+def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_reflection: int, fib_n: int,
+          per_element: bool = True, gridsize: int = 5, flags: int = 0, result: Optional[torch.Tensor] = None,
+          keys_cap: int = 0, tasks_per_unit: int = 0, seed_keys=None, device: str = "cuda"):
+    """Trace Tx -> Rx and accumulate into result (f32 [n_tx, n_rx, 2] on the
+    GPU, zero-initialised if None).  Returns (result, stats dict, keys or None)."""
+    if result is None:
+        result = torch.zeros(tx.n, rx.n, 2, dtype=torch.float32, device=device)
+    if result.dtype != torch.float32 or not result.is_contiguous() or not result.is_cuda or \
+            tuple(result.shape) != (tx.n, rx.n, 2):
+        raise ValueError("result must be a contiguous f32 CUDA tensor [n_tx, n_rx, 2]")
+    keep: list = []
+    a_tx, a_rx = _array(tx, keep), _array(rx, keep)
+    o = _lib.AmsTraceOpts()
+    o.wave_len = float(wave_len)
+    o.max_reflection = int(max_reflection)
+    o.gridsize = int(gridsize)
+    o.fib_n = int(fib_n)
+    o.per_element = 1 if per_element else 0
+    o.flags = int(flags)
+    o.tasks_per_unit = int(tasks_per_unit)
+    o.stream = C.c_void_p(torch.cuda.current_stream(result.device).cuda_stream)
+    keys = None
+    if keys_cap > 0:
+        keys = np.zeros((keys_cap, 2), dtype=np.uint64)
+        o.keys_out = keys.ctypes.data_as(C.POINTER(C.c_uint64))
+        o.keys_cap = int(keys_cap)
+    if seed_keys is not None:
+        seed = np.ascontiguousarray(seed_keys, dtype=np.uint64).reshape(-1, 2)
+        keep.append(seed)
+        o.seed_keys = seed.ctypes.data_as(C.POINTER(C.c_uint64))
+        o.n_seed = int(seed.shape[0])
+    st = _lib.AmsTraceStats()
+    _lib.check(_lib.lib().ams_trace_channel(ts.handle, C.byref(a_tx), C.byref(a_rx), C.byref(o),
+                                            C.c_void_p(result.data_ptr()), C.byref(st)), "ams_trace_channel")
+    stats = st.as_dict()
+    if keys is not None:
+        keys = keys[: min(keys_cap, stats["records_unique"])]
+    return result, stats, keys
+
+
+def trace_workload(w: Workload, ts: Optional[TraceScene] = None, **kw):
+    ts = ts or TraceScene(w.scene)
+    res, stats, keys = trace(ts, w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
+                             w.gridsize, **kw)
+    return torch.complex(res[..., 0], res[..., 1]), stats, keys
+
+
+def coverage_db(H: torch.Tensor, floor_db: float = -150.0) -> torch.Tensor:
+    """Coverage map 20 log10 |sum_tx H| with the floor of go.py:849-851, :944-964."""
+    amp = H.sum(0).abs()
+    db = 20 * torch.log10(amp)
+    db = torch.where(torch.isnan(db), torch.full_like(db, floor_db), db)
+    return db.clamp(min=floor_db)

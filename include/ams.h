@@ -1,0 +1,87 @@
+/*
+ * ams.h -- C ABI of the MI355X (gfx950) AutoMS channel tracer and phase-map
+ * contraction.  Plain pointers and sizes only; no torch types.
+ *
+ * Every entry point returns AMS_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is returned by ams_last_error().
+ * The reference catches and prints every error inside its op and returns
+ * normally (optixPathTracer.cpp:858-860); these entry points report instead.
+ */
+#ifndef AMS_H_
+#define AMS_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMS_OK 0
+#define AMS_ERR_ARG (-1)
+#define AMS_ERR_HIP (-2)
+#define AMS_ERR_ALLOC (-3)
+#define AMS_ERR_OVERFLOW (-4)
+
+typedef struct ams_scene ams_scene;
+
+/* A Tx or Rx element array.  Replaces the (src|dst, *_aabb_buffer, *_dir,
+ * *_bptype, *_r) arguments of torch_launch_pathtrace
+ * (optixPathTracer.cpp:781-791; layout [n0, n1, 3], n = n0*n1 at :814-821). */
+typedef struct {
+  int32_t n0, n1;   /* grid shape; element k = (k / n1, k % n1) */
+  const float *pos; /* HOST pointer, [n0*n1][3] f32 */
+  float dir[3];     /* pointing direction for the beam pattern */
+  int32_t bptype;   /* beam pattern 0..3, optixPathTracer.cu:107-136 */
+  float radius;     /* reception-sphere radius (go.py:633-634 defaults) */
+} ams_array;
+
+/* Trace options (the scalar arguments of torch_launch_pathtrace). */
+typedef struct {
+  double wave_len;        /* wave_len */
+  int32_t max_reflection; /* max_reflection, 1..5 (MAX_HIT_N) */
+  int32_t gridsize;       /* dst_girdsize: Rx tile edge */
+  int64_t fib_n;          /* size of the Fibonacci direction set */
+  int32_t per_element;    /* 1: fib_n rays from every Tx element; 0: fib_n rays in total,
+                             ray r leaving element r % n_tx */
+  int32_t flags;          /* bit 0: keep the reference's row wrap of the last tile column;
+                             bit 1: evaluate the round-1 tasks only (no neighbour expansion) */
+  int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
+  void *stream;           /* hipStream_t; NULL = default stream */
+  uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */
+  int64_t keys_cap;
+  const uint64_t *seed_keys; /* optional HOST [n_seed][2] (path, tile) keys used instead of
+                                forward discovery */
+  int64_t n_seed;
+} ams_trace_opts;
+
+typedef struct {
+  int64_t rays, records_raw, records_unique, paths, tasks, rounds, chains, successes, overflow;
+  int64_t fwd_queries, fwd_nodes, fwd_tris, fwd_spheres;
+  int64_t bwd_queries, bwd_nodes, bwd_tris;
+  double t_discover_ms, t_evaluate_ms, t_total_ms;
+} ams_trace_stats;
+
+/* Scene: scene_triangle_buffer [3T,3] f32, mat [T] i32, mat_prop [n_mat,19]
+ * f32 (optixPathTracer.cpp:808-813; go.py:679-737), all HOST pointers.
+ * refl_phase [n_mat] (rad) may be NULL (reference: no reflection phase).
+ * Builds the BVH once; the handle is reused across traces. */
+int ams_scene_create(const float *tri, int32_t n_tri, const int32_t *mat, int32_t n_mat,
+                     const float *mat_prop, const float *refl_phase, ams_scene **out);
+int ams_scene_destroy(ams_scene *scene);
+/* plane id of every triangle (smallest index of a coplanar triangle) */
+int ams_scene_plane_ids(const ams_scene *scene, int32_t *out);
+
+/* Replaces torch_launch_pathtrace (optixPathTracer.cpp:781-861): traces
+ * Tx -> Rx and ACCUMULATES H = sum A e^{-j phi} into H_out, a DEVICE f32
+ * buffer [n_tx][n_rx][2] (go.py:671-677, optixPathTracer.cu:259-264). */
+int ams_trace_channel(ams_scene *scene, const ams_array *tx, const ams_array *rx,
+                      const ams_trace_opts *opts, float *H_out, ams_trace_stats *stats);
+
+const char *ams_last_error(void);
+int ams_version(void);
+/* device bytes currently held by the library (cudaMemManager.h:75-81) */
+int64_t ams_mem_outstanding(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMS_H_ */
