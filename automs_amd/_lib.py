@@ -44,6 +44,7 @@ class AmsTraceOpts(C.Structure):
 STAT_INT_FIELDS = [
     "rays", "records_raw", "records_unique", "paths", "tasks", "rounds", "chains", "successes", "overflow",
     "fwd_queries", "fwd_nodes", "fwd_tris", "fwd_spheres", "bwd_queries", "bwd_nodes", "bwd_tris",
+    "n_discover_launches", "n_evaluate_launches",
 ]
 STAT_DBL_FIELDS = ["t_discover_ms", "t_evaluate_ms", "t_total_ms"]
 
@@ -89,20 +90,23 @@ def lib() -> C.CDLL:
 
 
 def _bind_contract(L):
-    """Declare the contraction entry points when the library exports them."""
-    vp = C.c_void_p
+    """Contraction, objective and Adam entry points (pointers as void*)."""
+    vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
     sigs = {
-        "ams_contract_fwd": [vp, vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp],
-        "ams_contract_bwd": [vp, vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp],
-        "ams_objective_fwd": [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, vp, vp, vp, vp],
-        "ams_objective_bwd": [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, vp, vp, vp, vp],
-        "ams_adam_step": [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, vp],
+        "ams_frag_bytes": ([i32, i32], i64),
+        "ams_frag_pack": ([vp, i32, i32, i64, i64, vp, vp], C.c_int),
+        "ams_cgemm_workspace_bytes": ([i32, i32, i32], i64),
+        "ams_cgemm_frag": ([vp, vp, i32, i32, i32, i32, vp, i64, i32, vp, vp], C.c_int),
+        "ams_cgemm_f32": ([vp, i64, vp, i64, i32, i32, i32, i32, vp, i64, vp], C.c_int),
+        "ams_objective_workspace_bytes": ([i32, i32], i64),
+        "ams_objective_fwd": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp], C.c_int),
+        "ams_objective_bwd": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp, vp], C.c_int),
+        "ams_adam_step": ([vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp], C.c_int),
     }
-    for name, args in sigs.items():
-        f = getattr(L, name, None)
-        if f is not None:
-            f.argtypes = args
-            f.restype = C.c_int
+    for name, (args, res) in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
 
 
 def check(rc: int, what: str) -> None:
@@ -114,4 +118,6 @@ def check(rc: int, what: str) -> None:
 EXPORTED = [
     "ams_scene_create", "ams_scene_destroy", "ams_scene_plane_ids", "ams_trace_channel",
     "ams_last_error", "ams_version", "ams_mem_outstanding",
+    "ams_frag_bytes", "ams_frag_pack", "ams_cgemm_workspace_bytes", "ams_cgemm_frag", "ams_cgemm_f32",
+    "ams_objective_workspace_bytes", "ams_objective_fwd", "ams_objective_bwd", "ams_adam_step",
 ]

@@ -1,0 +1,341 @@
+// ams_contract.hip -- phase-map contraction on gfx950 MFMA.
+//
+// The reference forms H = H_d + sum G diag(u) F and y = W H with torch
+// complex matmuls (go.py:107-164).  Here both products become ONE complex
+// GEMM  Y = [ (W G) diag(u) | W ] . [ F ; H_d ]  whose large operand is
+// stored once in a split-precision bf16 layout (x = hi + lo, each bf16) in
+// the lane order of v_mfma_f32_16x16x32_bf16; each real product is
+// hi*hi + hi*lo + lo*hi with FP32 accumulation (relative error ~1e-5).
+//
+// Fragment layout of a complex matrix X[R][Kd] (row r, reduction index k):
+//   frag[rb][kb][plane][lane][8]  (bf16), rb = r/16, kb = k/32,
+//   lane l holds X[rb*16 + (l&15)][kb*32 + 8*(l>>4) + e], e = 0..7,
+//   planes: 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo.  One (rb, kb) block = 4 KB.
+// The same layout serves the A operand (rows = output rows) and the B
+// operand (rows = output columns), so C = A . B^T with both operands
+// contiguous along the reduction index.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "../../include/ams.h"
+#include "ams_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+// One thread per (rb, kb, lane): 8 complex values -> 4 planes of 8 bf16.
+__global__ void __launch_bounds__(256) k_frag_pack(const float *__restrict__ X, int R, int Kd, long long sr,
+                                                   long long sk, int n_kb, long long n_items,
+                                                   u32x4 *__restrict__ frag) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_items) return;
+  int lane = (int)(t & 63);
+  long long blk = t >> 6;
+  int kb = (int)(blk % n_kb);
+  long long rb = blk / n_kb;
+  long long r = rb * 16 + (lane & 15);
+  int k0 = kb * 32 + 8 * (lane >> 4);
+  unsigned short p[4][8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    float re = 0.0f, im = 0.0f;
+    int k = k0 + e;
+    if (r < R && k < Kd) {
+      const float *src = X + 2 * (r * sr + (long long)k * sk);
+      re = src[0];
+      im = src[1];
+    }
+    unsigned short rh = bf16_rne(re), ih = bf16_rne(im);
+    p[0][e] = rh;
+    p[1][e] = bf16_rne(re - bf16_to_f(rh));
+    p[2][e] = ih;
+    p[3][e] = bf16_rne(im - bf16_to_f(ih));
+  }
+  u32x4 *dst = frag + blk * 256 + lane;  // 4 planes x 64 lanes of 16 B
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    u32x4 v;
+    v.x = p[q][0] | ((unsigned)p[q][1] << 16);
+    v.y = p[q][2] | ((unsigned)p[q][3] << 16);
+    v.z = p[q][4] | ((unsigned)p[q][5] << 16);
+    v.w = p[q][6] | ((unsigned)p[q][7] << 16);
+    dst[q * 64] = v;
+  }
+}
+
+__device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ __forceinline__ bf16x8 neg_bf(u32x4 v) {
+  u32x4 s = v ^ (u32x4){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  return __builtin_bit_cast(bf16x8, s);
+}
+#define MFMA(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
+
+// C[Ra][Rb] (complex f32) = A . B^T  (CONJB: A . conj(B)^T) over reduction
+// blocks [kb0, kb1).  One wave owns MT x NT tiles of 16 x 16; waves of a
+// workgroup share the column group (so B blocks are reused through L1/L2).
+// With splits > 1 (gridDim.z), partial sums go to part[z][Ra][Rb].
+template <int MT, int NT, bool CONJB>
+__global__ void __launch_bounds__(256) k_cgemm(const u32x4 *__restrict__ A, const u32x4 *__restrict__ B, int Ra,
+                                               int Rb, int n_kb, int m_groups, int n_groups,
+                                               float *__restrict__ C, long long ldc, float *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int mg = gw % m_groups, ng = gw / m_groups;
+  if (ng >= n_groups) return;
+  const int splits = gridDim.z;
+  const int kb0 = (int)((long long)n_kb * blockIdx.z / splits);
+  const int kb1 = (int)((long long)n_kb * (blockIdx.z + 1) / splits);
+  const int ra_blocks = (Ra + 15) >> 4, rb_blocks = (Rb + 15) >> 4;
+  f32x4 acc_re[MT][NT], acc_im[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; i++)
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+      acc_re[i][j] = (f32x4){0, 0, 0, 0};
+      acc_im[i][j] = (f32x4){0, 0, 0, 0};
+    }
+  for (int kb = kb0; kb < kb1; kb++) {
+    u32x4 a[MT][4], b[NT][4];
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+      int rb = mg * MT + i;
+      rb = rb < ra_blocks ? rb : ra_blocks - 1;  // clamped rows are never stored
+      const u32x4 *p = A + ((long long)rb * n_kb + kb) * 256 + lane;
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[i][q] = p[q * 64];
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+      int cb = ng * NT + j;
+      cb = cb < rb_blocks ? cb : rb_blocks - 1;
+      const u32x4 *p = B + ((long long)cb * n_kb + kb) * 256 + lane;
+#pragma unroll
+      for (int q = 0; q < 4; q++) b[j][q] = p[q * 64];
+    }
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+      bf16x8 arh = as_bf(a[i][0]), arl = as_bf(a[i][1]), aih = as_bf(a[i][2]), ail = as_bf(a[i][3]);
+      bf16x8 narh = neg_bf(a[i][0]), narl = neg_bf(a[i][1]);
+      bf16x8 naih = neg_bf(a[i][2]), nail = neg_bf(a[i][3]);
+#pragma unroll
+      for (int j = 0; j < NT; j++) {
+        bf16x8 brh = as_bf(b[j][0]), brl = as_bf(b[j][1]), bih = as_bf(b[j][2]), bil = as_bf(b[j][3]);
+        f32x4 re = acc_re[i][j], im = acc_im[i][j];
+        // real part: ar*br -/+ ai*bi
+        re = MFMA(arh, brh, re);
+        re = MFMA(arh, brl, re);
+        re = MFMA(arl, brh, re);
+        if (CONJB) {
+          re = MFMA(aih, bih, re);
+          re = MFMA(aih, bil, re);
+          re = MFMA(ail, bih, re);
+        } else {
+          re = MFMA(naih, bih, re);
+          re = MFMA(naih, bil, re);
+          re = MFMA(nail, bih, re);
+        }
+        // imaginary part: ai*br +/- ar*bi
+        im = MFMA(aih, brh, im);
+        im = MFMA(aih, brl, im);
+        im = MFMA(ail, brh, im);
+        if (CONJB) {
+          im = MFMA(narh, bih, im);
+          im = MFMA(narh, bil, im);
+          im = MFMA(narl, bih, im);
+        } else {
+          im = MFMA(arh, bih, im);
+          im = MFMA(arh, bil, im);
+          im = MFMA(arl, bih, im);
+        }
+        acc_re[i][j] = re;
+        acc_im[i][j] = im;
+      }
+    }
+  }
+  // D layout of 16x16 MFMA: lane l holds rows 4*(l>>4)+v, column l&15
+  float *out = splits > 1 ? part + (long long)blockIdx.z * 2 * (long long)Ra * Rb : C;
+  long long ld = splits > 1 ? Rb : ldc;
+#pragma unroll
+  for (int i = 0; i < MT; i++) {
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+      int col = (ng * NT + j) * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row = (mg * MT + i) * 16 + 4 * (lane >> 4) + v;
+        if (row < Ra && col < Rb) {
+          float2 val = make_float2(acc_re[i][j][v], acc_im[i][j][v]);
+          *reinterpret_cast<float2 *>(out + 2 * ((long long)row * ld + col)) = val;
+        }
+      }
+    }
+  }
+}
+
+// Ordered sum of split partials: C = sum_z part[z]  (deterministic).
+__global__ void k_reduce_parts(const float *__restrict__ part, int splits, int Ra, int Rb, float *__restrict__ C,
+                               long long ldc) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long n = (long long)Ra * Rb;
+  if (t >= n) return;
+  int row = (int)(t / Rb), col = (int)(t % Rb);
+  float re = 0.0f, im = 0.0f;
+  for (int z = 0; z < splits; z++) {
+    const float *p = part + (long long)z * 2 * n + 2 * t;
+    re += p[0];
+    im += p[1];
+  }
+  C[2 * ((long long)row * ldc + col)] = re;
+  C[2 * ((long long)row * ldc + col) + 1] = im;
+}
+
+// Plain FP32 complex GEMM C = A . B^T on the vector ALU (comparison path).
+// A [Ra][Kd] and B [Rb][Kd] complex f32 with row strides lda, ldb.
+__global__ void __launch_bounds__(256) k_cgemm_f32(const float2 *__restrict__ A, long long lda,
+                                                   const float2 *__restrict__ B, long long ldb, int Ra, int Rb,
+                                                   int Kd, int conj_b, float2 *__restrict__ C, long long ldc) {
+  __shared__ float2 sa[16][33], sb[64][33];
+  int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 columns x 4 row groups
+  int col = blockIdx.x * 64 + tx;
+  int row0 = blockIdx.y * 16;
+  float accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < Kd; k0 += 32) {
+    for (int e = threadIdx.x; e < 16 * 32; e += 256) {
+      int r = e / 32, k = e % 32;
+      sa[r][k] = (row0 + r < Ra && k0 + k < Kd) ? A[(long long)(row0 + r) * lda + k0 + k] : make_float2(0, 0);
+    }
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+      int r = e / 32, k = e % 32;
+      int cc = blockIdx.x * 64 + r;
+      float2 v = (cc < Rb && k0 + k < Kd) ? B[(long long)cc * ldb + k0 + k] : make_float2(0, 0);
+      if (conj_b) v.y = -v.y;
+      sb[r][k] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 32; k++) {
+      float2 b = sb[tx][k];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        float2 a = sa[ty * 4 + v][k];
+        accr[v] += a.x * b.x - a.y * b.y;
+        acci[v] += a.x * b.y + a.y * b.x;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    int row = row0 + ty * 4 + v;
+    if (row < Ra && col < Rb) C[(long long)row * ldc + col] = make_float2(accr[v], acci[v]);
+  }
+}
+
+template <int MT, int NT, bool CJ>
+void launch_cgemm(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, int splits, float *C, long long ldc,
+                  float *part, hipStream_t st) {
+  int m_groups = ((Ra + 15) / 16 + MT - 1) / MT;
+  int n_groups = ((Rb + 15) / 16 + NT - 1) / NT;
+  long long waves = (long long)m_groups * n_groups;
+  unsigned blocks = (unsigned)((waves + 3) / 4);
+  k_cgemm<MT, NT, CJ><<<dim3(blocks, 1, splits), 256, 0, st>>>(A, B, Ra, Rb, n_kb, m_groups, n_groups, C, ldc,
+                                                               part);
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ams_frag_bytes(int32_t R, int32_t Kd) {
+  if (R <= 0 || Kd <= 0) return 0;
+  return (int64_t)((R + 15) / 16) * ((Kd + 31) / 32) * 4096;
+}
+
+int ams_frag_pack(const float *X, int32_t R, int32_t Kd, int64_t sr, int64_t sk, void *frag, void *stream) {
+  if (!X || !frag || R <= 0 || Kd <= 0) {
+    ams::set_error("ams_frag_pack: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  int n_kb = (Kd + 31) / 32;
+  long long n_items = (long long)((R + 15) / 16) * n_kb * 64;
+  k_frag_pack<<<(unsigned)((n_items + 255) / 256), 256, 0, (hipStream_t)stream>>>(X, R, Kd, sr, sk, n_kb, n_items,
+                                                                                  (u32x4 *)frag);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_frag_pack: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int64_t ams_cgemm_workspace_bytes(int32_t Ra, int32_t Rb, int32_t splits) {
+  return splits > 1 ? (int64_t)splits * 2 * Ra * (int64_t)Rb * 4 : 0;
+}
+
+int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b, float *C,
+                   int64_t ldc, int32_t splits, void *workspace, void *stream) {
+  if (!A || !B || !C || Ra <= 0 || Rb <= 0 || Kd <= 0 || ldc < Rb || splits < 1 ||
+      (splits > 1 && !workspace)) {
+    ams::set_error("ams_cgemm_frag: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  int n_kb = (Kd + 31) / 32;
+  if (splits > n_kb) splits = n_kb;
+  hipStream_t st = (hipStream_t)stream;
+  const u32x4 *a = (const u32x4 *)A, *b = (const u32x4 *)B;
+  float *part = (float *)workspace;
+  int ra_blocks = (Ra + 15) / 16;
+  if (ra_blocks == 1) {
+    if (conj_b) launch_cgemm<1, 4, true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+    else launch_cgemm<1, 4, false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+  } else if (ra_blocks == 2) {
+    if (conj_b) launch_cgemm<2, 4, true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+    else launch_cgemm<2, 4, false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+  } else {
+    if (conj_b) launch_cgemm<4, 2, true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+    else launch_cgemm<4, 2, false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && splits > 1) {
+    long long n = (long long)Ra * Rb;
+    k_reduce_parts<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, Ra, Rb, C, ldc);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_cgemm_frag: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_cgemm_f32(const float *A, int64_t lda, const float *B, int64_t ldb, int32_t Ra, int32_t Rb, int32_t Kd,
+                  int32_t conj_b, float *C, int64_t ldc, void *stream) {
+  if (!A || !B || !C || Ra <= 0 || Rb <= 0 || Kd <= 0) {
+    ams::set_error("ams_cgemm_f32: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  dim3 grid((Rb + 63) / 64, (Ra + 15) / 16);
+  k_cgemm_f32<<<grid, 256, 0, (hipStream_t)stream>>>((const float2 *)A, lda, (const float2 *)B, ldb, Ra, Rb, Kd,
+                                                     conj_b, (float2 *)C, ldc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_cgemm_f32: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+}  // extern "C"

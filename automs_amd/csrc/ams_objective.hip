@@ -1,0 +1,246 @@
+// ams_objective.hip -- coverage objectives over received signals y[g][c][r]
+// (group g = one phase map, code c = one AP weight vector, r = Rx) and the
+// fused Adam step of the phase-map optimiser.
+//
+// mode 0 (mean log SNR):  L_g = -(1/N) sum_r log2(1 + max_c |y_gcr|^2 / N0)
+// mode 1 (reference capacity loss, go.py:166-237 with loss_mode "capacity"):
+//   cap_cr = log2(1 + (|y_cr| + eps32)^2 / N0); rows with mask[c] = 1 are
+//   zeroed (go.py:220); L = -w sum_r max_c cap' + || cap' . [cap' != max] ||_F
+// Gradients follow PyTorch's convention for complex inputs,
+// g = dL/dRe(y) + j dL/dIm(y).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/ams.h"
+#include "ams_internal.h"
+
+namespace {
+
+constexpr float kEps32 = 1.1920928955078125e-07f;  // torch.finfo(torch.float32).eps
+constexpr float kInvLn2 = 1.4426950408889634f;
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sh[k];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ float cap_of(float2 y, float n0) {
+  float a = sqrtf(y.x * y.x + y.y * y.y) + kEps32;
+  return log2f(1.0f + a * a / n0);
+}
+
+// partial[g][block] = (S1, S2): mode 0: S1 = sum log2(1+max SNR); mode 1:
+// S1 = sum max cap', S2 = sum of squares of non-max cap'.
+__global__ void __launch_bounds__(256) k_obj_fwd(const float2 *__restrict__ Y, int C, int N, float n0, int mode,
+                                                 const unsigned char *__restrict__ rowmask,
+                                                 double *__restrict__ partial, int *__restrict__ argmax) {
+  __shared__ double sh[8];
+  int g = blockIdx.y;
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const float2 *Yg = Y + (size_t)g * C * N;
+  double s1 = 0.0, s2 = 0.0;
+  if (r < N) {
+    if (mode == 0) {
+      float best = -1.0f;
+      int bi = 0;
+      for (int c = 0; c < C; c++) {
+        float2 y = Yg[(size_t)c * N + r];
+        float s = y.x * y.x + y.y * y.y;
+        if (s > best) {
+          best = s;
+          bi = c;
+        }
+      }
+      s1 = (double)log2f(1.0f + best / n0);
+      if (argmax) argmax[(size_t)g * N + r] = bi;
+    } else {
+      float best = -1.0f;
+      int bi = 0;
+      for (int c = 0; c < C; c++) {
+        float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
+        if (v > best) {
+          best = v;
+          bi = c;
+        }
+      }
+      for (int c = 0; c < C; c++) {
+        float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
+        if (v != best) s2 += (double)v * v;
+      }
+      s1 = best;
+      if (argmax) argmax[(size_t)g * N + r] = bi;
+    }
+  }
+  double t1 = block_sum(s1, sh);
+  double t2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * ((size_t)g * gridDim.x + blockIdx.x)] = t1;
+    partial[2 * ((size_t)g * gridDim.x + blockIdx.x) + 1] = t2;
+  }
+}
+
+// loss[g] from the partials, in block order (deterministic).
+__global__ void k_obj_finish(const double *__restrict__ partial, int nb, int N, int mode, float w,
+                             double *__restrict__ sums, float *__restrict__ loss) {
+  int g = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int b = 0; b < nb; b++) {
+    s1 += partial[2 * ((size_t)g * nb + b)];
+    s2 += partial[2 * ((size_t)g * nb + b) + 1];
+  }
+  sums[2 * g] = s1;
+  sums[2 * g + 1] = s2;
+  if (mode == 0) loss[g] = (float)(-s1 / N);
+  else loss[g] = (float)(-(double)w * s1 + sqrt(s2));
+}
+
+__global__ void __launch_bounds__(256) k_obj_bwd(const float2 *__restrict__ Y, int C, int N, float n0, int mode,
+                                                 float w, const unsigned char *__restrict__ rowmask,
+                                                 const int *__restrict__ argmax, const double *__restrict__ sums,
+                                                 const float *__restrict__ gloss, float2 *__restrict__ gY) {
+  int g = blockIdx.y;
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  const float2 *Yg = Y + (size_t)g * C * N;
+  float2 *Gg = gY + (size_t)g * C * N;
+  float gl = gloss[g];
+  int bi = argmax[(size_t)g * N + r];
+  if (mode == 0) {
+    for (int c = 0; c < C; c++) {
+      float2 out = make_float2(0.0f, 0.0f);
+      if (c == bi) {
+        float2 y = Yg[(size_t)c * N + r];
+        float s = y.x * y.x + y.y * y.y;
+        // dL/ds = -(1/N) / (ln2 (N0 + s)); g = 2 y dL/ds
+        float dls = -gl * kInvLn2 / ((float)N * (n0 + s));
+        out = make_float2(2.0f * dls * y.x, 2.0f * dls * y.y);
+      }
+      Gg[(size_t)c * N + r] = out;
+    }
+  } else {
+    float fro = (float)sqrt(sums[2 * g + 1]);
+    float best = -1.0f;
+    for (int c = 0; c < C; c++) {
+      float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
+      if (v > best) best = v;
+    }
+    for (int c = 0; c < C; c++) {
+      float2 out = make_float2(0.0f, 0.0f);
+      if (!(rowmask && rowmask[c])) {
+        float2 y = Yg[(size_t)c * N + r];
+        float v = cap_of(y, n0);
+        float dcap = 0.0f;
+        if (c == bi) dcap -= w;
+        if (v != best && fro > 0.0f) dcap += v / fro;
+        float amp = sqrtf(y.x * y.x + y.y * y.y);
+        float a = amp + kEps32;
+        float dca = kInvLn2 * 2.0f * a / n0 / (1.0f + a * a / n0);  // dcap / d|y|
+        float gabs = gl * dcap * dca;
+        if (amp > 0.0f) out = make_float2(gabs * y.x / amp, gabs * y.y / amp);
+      }
+      Gg[(size_t)c * N + r] = out;
+    }
+  }
+}
+
+// torch.optim.Adam (no weight decay, no amsgrad), single-tensor form:
+// m = lerp(m, g, 1-b1);  v = b2 v + (1-b2) g^2;
+// p -= (lr / (1-b1^t)) * m / (sqrt(v) / sqrt(1-b2^t) + eps)
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, long long n, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2_sqrt) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = m[i] + (1.0f - b1) * (gi - m[i]);
+  float vi = v[i] * b2 + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float denom = sqrtf(vi) / bc2_sqrt + eps;
+  p[i] = p[i] - (lr / bc1) * (mi / denom);
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ams_objective_workspace_bytes(int32_t G, int32_t N) {
+  int64_t nb = (N + 255) / 256;
+  return (int64_t)G * nb * 16 + (int64_t)G * 16 + (int64_t)G * N * 4;
+}
+
+// Y complex f32 [G][C][N]; loss f32 [G]; workspace from
+// ams_objective_workspace_bytes (keeps argmax and sums for the backward).
+int ams_objective_fwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                      const unsigned char *rowmask, float *loss, void *workspace, void *stream) {
+  if (!Y || !loss || !workspace || G <= 0 || C <= 0 || N <= 0 || (mode != 0 && mode != 1) || !(n0 > 0)) {
+    ams::set_error("ams_objective_fwd: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int nb = (N + 255) / 256;
+  double *partial = (double *)workspace;
+  double *sums = partial + 2 * (size_t)G * nb;
+  int *argmax = (int *)(sums + 2 * (size_t)G);
+  k_obj_fwd<<<dim3(nb, G), 256, 0, st>>>((const float2 *)Y, C, N, n0, mode, rowmask, partial, argmax);
+  k_obj_finish<<<G, 64, 0, st>>>(partial, nb, N, mode, w, sums, loss);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_objective_fwd: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+// gY complex f32 [G][C][N] = gloss[g] * dL_g/dY (workspace of the forward).
+int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                      const unsigned char *rowmask, const float *gloss, const void *workspace, float *gY,
+                      void *stream) {
+  if (!Y || !gY || !gloss || !workspace || G <= 0 || C <= 0 || N <= 0 || (mode != 0 && mode != 1)) {
+    ams::set_error("ams_objective_bwd: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int nb = (N + 255) / 256;
+  const double *partial = (const double *)workspace;
+  const double *sums = partial + 2 * (size_t)G * nb;
+  const int *argmax = (const int *)(sums + 2 * (size_t)G);
+  k_obj_bwd<<<dim3(nb, G), 256, 0, st>>>((const float2 *)Y, C, N, n0, mode, w, rowmask, argmax, sums, gloss,
+                                         (float2 *)gY);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_objective_bwd: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_adam_step(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
+                  float beta2, float eps, int32_t step, void *stream) {
+  if (!param || !grad || !m || !v || n <= 0 || step < 1) {
+    ams::set_error("ams_adam_step: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  double bc1 = 1.0 - std::pow((double)beta1, step);
+  double bc2 = 1.0 - std::pow((double)beta2, step);
+  k_adam<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(param, grad, m, v, n, lr, beta1, beta2, eps,
+                                                                       (float)bc1, (float)std::sqrt(bc2));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_adam_step: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+}  // extern "C"

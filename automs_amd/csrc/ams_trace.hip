@@ -832,6 +832,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     HIPCHK(hipEventRecord(e0, st));
     k_discover<<<blocks, 256, 0, st>>>(da);
     HIPCHK(hipGetLastError());
+    S.n_discover_launches++;
     HIPCHK(hipEventRecord(e1, st));
     unsigned long long hcnt = 0;
     HIPCHK(hipMemcpyAsync(&hcnt, d_cnt, 8, hipMemcpyDeviceToHost, st));
@@ -997,6 +998,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         eb.unit_end += u0;
         k_evaluate<<<dim3(bpt, un), block, 0, st>>>(eb);
         HIPCHK(hipGetLastError());
+        S.n_evaluate_launches++;
       }
       HIPCHK(hipEventRecord(e1, st));
       std::vector<unsigned char> ok(nt);

@@ -1,0 +1,331 @@
+"""Phase-map optimiser on MI355X: the contraction H = H_d + G diag(u) F, the
+AP codebook product and the coverage objective, with hand-written forward
+and backward kernels (libams.so) behind torch.autograd.Function.
+
+Mirrors go.py's Metasurface / Model / optimize_model (go.py:17-336) for
+metasurface chains of length one (max_ms_reflex <= 1), which is what
+physical_exp_nni_model.py runs (:113-128).
+
+Reassociation: y = W (H_d + G diag(u) F) = [ (W G) diag(u) | W ] . [ F ; H_d ],
+so one complex GEMM against the stacked channel operand [F ; H_d] yields the
+received signals of every code; its large operand is packed once into the
+split-bf16 MFMA layout (two copies: reduction over the metasurface index for
+the forward, over Rx for the backward).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence
+
+import torch
+import torch.nn as nn
+
+from . import _lib
+
+N0_90DBM = 10 ** (-90 / 10)  # go.py:177-178, noise floor -90 dB (mW)
+
+
+def _stream(dev=None) -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+def frag_pack(X: torch.Tensor, R: int, Kd: int, sr: int, sk: int) -> torch.Tensor:
+    """Pack complex64 X (element (r, k) at offset r*sr + k*sk) into the
+    split-bf16 fragment layout; returns a uint8 device buffer."""
+    L = _lib.lib()
+    nbytes = L.ams_frag_bytes(R, Kd)
+    out = torch.empty(nbytes, dtype=torch.uint8, device=X.device)
+    xr = torch.view_as_real(X) if X.is_complex() else X
+    _lib.check(L.ams_frag_pack(_ptr(xr), R, Kd, sr, sk, _ptr(out), _stream()),
+               "ams_frag_pack")
+    return out
+
+
+def cgemm_frag(fa: torch.Tensor, fb: torch.Tensor, Ra: int, Rb: int, Kd: int, conj_b: bool,
+               splits: int = 1) -> torch.Tensor:
+    """C[Ra, Rb] = A B^T (or A conj(B)^T) from fragment-packed operands."""
+    L = _lib.lib()
+    out = torch.empty(Ra, Rb, dtype=torch.complex64, device=fa.device)
+    ws = None
+    if splits > 1:
+        ws = torch.empty(max(1, L.ams_cgemm_workspace_bytes(Ra, Rb, splits)), dtype=torch.uint8, device=fa.device)
+    _lib.check(L.ams_cgemm_frag(_ptr(fa), _ptr(fb), Ra, Rb, Kd, 1 if conj_b else 0,
+                                _ptr(out), Rb, splits,
+                                _ptr(ws) if ws is not None else None, _stream()), "ams_cgemm_frag")
+    return out
+
+
+def cgemm_f32(A: torch.Tensor, B: torch.Tensor, conj_b: bool) -> torch.Tensor:
+    """Plain FP32 C = A B^T (complex64, row-major, contiguous)."""
+    L = _lib.lib()
+    A = A.contiguous()
+    B = B.contiguous()
+    Ra, Kd = A.shape
+    Rb = B.shape[0]
+    out = torch.empty(Ra, Rb, dtype=torch.complex64, device=A.device)
+    _lib.check(L.ams_cgemm_f32(_ptr(A), Kd, _ptr(B), Kd, Ra, Rb, Kd, 1 if conj_b else 0, _ptr(out), Rb, _stream()),
+               "ams_cgemm_f32")
+    return out
+
+
+class ChannelOperand:
+    """The stacked channel operand [F_0; ...; F_{m-1}; H_d] of shape [Kc, N]
+    packed once in both fragment layouts (kept resident in HBM)."""
+
+    def __init__(self, F_list: Sequence[torch.Tensor], H_d: torch.Tensor, keep_plain: bool = False):
+        parts = [f.to(torch.complex64) for f in F_list] + [H_d.to(torch.complex64)]
+        Fcat = torch.cat(parts, 0).contiguous()
+        self.Kc, self.N = Fcat.shape
+        self.K_list = [int(f.shape[0]) for f in F_list]
+        self.M = int(H_d.shape[0])
+        self.device = Fcat.device
+        # forward: rows = Rx, reduction over the stacked index
+        self.frag_fwd = frag_pack(Fcat, self.N, self.Kc, 1, self.N)
+        # backward: rows = stacked index, reduction over Rx
+        self.frag_bwd = frag_pack(Fcat, self.Kc, self.N, self.N, 1)
+        self.plain = Fcat if keep_plain else None
+        self.bwd_splits = max(1, min(64, (self.N // 32) // 8))
+
+    def nbytes(self) -> int:
+        return self.frag_fwd.numel() + self.frag_bwd.numel()
+
+
+class Contract(torch.autograd.Function):
+    """Y = A . [F ; H_d]  (A: [R, Kc] complex64) with the split-bf16 MFMA GEMM;
+    backward gA = gY . [F ; H_d]^H on the same kernels."""
+
+    @staticmethod
+    def forward(ctx, A: torch.Tensor, op: ChannelOperand):
+        A = A.contiguous()
+        R = A.shape[0]
+        fa = frag_pack(A, R, op.Kc, op.Kc, 1)
+        ctx.op = op
+        return cgemm_frag(fa, op.frag_fwd, R, op.N, op.Kc, conj_b=False)
+
+    @staticmethod
+    def backward(ctx, gY: torch.Tensor):
+        op = ctx.op
+        gY = gY.contiguous()
+        R = gY.shape[0]
+        fg = frag_pack(gY, R, op.N, op.N, 1)
+        gA = cgemm_frag(fg, op.frag_bwd, R, op.Kc, op.N, conj_b=True, splits=op.bwd_splits)
+        return gA, None
+
+
+class ContractPlain(torch.autograd.Function):
+    """The same product in plain FP32 (vector ALU kernel), for comparison."""
+
+    @staticmethod
+    def forward(ctx, A: torch.Tensor, op: ChannelOperand):
+        ctx.op = op
+        return cgemm_f32(A, op.plain.t().contiguous(), conj_b=False)
+
+    @staticmethod
+    def backward(ctx, gY: torch.Tensor):
+        op = ctx.op
+        return cgemm_f32(gY, op.plain, conj_b=True), None
+
+
+class Objective(torch.autograd.Function):
+    """Coverage objective over Y [G*C, N] (G groups of C codes): returns the
+    per-group loss [G].  mode 0: -mean_r log2(1 + max_c |y|^2 / n0);
+    mode 1: the reference capacity loss (go.py:214-225)."""
+
+    @staticmethod
+    def forward(ctx, Y: torch.Tensor, groups: int, mode: int, n0: float, w: float,
+                rowmask: Optional[torch.Tensor]):
+        L = _lib.lib()
+        Y = Y.contiguous()
+        GC, N = Y.shape
+        Cc = GC // groups
+        ws = torch.empty(L.ams_objective_workspace_bytes(groups, N), dtype=torch.uint8, device=Y.device)
+        loss = torch.empty(groups, dtype=torch.float32, device=Y.device)
+        rm = rowmask.to(device=Y.device, dtype=torch.uint8).contiguous() if rowmask is not None else None
+        _lib.check(L.ams_objective_fwd(_ptr(Y), groups, Cc, N, mode, n0, w,
+                                       _ptr(rm) if rm is not None else None, _ptr(loss), _ptr(ws), _stream()),
+                   "ams_objective_fwd")
+        ctx.save_for_backward(Y)
+        ctx.ws, ctx.rm, ctx.cfg = ws, rm, (groups, Cc, N, mode, n0, w)
+        return loss
+
+    @staticmethod
+    def backward(ctx, gloss: torch.Tensor):
+        L = _lib.lib()
+        (Y,) = ctx.saved_tensors
+        groups, Cc, N, mode, n0, w = ctx.cfg
+        gY = torch.empty_like(Y)
+        gl = gloss.to(torch.float32).contiguous()
+        _lib.check(L.ams_objective_bwd(_ptr(Y), groups, Cc, N, mode, n0, w,
+                                       _ptr(ctx.rm) if ctx.rm is not None else None, _ptr(gl), _ptr(ctx.ws),
+                                       _ptr(gY), _stream()),
+                   "ams_objective_bwd")
+        return gY, None, None, None, None, None
+
+
+@dataclass
+class Metasurface:
+    """go.py:17-24; max_phase in degrees."""
+    pos: torch.Tensor
+    size: tuple
+    send_dir: torch.Tensor
+    recv_dir: torch.Tensor
+    blackhole: list
+    max_phase: float = 360.0
+
+    @property
+    def max_phase_rad(self) -> float:
+        return self.max_phase / 180 * math.pi
+
+
+class PhaseMapModel(nn.Module):
+    """Codebook + metasurface phase model (go.py:27-245, chains of length <= 1).
+
+    channels: {"src_to_dst": H_d [M, N], "src_to_ms{i}": G_i [M, K_i],
+               "ms{i}_to_dst": F_i [K_i, N]} complex on the GPU.
+    n_maps > 1 optimises that many metasurface phase maps at once (rows of
+    the GEMM stacked as (map, code)); the loss is their sum.
+    """
+
+    def __init__(self, name: str, channels: Dict[str, torch.Tensor], ms_list: List[Metasurface],
+                 code_num: int = 30, w: float = 1.0, objective: str = "logsnr", n0: float = N0_90DBM,
+                 n_maps: int = 1, optimize_codebook: bool = True, antenna_theta: Optional[torch.Tensor] = None,
+                 ms_theta: Optional[List[torch.Tensor]] = None, plain: bool = False, seed: int = 0):
+        super().__init__()
+        dev = channels["src_to_dst"].device
+        self.name = name
+        self.ms_list = ms_list
+        self.w = w
+        self.n0 = n0
+        self.mode = {"logsnr": 0, "capacity": 1}[objective]
+        self.n_maps = n_maps
+        self.plain = plain
+        M = channels["src_to_dst"].shape[0]
+        self.code_num = code_num
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        # go.py:59-66: amplitude 1/sqrt(M/36); the 4 corners of a 6x6 AP are off
+        A = torch.ones(code_num, M, dtype=torch.float32) / math.sqrt(M / 36)
+        if M == 36:
+            A[:, [0, 5, 30, 35]] = 0
+        self.register_buffer("antenna_A", A.to(dev))
+        th = antenna_theta if antenna_theta is not None else torch.rand(code_num, M, generator=g) * 2 * math.pi
+        self.antenna_theta = nn.Parameter(th.to(dev, torch.float32).clone(), requires_grad=optimize_codebook)
+        self.ms_theta = nn.ParameterList()
+        self.ms_A = []
+        for i in range(len(ms_list)):
+            K = channels["src_to_ms{%d}" % i].shape[1]
+            self.ms_A.append(torch.ones(K, dtype=torch.float32, device=dev))
+            t0 = ms_theta[i] if ms_theta is not None else torch.zeros(n_maps, K)
+            self.ms_theta.append(nn.Parameter(t0.to(dev, torch.float32).reshape(n_maps, K).clone()))
+        self.G_list = [channels["src_to_ms{%d}" % i].to(torch.complex64) for i in range(len(ms_list))]
+        F_list = [channels["ms{%d}_to_dst" % i] for i in range(len(ms_list))]
+        self.op = ChannelOperand(F_list, channels["src_to_dst"], keep_plain=plain)
+        self.loss_list: List[float] = []
+        self.rowmask_fn = None
+
+    def stacked_operand(self) -> torch.Tensor:
+        """A [n_maps * C, Kc] = [ (W G_i) diag(u_i) ... | W ] per map."""
+        W = self.antenna_A * torch.exp(-1j * self.antenna_theta)  # go.py:163
+        rows = []
+        for m in range(self.n_maps):
+            parts = []
+            for i, ms in enumerate(self.ms_list):
+                phase = torch.remainder(self.ms_theta[i][m], ms.max_phase_rad)  # go.py:131
+                u = self.ms_A[i] * torch.exp(1j * phase)
+                parts.append((W @ self.G_list[i]) * u)
+            parts.append(W.to(torch.complex64))
+            rows.append(torch.cat(parts, 1))
+        return torch.cat(rows, 0)
+
+    def received(self) -> torch.Tensor:
+        A = self.stacked_operand()
+        fn = ContractPlain if self.plain else Contract
+        return fn.apply(A, self.op)
+
+    def forward(self) -> torch.Tensor:
+        Y = self.received()
+        rowmask = None
+        if self.mode == 1:
+            # go.py:220: rows of cap zeroed where torch.rand(C) < 0.1 (CPU generator)
+            rowmask = (torch.rand(self.code_num * self.n_maps) < 0.1) if self.rowmask_fn is None \
+                else self.rowmask_fn()
+        loss = Objective.apply(Y, self.n_maps, self.mode, self.n0, self.w, rowmask)
+        self.loss = loss.sum()
+        return self.loss
+
+    @torch.no_grad()
+    def coverage_db(self) -> torch.Tensor:
+        """Mean over Rx of the best code's received power in dB, per map."""
+        Y = self.received().reshape(self.n_maps, self.code_num, -1)
+        p = (Y.real ** 2 + Y.imag ** 2).amax(1)
+        return (10 * torch.log10(p + 1e-30)).mean(1)
+
+
+class FusedAdam:
+    """torch.optim.Adam semantics (defaults beta=(0.9, 0.999), eps=1e-8) with
+    one fused kernel over all parameters (ams_adam_step)."""
+
+    def __init__(self, params, lr: float = 0.3, betas=(0.9, 0.999), eps: float = 1e-8):
+        self.params = [p for p in params if p.requires_grad]
+        self.lr, self.b1, self.b2, self.eps = lr, betas[0], betas[1], eps
+        n = sum(p.numel() for p in self.params)
+        dev = self.params[0].device
+        self.flat = torch.cat([p.detach().reshape(-1) for p in self.params]).contiguous()
+        self.m = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.v = torch.zeros(n, dtype=torch.float32, device=dev)
+        off = 0
+        for p in self.params:  # parameters become views of one flat buffer
+            k = p.numel()
+            p.data = self.flat[off:off + k].view_as(p)
+            off += k
+        self.t = 0
+
+    def zero_grad(self):
+        for p in self.params:
+            p.grad = None
+
+    @torch.no_grad()
+    def step(self):
+        self.t += 1
+        g = torch.cat([(p.grad if p.grad is not None else torch.zeros_like(p)).reshape(-1) for p in self.params])
+        g = g.to(torch.float32).contiguous()
+        _lib.check(_lib.lib().ams_adam_step(_ptr(self.flat), _ptr(g), _ptr(self.m), _ptr(self.v), self.flat.numel(),
+                                            self.lr, self.b1, self.b2, self.eps, self.t, _stream()),
+                   "ams_adam_step")
+
+
+def optimize_model(model: PhaseMapModel, lr: float = 0.3, step: int = 200, fused: bool = True,
+                   record: bool = True, callback=None):
+    """Adam over the model parameters keeping the best-loss parameters
+    (go.py:296-336).  The best loss is tracked on the device (no per-step
+    host sync); the loss history is read back once at the end."""
+    params = [p for p in model.parameters() if p.requires_grad]
+    opt = FusedAdam(params, lr=lr) if fused else torch.optim.Adam(params, lr=lr)
+    best = torch.tensor(float("inf"), device=model.antenna_theta.device)
+    best_state = [p.detach().clone() for p in params]
+    hist = []
+    for it in range(step):
+        loss = model()
+        with torch.no_grad():
+            better = loss.detach() < best
+            best = torch.where(better, loss.detach(), best)
+            for b, p in zip(best_state, params):
+                b.copy_(torch.where(better, p.detach(), b))
+        if record:
+            hist.append(loss.detach())
+        loss.backward()
+        opt.step()
+        opt.zero_grad()
+        if callback is not None:
+            callback(it, model)
+    with torch.no_grad():
+        for b, p in zip(best_state, params):
+            p.copy_(b)
+    if record and hist:
+        model.loss_list.extend(torch.stack(hist).cpu().tolist())
+    return model

@@ -1,0 +1,165 @@
+#!/usr/bin/env python
+"""Benchmark of the AutoMS channel and coverage path on one MI355X.
+
+A step is one full channel trace of the workload: forward discovery of
+(path, Rx-tile) records, GPU dedup, and the backward evaluation rounds with
+neighbour expansion, accumulating complex H[tx, rx].  Inputs (scene handle,
+Tx/Rx arrays) are resident before the timed region.
+
+Prints one JSON line:
+  value        closest-hit queries per second (Mrays/s), whole job
+  rx_channels_per_s   n_tx * n_rx / step seconds, summed over the traces
+  roofline     the dominant kernel's algorithmic bytes / HIP-event time
+  cpu_baseline the FP64 oracle on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(stats: dict) -> dict:
+    """Algorithmic bytes per kernel: 32 B per BVH node visit, 48 B per
+    triangle test, 16 B per sphere test, 64 B of ray / hit I/O per query."""
+    fwd = 32 * stats["fwd_nodes"] + 48 * stats["fwd_tris"] + 16 * stats["fwd_spheres"] + 64 * stats["fwd_queries"]
+    bwd = 32 * stats["bwd_nodes"] + 48 * stats["bwd_tris"] + 64 * stats["bwd_queries"]
+    return {"k_discover": fwd, "k_evaluate": bwd}
+
+
+def workload_traces(name: str):
+    from automs_amd import scenes
+    if name == "room_boxes":
+        return {"H": scenes.room_boxes()}
+    if name == "room_boxes_ms":
+        return scenes.room_boxes_ms()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int) -> dict:
+    """FP64 oracle (OpenMP) on the same scene with fewer rays per trace."""
+    from automs_amd import scenes
+    from oracle import oracle as O
+    if name == "room_boxes":
+        traces = {"H": scenes.room_boxes(fib_n=sample_fib)}
+    else:
+        traces = scenes.room_boxes_ms(fib_n=sample_fib, ms_fib_n=sample_ms_fib)
+    q = 0
+    t0 = time.perf_counter()
+    for w in traces.values():
+        _, st, _ = O.trace_workload(w)
+        q += st["fwd_queries"] + st["bwd_queries"]
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": q / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"{name}: every trace with {sample_fib} Fibonacci rays per AP element"
+                      f"{'' if name == 'room_boxes' else f' and {sample_ms_fib} rays in total from the metasurface'}"
+                      f" ({q} closest-hit queries in {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="room_boxes", choices=["room_boxes", "room_boxes_ms"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-fib", type=int, default=1 << 12)
+    ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the path shards by Rx tiles "
+              "across GPUs by design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from automs_amd.tracer import TraceScene, trace
+
+    dev = torch.device("cuda:0")
+    traces = workload_traces(args.workload)
+    handles = {k: TraceScene(w.scene) for k, w in traces.items()}
+    outs = {k: torch.zeros(w.tx.n, w.rx.n, 2, dtype=torch.float32, device=dev) for k, w in traces.items()}
+
+    def step():
+        agg = {}
+        for k, w in traces.items():
+            outs[k].zero_()
+            _, st, _ = trace(handles[k], w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
+                             w.gridsize, result=outs[k])
+            for f, v in st.items():
+                agg[f] = agg.get(f, 0) + v
+        return agg
+
+    for i in range(args.warmup):
+        step()
+        print(f"warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tot = {}
+    for i in range(args.steps):
+        st = step()
+        for f, v in st.items():
+            tot[f] = tot.get(f, 0) + v
+        print(f"step {i + 1}/{args.steps}: {st['t_total_ms']:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    queries = tot["fwd_queries"] + tot["bwd_queries"]
+    kb = kernel_bytes(tot)
+    kt = {"k_discover": tot["t_discover_ms"], "k_evaluate": tot["t_evaluate_ms"]}
+    kl = {"k_discover": tot["n_discover_launches"], "k_evaluate": tot["n_evaluate_launches"]}
+    dom = max(kt, key=kt.get)
+    achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
+    n_channels = sum(w.tx.n * w.rx.n for w in traces.values())
+    line = {
+        "metric": "Mrays/s & Rx-channels/s; phase-map opt sec to <=0.1 dB of plain optimum",
+        "value": queries / dt / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "traces": {k: {"n_tx": w.tx.n, "n_rx": w.rx.n, "tris": w.scene.n_tri, "max_reflection": w.max_reflection,
+                           "rays": (w.fib_n * w.tx.n if w.per_element else w.fib_n)} for k, w in traces.items()},
+            "parallelism": "single GPU",
+        },
+        "rx_channels_per_s": n_channels * args.steps / dt,
+        "primary_rays_per_s": tot["rays"] / dt,
+        "per_step": {k: tot[k] / args.steps for k in ("rays", "paths", "tasks", "rounds", "chains", "successes",
+                                                      "fwd_queries", "bwd_queries", "t_discover_ms",
+                                                      "t_evaluate_ms", "t_total_ms")},
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": kt[dom] / max(1, kl[dom]),
+            "algorithmic_bytes_per_launch": kb[dom] / max(1, kl[dom]),
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_fib, args.cpu_sample_ms_fib)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -57,7 +57,8 @@ typedef struct {
   int64_t rays, records_raw, records_unique, paths, tasks, rounds, chains, successes, overflow;
   int64_t fwd_queries, fwd_nodes, fwd_tris, fwd_spheres;
   int64_t bwd_queries, bwd_nodes, bwd_tris;
-  double t_discover_ms, t_evaluate_ms, t_total_ms;
+  int64_t n_discover_launches, n_evaluate_launches;
+  double t_discover_ms, t_evaluate_ms, t_total_ms; /* kernel times from HIP events; wall total */
 } ams_trace_stats;
 
 /* Scene: scene_triangle_buffer [3T,3] f32, mat [T] i32, mat_prop [n_mat,19]
@@ -75,6 +76,38 @@ int ams_scene_plane_ids(const ams_scene *scene, int32_t *out);
  * buffer [n_tx][n_rx][2] (go.py:671-677, optixPathTracer.cu:259-264). */
 int ams_trace_channel(ams_scene *scene, const ams_array *tx, const ams_array *rx,
                       const ams_trace_opts *opts, float *H_out, ams_trace_stats *stats);
+
+/* ---- phase-map contraction (go.py:107-164) -----------------------------
+ * Complex matrices are interleaved f32 (re, im) in DEVICE memory.
+ * Fragment layout: a complex matrix X[R][Kd] split into bf16 hi + lo parts in
+ * the lane order of v_mfma_f32_16x16x32_bf16, ams_frag_bytes(R, Kd) bytes. */
+int64_t ams_frag_bytes(int32_t R, int32_t Kd);
+/* pack X with element (r, k) at X + 2*(r*sr + k*sk) floats */
+int ams_frag_pack(const float *X, int32_t R, int32_t Kd, int64_t sr, int64_t sk, void *frag, void *stream);
+/* C[Ra][Rb] = A . B^T (conj_b: A . conj(B)^T), split-bf16 MFMA, FP32
+ * accumulation; splits > 1 splits the reduction and needs
+ * ams_cgemm_workspace_bytes(Ra, Rb, splits) of workspace. */
+int64_t ams_cgemm_workspace_bytes(int32_t Ra, int32_t Rb, int32_t splits);
+int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b, float *C,
+                   int64_t ldc, int32_t splits, void *workspace, void *stream);
+/* the same product in plain FP32 on the vector ALU (row strides in elements) */
+int ams_cgemm_f32(const float *A, int64_t lda, const float *B, int64_t ldb, int32_t Ra, int32_t Rb, int32_t Kd,
+                  int32_t conj_b, float *C, int64_t ldc, void *stream);
+
+/* ---- coverage objective (go.py:159-245) and Adam (go.py:296-336) -------
+ * Y complex f32 [G][C][N]: G groups (phase maps) x C codes x N Rx.
+ * mode 0: L_g = -(1/N) sum_r log2(1 + max_c |y|^2 / n0)
+ * mode 1: the reference capacity loss with row mask rowmask[C] (u8, device)
+ *         and weight w (go.py:214-225). */
+int64_t ams_objective_workspace_bytes(int32_t G, int32_t N);
+int ams_objective_fwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                      const unsigned char *rowmask, float *loss, void *workspace, void *stream);
+int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                      const unsigned char *rowmask, const float *gloss, const void *workspace, float *gY,
+                      void *stream);
+/* torch.optim.Adam step (no weight decay), step counted from 1 */
+int ams_adam_step(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
+                  float beta2, float eps, int32_t step, void *stream);
 
 const char *ams_last_error(void);
 int ams_version(void);

@@ -1,0 +1,67 @@
+"""Torch-CPU (float64 / complex128) restatement of the reference's phase-map
+model, used as the oracle of the contraction, objective and gradients.
+
+TEST INFRASTRUCTURE ONLY (tests/, bench.py cpu_baseline).
+
+Follows go.py:107-164 (get_channel for chains of length one, AP codebook
+product) and go.py:166-225 (capacity loss), and adds the mean-log-SNR
+objective of the synthetic workloads.  Gradients come from torch autograd on
+the CPU; torch is an installed library, not the reference.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional, Sequence
+
+import torch
+
+EPS32 = float(torch.finfo(torch.float32).eps)
+
+
+def received(antenna_A: torch.Tensor, antenna_theta: torch.Tensor, H_d: torch.Tensor,
+             G_list: Sequence[torch.Tensor], F_list: Sequence[torch.Tensor], ms_theta: Sequence[torch.Tensor],
+             max_phase: Sequence[float], ms_A: Optional[Sequence[torch.Tensor]] = None) -> torch.Tensor:
+    """y = W (H_d + sum_i G_i diag(u_i) F_i), go.py:107-164; ms_theta[i] is
+    [K_i] or [maps, K_i] (then y is [maps * C, N])."""
+    W = antenna_A.to(torch.float64) * torch.exp(-1j * antenna_theta.to(torch.float64))
+    maps = ms_theta[0].reshape(-1, G_list[0].shape[1]).shape[0] if len(ms_theta) else 1
+    out = []
+    for m in range(maps):
+        ch = H_d.to(torch.complex128)
+        for i in range(len(G_list)):
+            th = ms_theta[i].reshape(maps, -1)[m].to(torch.float64)
+            real_phase = torch.remainder(th, max_phase[i])  # go.py:131
+            A = ms_A[i].to(torch.float64) if ms_A is not None else torch.ones_like(real_phase)
+            u = A * torch.exp(1j * real_phase)
+            ch = ch + torch.matmul(G_list[i].to(torch.complex128) * u, F_list[i].to(torch.complex128))
+        out.append(torch.matmul(W, ch))
+    return torch.cat(out, 0)
+
+
+def loss_logsnr(Y: torch.Tensor, groups: int, n0: float) -> torch.Tensor:
+    """-(1/N) sum_r log2(1 + max_c |y|^2 / n0) per group, summed."""
+    GC, N = Y.shape
+    s = (Y.real ** 2 + Y.imag ** 2).reshape(groups, GC // groups, N)
+    best, _ = s.max(1)
+    return (-torch.log2(1 + best / n0).mean(1)).sum()
+
+
+def loss_capacity(Y: torch.Tensor, n0: float, w: float, rowmask: Optional[torch.Tensor]) -> torch.Tensor:
+    """go.py:166-225 with loss_mode 'capacity' (one group)."""
+    recv_amp = Y.abs()
+    rx_rss = 20 * torch.log10(recv_amp + EPS32)
+    rss_power = 10 ** (rx_rss / 10)
+    cap = torch.log2(1 + rss_power / n0)
+    rand_cap = cap.clone()
+    if rowmask is not None:
+        rand_cap[rowmask.bool()] = 0
+    max_cap, _ = rand_cap.max(0)
+    mask = max_cap != rand_cap
+    masked_cap = rand_cap * mask
+    return -w * max_cap.sum() + masked_cap.norm("fro")
+
+
+def coverage_db(Y: torch.Tensor, groups: int = 1) -> torch.Tensor:
+    GC, N = Y.shape
+    p = (Y.real ** 2 + Y.imag ** 2).reshape(groups, GC // groups, N).amax(1)
+    return (10 * torch.log10(p + 1e-30)).mean(1)

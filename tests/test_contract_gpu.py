@@ -1,0 +1,177 @@
+"""GPU parity of the split-bf16 MFMA contraction, the objectives, their
+hand-written backward and the fused Adam step.
+
+Oracle: torch CPU complex128 / float64 restatement (oracle/contract_ref.py)
+and central finite differences.  Tolerances stated per test.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def _crand(*shape, scale=1.0, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.randn(*shape, generator=g, dtype=torch.float64)
+            + 1j * torch.randn(*shape, generator=g, dtype=torch.float64)) * scale
+
+
+@pytest.mark.parametrize("Ra,Rb,Kd", [(16, 64, 32), (30, 4096, 1040), (37, 333, 70), (256, 512, 2304)])
+@pytest.mark.parametrize("conj_b", [False, True])
+def test_cgemm_frag_matches_complex128(gpu, Ra, Rb, Kd, conj_b):
+    """Split-bf16 MFMA GEMM: relative Frobenius error < 3e-5 (hi/lo split
+    keeps ~16 significant bits per operand)."""
+    from automs_amd.phasemap import cgemm_frag, frag_pack
+    _flush(f"case: cgemm Ra={Ra} Rb={Rb} Kd={Kd} conj={conj_b}")
+    A = _crand(Ra, Kd, seed=1)
+    B = _crand(Rb, Kd, seed=2)
+    ref = A @ (B.conj() if conj_b else B).T
+    fa = frag_pack(A.to(torch.complex64).to(gpu), Ra, Kd, Kd, 1)
+    fb = frag_pack(B.to(torch.complex64).to(gpu), Rb, Kd, Kd, 1)
+    for splits in (1, 4):
+        out = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits).cpu().to(torch.complex128)
+        rel = (out - ref).norm() / ref.norm()
+        _flush(f"  splits={splits} rel={rel:.3e}")
+        assert rel < 3e-5
+
+
+def test_cgemm_f32_plain(gpu):
+    from automs_amd.phasemap import cgemm_f32
+    A = _crand(30, 300, seed=3)
+    B = _crand(200, 300, seed=4)
+    for conj_b in (False, True):
+        ref = A @ (B.conj() if conj_b else B).T
+        out = cgemm_f32(A.to(torch.complex64).to(gpu), B.to(torch.complex64).to(gpu), conj_b).cpu()
+        rel = (out.to(torch.complex128) - ref).norm() / ref.norm()
+        assert rel < 1e-6
+
+
+def _channels(M=16, K=256, N=1024, seed=0, dev="cpu"):
+    # magnitudes of a 60 GHz indoor channel: |H| ~ 1e-4 .. 1e-3
+    Hd = _crand(M, N, scale=3e-5, seed=seed + 10)
+    G = _crand(M, K, scale=3e-4, seed=seed + 11)
+    F = _crand(K, N, scale=3e-4, seed=seed + 12)
+    return Hd, G, F
+
+
+@pytest.mark.parametrize("objective", ["logsnr", "capacity"])
+def test_model_loss_and_gradient_vs_oracle(gpu, objective):
+    """Loss within 1e-4 relative of the complex128 oracle; gradients (the
+    hand-written backward through Contract + Objective) within 2e-3 relative
+    in norm of the oracle's autograd gradient."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    from oracle import contract_ref as R
+    _flush(f"case: model loss/grad objective={objective}")
+    Hd, G, F = _channels()
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (16, 16), None, None, [], max_phase=300.0)]
+    gth = torch.Generator().manual_seed(7)
+    th_ms = torch.rand(1, 256, generator=gth) * 2 * math.pi
+    model = PhaseMapModel("t", ch, ms, code_num=30, w=0.5, objective=objective, ms_theta=[th_ms])
+    mask = torch.rand(30, generator=torch.Generator().manual_seed(3)) < 0.1
+    model.rowmask_fn = lambda: mask
+    loss = model()
+    loss.backward()
+    g_ap = model.antenna_theta.grad.cpu().double()
+    g_ms = model.ms_theta[0].grad.cpu().double().reshape(-1)
+    # oracle
+    ta = model.antenna_theta.detach().cpu().double().requires_grad_(True)
+    tm = th_ms.double().reshape(-1).clone().requires_grad_(True)
+    Y = R.received(model.antenna_A.cpu(), ta, Hd, [G], [F], [tm], [300 / 180 * math.pi])
+    lo = R.loss_logsnr(Y, 1, model.n0) if objective == "logsnr" else R.loss_capacity(Y, model.n0, 0.5, mask)
+    lo.backward()
+    rel_l = abs(loss.item() - lo.item()) / abs(lo.item())
+    rel_ga = (g_ap - ta.grad).norm() / ta.grad.norm()
+    rel_gm = (g_ms - tm.grad).norm() / tm.grad.norm()
+    _flush(f"  loss gpu={loss.item():.6f} oracle={lo.item():.6f} rel={rel_l:.2e} grad_ap rel={rel_ga:.2e} "
+           f"grad_ms rel={rel_gm:.2e}")
+    assert rel_l < 1e-4
+    assert rel_ga < 2e-3
+    assert rel_gm < 2e-3
+
+
+def test_gradient_vs_finite_differences(gpu):
+    """Central differences (h = 1e-3 rad) of the oracle loss at coordinates
+    away from ties agree with the GPU gradient to 2e-2 relative."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    from oracle import contract_ref as R
+    _flush("case: finite differences logsnr")
+    Hd, G, F = _channels(K=128, N=512, seed=5)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (8, 16), None, None, [], max_phase=360.0)]
+    th_ms = torch.rand(1, 128, generator=torch.Generator().manual_seed(9)) * 2 * math.pi
+    model = PhaseMapModel("t", ch, ms, code_num=8, objective="logsnr", ms_theta=[th_ms])
+    loss = model()
+    loss.backward()
+    g_ms = model.ms_theta[0].grad.cpu().double().reshape(-1)
+    A = model.antenna_A.cpu()
+    ta = model.antenna_theta.detach().cpu().double()
+    h = 1e-3
+
+    def f(th):
+        Y = R.received(A, ta, Hd, [G], [F], [th], [2 * math.pi])
+        return R.loss_logsnr(Y, 1, model.n0).item()
+
+    base = th_ms.double().reshape(-1)
+    idx = [3, 17, 40, 77, 101, 127]
+    fd = []
+    for k in idx:
+        tp, tm_ = base.clone(), base.clone()
+        tp[k] += h
+        tm_[k] -= h
+        fd.append((f(tp) - f(tm_)) / (2 * h))
+    fd = torch.tensor(fd)
+    gg = g_ms[idx]
+    rel = (gg - fd).norm() / fd.norm()
+    _flush(f"  fd={fd.numpy()} gpu={gg.numpy()} rel={rel:.2e}")
+    assert rel < 2e-2
+
+
+def test_fused_adam_matches_torch(gpu):
+    from automs_amd.phasemap import FusedAdam
+    g = torch.Generator().manual_seed(0)
+    p0 = torch.randn(1000, generator=g)
+    grads = [torch.randn(1000, generator=g) for _ in range(5)]
+    a = torch.nn.Parameter(p0.clone().to(gpu))
+    b = torch.nn.Parameter(p0.clone().to(gpu))
+    oa = FusedAdam([a], lr=0.3)
+    ob = torch.optim.Adam([b], lr=0.3)
+    for gr in grads:
+        a.grad = gr.to(gpu)
+        b.grad = gr.to(gpu)
+        oa.step()
+        ob.step()
+    err = (a.detach() - b.detach()).abs().max().item()
+    _flush(f"case: adam max abs diff {err:.2e}")
+    assert err < 1e-5
+
+
+def test_batched_maps_match_single(gpu):
+    """Four phase maps in one GEMM give the per-map losses of four separate
+    models."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    Hd, G, F = _channels(M=16, K=192, N=700, seed=21)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (12, 16), None, None, [], max_phase=360.0)]
+    th = torch.rand(4, 192, generator=torch.Generator().manual_seed(2)) * 2 * math.pi
+    th_ap = torch.rand(8, 16, generator=torch.Generator().manual_seed(4)) * 2 * math.pi
+    big = PhaseMapModel("b", ch, ms, code_num=8, n_maps=4, ms_theta=[th], antenna_theta=th_ap)
+    lb = big().item()
+    ls = 0.0
+    for m in range(4):
+        one = PhaseMapModel("s", ch, ms, code_num=8, n_maps=1, ms_theta=[th[m:m + 1]], antenna_theta=th_ap)
+        ls += one().item()
+    _flush(f"case: batched maps {lb:.6f} vs {ls:.6f}")
+    assert abs(lb - ls) / abs(ls) < 1e-5
