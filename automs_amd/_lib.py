@@ -102,6 +102,8 @@ def _bind_contract(L):
         "ams_objective_fwd": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp], C.c_int),
         "ams_objective_bwd": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp, vp], C.c_int),
         "ams_adam_step": ([vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp], C.c_int),
+        "ams_host_bvh_build": ([vp, vp, i32, i32, vp, i32, vp, vp], C.c_int),
+        "ams_host_plane_ids": ([vp, i32, vp], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -120,4 +122,5 @@ EXPORTED = [
     "ams_last_error", "ams_version", "ams_mem_outstanding",
     "ams_frag_bytes", "ams_frag_pack", "ams_cgemm_workspace_bytes", "ams_cgemm_frag", "ams_cgemm_f32",
     "ams_objective_workspace_bytes", "ams_objective_fwd", "ams_objective_bwd", "ams_adam_step",
+    "ams_host_bvh_build", "ams_host_plane_ids",
 ]

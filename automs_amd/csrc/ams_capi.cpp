@@ -7,7 +7,10 @@
 #include <string>
 #include <unordered_map>
 
+#include <vector>
+
 #include "../../include/ams.h"
+#include "ams_bvh.h"
 #include "ams_internal.h"
 
 namespace {
@@ -154,6 +157,36 @@ int ams_trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, co
     return AMS_ERR_ARG;
   }
   return ams::trace_channel(s, tx, rx, o, H_out, stats);
+}
+
+int ams_host_bvh_build(const double *lo, const double *hi, int32_t n, int32_t leaf_max, void *nodes_out,
+                       int32_t nodes_cap, int32_t *order_out, int32_t *n_nodes_out) {
+  if (!lo || !hi || n < 0 || !nodes_out || !order_out || !n_nodes_out) {
+    ams::set_error("ams_host_bvh_build: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  AmsBvhBuild B;
+  ams_bvh_build(n, lo, hi, leaf_max, &B);
+  *n_nodes_out = (int32_t)B.nodes.size();
+  if ((int32_t)B.nodes.size() > nodes_cap) {
+    ams::set_error("ams_host_bvh_build: nodes_cap too small");
+    return AMS_ERR_ARG;
+  }
+  std::memcpy(nodes_out, B.nodes.data(), sizeof(AmsNode) * B.nodes.size());
+  for (int i = 0; i < n; i++) order_out[i] = B.order[i];
+  return AMS_OK;
+}
+
+int ams_host_plane_ids(const float *tri, int32_t n_tri, int32_t *plane_out) {
+  if ((n_tri > 0 && !tri) || !plane_out || n_tri < 0) {
+    ams::set_error("ams_host_plane_ids: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  std::vector<int> pl(n_tri);
+  std::vector<double> pn(3 * (size_t)n_tri), pd(n_tri);
+  if (n_tri > 0) ams_plane_ids(n_tri, tri, pl.data(), pn.data(), pd.data());
+  for (int i = 0; i < n_tri; i++) plane_out[i] = pl[i];
+  return AMS_OK;
 }
 
 }  // extern "C"

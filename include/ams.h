@@ -109,6 +109,14 @@ int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t m
 int ams_adam_step(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
                   float beta2, float eps, int32_t step, void *stream);
 
+/* ---- host-only helpers (no device calls; used by the CPU test suite) ---- */
+/* the binned-SAH threaded BVH of ams_scene_create: nodes_out gets 32-byte
+ * nodes {lo[3] f32, miss i32, hi[3] f32, info i32}, order_out the leaf order */
+int ams_host_bvh_build(const double *lo, const double *hi, int32_t n, int32_t leaf_max, void *nodes_out,
+                       int32_t nodes_cap, int32_t *order_out, int32_t *n_nodes_out);
+/* plane id of every triangle as ams_scene_plane_ids reports it */
+int ams_host_plane_ids(const float *tri, int32_t n_tri, int32_t *plane_out);
+
 const char *ams_last_error(void);
 int ams_version(void);
 /* device bytes currently held by the library (cudaMemManager.h:75-81) */

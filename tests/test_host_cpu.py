@@ -1,0 +1,191 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+symbol include/ams.h declares; the threaded BVH the kernels traverse gives
+brute-force closest hits; plane ids agree with the oracle's; geometry
+helpers and seeded scenes follow their definitions; the contraction oracle's
+autograd gradient matches central differences (complex128)."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_declared_symbol():
+    from automs_amd import _lib
+    header = (ROOT / "include" / "ams.h").read_text()
+    declared = set(re.findall(r"\b(ams_[a-z0-9_]+)\s*\(", header))
+    L = _lib.lib()
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(_lib.EXPORTED) == declared
+    assert L.ams_version() == 1
+
+
+def _bvh(lo, hi, leaf_max=4):
+    from automs_amd import _lib
+    L = _lib.lib()
+    n = lo.shape[0]
+    nodes = np.zeros((2 * n + 1, 8), np.float32)
+    order = np.zeros(max(n, 1), np.int32)
+    nn = C.c_int32()
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    _lib.check(L.ams_host_bvh_build(C.c_void_p(lo.ctypes.data), C.c_void_p(hi.ctypes.data), n, leaf_max,
+                                    C.c_void_p(nodes.ctypes.data), nodes.shape[0],
+                                    C.c_void_p(order.ctypes.data), C.c_void_p(C.addressof(nn))), "bvh")
+    nodes = nodes[: nn.value]
+    ints = nodes.view(np.int32)
+    return nodes[:, 0:3], ints[:, 3], nodes[:, 4:7], ints[:, 7], order
+
+
+def _mt(o, d, v0, v1, v2):
+    e1, e2 = v1 - v0, v2 - v0
+    p = np.cross(d, e2)
+    det = e1 @ p
+    if det == 0:
+        return None
+    tv = o - v0
+    u = (tv @ p) / det
+    if u < 0 or u > 1:
+        return None
+    q = np.cross(tv, e1)
+    v = (d @ q) / det
+    if v < 0 or u + v > 1:
+        return None
+    return (e2 @ q) / det
+
+
+def test_threaded_bvh_closest_hit_equals_brute_force():
+    rng = np.random.default_rng(1)
+    T = 200
+    tri = (rng.uniform(-4, 4, (T, 1, 3)) + rng.normal(0, 0.5, (T, 3, 3)))
+    lo, hi = tri.min(1), tri.max(1)
+    blo, miss, bhi, info, order = _bvh(lo, hi)
+    n_nodes = len(miss)
+    # structure: preorder, every miss link points past its subtree, leaves cover all prims once
+    assert miss[0] == n_nodes and (miss > np.arange(n_nodes)).all()
+    seen = []
+    for i in range(n_nodes):
+        if info[i]:
+            first, cnt = info[i] >> 3, info[i] & 7
+            assert 1 <= cnt <= 4
+            seen.extend(order[first:first + cnt])
+            # leaf bounds contain their triangles
+            for p in order[first:first + cnt]:
+                assert (blo[i] <= lo[p] + 1e-6).all() and (bhi[i] >= hi[p] - 1e-6).all()
+    assert sorted(seen) == list(range(T))
+    for _ in range(200):
+        o = rng.uniform(-5, 5, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        inv = 1 / np.where(d == 0, 1e-30, d)
+        best, bi = np.inf, -1
+        i = 0
+        while i < n_nodes:  # the kernels' stackless loop
+            t0 = np.minimum((blo[i] - o) * inv, (bhi[i] - o) * inv).max()
+            t1 = np.maximum((blo[i] - o) * inv, (bhi[i] - o) * inv).min()
+            if max(t0, 1e-5) <= min(t1, best) * (1 + 4e-7):
+                if info[i] == 0:
+                    i += 1
+                    continue
+                for p in order[(info[i] >> 3):(info[i] >> 3) + (info[i] & 7)]:
+                    t = _mt(o, d, *tri[p])
+                    if t is not None and 1e-5 <= t < best:
+                        best, bi = t, p
+            i = miss[i]
+        bbest, bbi = np.inf, -1
+        for p in range(T):
+            t = _mt(o, d, *tri[p])
+            if t is not None and 1e-5 <= t < bbest:
+                bbest, bbi = t, p
+        assert bi == bbi
+
+
+def test_plane_ids_match_oracle():
+    from automs_amd import _lib, scenes
+    from oracle import oracle as O
+    sc = scenes.room_boxes_scene(seed=3)
+    tri = np.ascontiguousarray(sc.tri, np.float32)
+    out = np.zeros(sc.n_tri, np.int32)
+    _lib.check(_lib.lib().ams_host_plane_ids(C.c_void_p(tri.ctypes.data), sc.n_tri, C.c_void_p(out.ctypes.data)),
+               "plane ids")
+    assert np.array_equal(out, O.OracleScene.from_scene(sc).plane_ids())
+    # the two triangles of every face share a plane; the room floor and the
+    # box bottoms (resting on it) share one plane too
+    assert (out[0::2] == out[1::2]).all()
+    floor = out[2]
+    assert all(out[12 * (b + 1) + 2] == floor for b in range(20))
+
+
+def test_geometry_helpers():
+    from automs_amd import geometry as Gm
+    wl = 0.005
+    pos = Gm.gen_pos_matrix([1.0, 2.0, 3.0], [1, 0, 0], [0, 0, -1], (4, 4), wl / 2)
+    assert pos.shape == (4, 4, 3)
+    assert torch.allclose(pos.reshape(-1, 3).mean(0), torch.tensor([1.0, 2.0, 3.0]))
+    assert torch.allclose((pos[1, 0] - pos[0, 0]).norm(), torch.tensor(wl / 2), atol=1e-6)
+    assert torch.allclose(pos[1, 0] - pos[0, 0], torch.tensor([0, 0, -wl / 2]), atol=1e-6)  # i axis = down
+    assert torch.allclose(pos[0, 1] - pos[0, 0], torch.tensor([0, wl / 2, 0]), atol=1e-6)  # j = k x i
+    ms, bh, n = Gm.gen_ms([10.0, 4.0, 1.5], [-1, 0, 0], [0, 0, -1], (32, 32), wl / 2)
+    assert torch.allclose(n, torch.tensor([-1.0, 0, 0]))
+    assert torch.allclose(ms.reshape(-1, 3)[:, 0], torch.tensor(9.999))
+    bh = np.array(bh)
+    assert bh.shape == (2, 3, 3) and np.allclose(bh[..., 0], 9.999)
+    assert np.isclose(bh[..., 1].max() - bh[..., 1].min(), 32 * wl / 2, atol=1e-6)
+    grid = Gm.gen_pos_from_endpoint([0.1, 9.9, 64], [0.1, 7.9, 64], 1.2)
+    assert grid.shape == (64, 64, 3) and torch.allclose(grid[..., 2], torch.tensor(1.2))
+    v, k, i = Gm.gen_sth_from_wall([[0, 0, 3], [0, 0, 0], [0, 4, 0]], (0.5, 0.25))
+    assert torch.allclose(v, torch.tensor([0.0, 1.0, 1.5])) and torch.allclose(k.abs(), torch.tensor([1.0, 0, 0]))
+    d = Gm.gen_direction_from_angle(0, 90, [0, 0, 1], [1, 0, 0])
+    assert torch.allclose(d, torch.tensor([0.0, 1.0, 0.0]), atol=1e-7) or \
+        torch.allclose(d, torch.tensor([0.0, -1.0, 0.0]), atol=1e-7)
+
+
+def test_seeded_scene_is_deterministic():
+    from automs_amd import scenes
+    a = scenes.room_boxes_scene(seed=0)
+    b = scenes.room_boxes_scene(seed=0)
+    c = scenes.room_boxes_scene(seed=1)
+    assert a.n_tri == 252 and np.array_equal(a.tri, b.tri) and not np.array_equal(a.tri, c.tri)
+    gam = a.mat_prop[:, 0]
+    assert gam.min() >= 0.3 and gam.max() <= 0.7 and (a.mat_prop == gam[:, None]).all()
+    # boxes rest on the floor, inside the room
+    boxes = a.tri[12:].reshape(20, 12, 3, 3)
+    assert np.allclose(boxes[..., 2].min((1, 2)), 0.0)
+    assert (boxes[..., 0] >= 0).all() and (boxes[..., 0] <= 10).all()
+    assert (boxes[..., 1] >= 0).all() and (boxes[..., 1] <= 8).all()
+
+
+def test_contraction_oracle_gradient_matches_finite_differences():
+    """Pins the oracle of the phase-map gradient (torch autograd on CPU,
+    complex128) with central differences h = 1e-5 on both objectives."""
+    from oracle import contract_ref as R
+    g = torch.Generator().manual_seed(0)
+
+    def crand(*s, scale):
+        return (torch.randn(*s, generator=g, dtype=torch.float64)
+                + 1j * torch.randn(*s, generator=g, dtype=torch.float64)) * scale
+    M, K, N, Cc = 6, 20, 40, 5
+    Hd, G, F = crand(M, N, scale=3e-5), crand(M, K, scale=3e-4), crand(K, N, scale=3e-4)
+    A = torch.ones(Cc, M)
+    ta = torch.rand(Cc, M, generator=g, dtype=torch.float64) * 2 * math.pi
+    tm = torch.rand(K, generator=g, dtype=torch.float64) * 2 * math.pi
+    mask = torch.tensor([False, True, False, False, False])
+    for name, fn in (("logsnr", lambda Y: R.loss_logsnr(Y, 1, 1e-9)),
+                     ("capacity", lambda Y: R.loss_capacity(Y, 1e-9, 0.5, mask))):
+        tmv = tm.clone().requires_grad_(True)
+        loss = fn(R.received(A, ta, Hd, [G], [F], [tmv], [2 * math.pi]))
+        loss.backward()
+        h = 1e-5
+        for k in (0, 7, 13):
+            p, m = tm.clone(), tm.clone()
+            p[k] += h
+            m[k] -= h
+            fd = (fn(R.received(A, ta, Hd, [G], [F], [p], [2 * math.pi])).item()
+                  - fn(R.received(A, ta, Hd, [G], [F], [m], [2 * math.pi])).item()) / (2 * h)
+            assert tmv.grad[k].item() == pytest.approx(fd, rel=1e-4, abs=1e-9), (name, k)

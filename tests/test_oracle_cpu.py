@@ -1,0 +1,240 @@
+"""Pins the FP64 oracle (oracle/ams_oracle.c) to closed forms, brute force
+and invariants; CPU only.
+
+Each check is chosen so that a plausible slip (dropped Fresnel term, wrong
+image sign, transposed index, wrong bin rounding) fails it:
+  * beam patterns: the four closed forms of optixPathTracer.cu:107-136,
+    continuity of type 2 at 5pi/12 and its jump at 2pi/3;
+  * Friis line of sight: H = lambda/(4 pi d) e^{-j 2 pi d / lambda};
+  * one mirror: |Gamma| lambda/(4 pi |I - r|) with the image I = mirror(s)
+    and the reflection phase;
+  * two parallel walls: every image order k <= K with amplitude
+    Gamma^k lambda / (4 pi L_k);
+  * penetrable door (mat_prop < 0): the transmission path keeps |value|;
+  * blackhole (0): a successful path of zero amplitude;
+  * Fraunhofer split: two small plates give
+    G_A G_B lambda/(4 pi d1) * lambda/(4 pi (d2 + d3));
+  * BVH closest hit = brute force closest hit on random scenes;
+  * a full oracle trace over a floor reproduces LOS + floor image at every Rx.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+WL = 3e8 / 60e9
+
+
+def quad(p0, p1, p2, p3):
+    """two triangles (p0, p1, p2), (p0, p2, p3)"""
+    return [list(p0) + list(p1) + list(p2), list(p0) + list(p2) + list(p3)]
+
+
+def make_scene(tris, mats, mat_vals, phases=None):
+    tri = np.asarray(tris, np.float32).reshape(-1, 9)
+    mp = np.repeat(np.asarray(mat_vals, np.float32)[:, None], 19, axis=1)
+    rp = None if phases is None else np.asarray(phases, np.float32)
+    return O.OracleScene(tri, np.asarray(mats, np.int32), mp, rp)
+
+
+def test_beam_pattern_closed_forms():
+    z = [0.0, 0.0, 1.0]
+    for th in np.linspace(0, np.pi, 37):
+        d = [math.sin(th), 0.0, math.cos(th)]
+        c = math.cos(th)
+        assert O.beam_pattern(d, z, 0) == 1.0
+        assert O.beam_pattern(d, z, 1) == pytest.approx(0.0 if c < 0 else 2.4495 * c, abs=1e-12)
+        assert O.beam_pattern(d, z, 3) == pytest.approx(0.0 if c < 0 else 1.4142, abs=1e-12)
+        if th > 2 * np.pi / 3:
+            exp2 = 0.1
+        elif th > 5 * np.pi / 12:
+            exp2 = 10 ** ((2 * np.pi / 3 - th) / np.pi * 4 * (math.log10(math.cos(5 * np.pi / 12)) + 1) - 1) * 2.4495
+        else:
+            exp2 = 2.4495 * c
+        assert O.beam_pattern(d, z, 2) == pytest.approx(exp2, rel=1e-12)
+    # type 2 is continuous at 5pi/12; at 2pi/3 the reference jumps from
+    # 0.1 * 2.4495 to 0.1 (its power law keeps the 2.4495 factor)
+    def bp2(th):
+        return O.beam_pattern([math.sin(th), 0, math.cos(th)], z, 2)
+    th0 = 5 * np.pi / 12
+    assert bp2(th0 - 1e-9) == pytest.approx(bp2(th0 + 1e-9), rel=1e-6)
+    th1 = 2 * np.pi / 3
+    assert bp2(th1 - 1e-9) == pytest.approx(0.24495, rel=1e-6)
+    assert bp2(th1 + 1e-9) == pytest.approx(0.1, rel=1e-12)
+
+
+def test_fibonacci_directions():
+    n = 1000
+    d = np.array([O.fib_dir(k, n) for k in range(n)])
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-12)
+    assert np.allclose(d[:, 2], 1 - (2 * np.arange(n) + 1) / n)
+    # near-uniform: mean direction ~ 0 and each octant holds ~1/8
+    assert np.abs(d.mean(0)).max() < 1e-2
+    oct_ = (d[:, 0] > 0) * 4 + (d[:, 1] > 0) * 2 + (d[:, 2] > 0)
+    counts = np.bincount(oct_, minlength=8)
+    assert counts.min() > 100 and counts.max() < 150
+
+
+def test_bvh_matches_brute_force():
+    rng = np.random.default_rng(0)
+    for trial in range(5):
+        T = 300
+        c = rng.uniform(-5, 5, (T, 1, 3))
+        tri = (c + rng.normal(0, 0.6, (T, 3, 3))).reshape(T, 9)
+        sc = O.OracleScene(tri, np.zeros(T, np.int32), np.full((1, 19), 0.5, np.float32))
+        for _ in range(300):
+            o = rng.uniform(-6, 6, 3)
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            a = sc.closest_hit(o, d, use_bvh=True)
+            b = sc.closest_hit(o, d, use_bvh=False)
+            assert a[0] == b[0]
+            if a[0] >= 0:
+                assert a[1] == pytest.approx(b[1], rel=1e-12)
+
+
+def _far_quad(z=0.0, half=50.0):
+    return quad((-half, -half, z), (half, -half, z), (half, half, z), (-half, half, z))
+
+
+def test_friis_line_of_sight():
+    # a scene whose only triangles are far away: the direct path is unobstructed
+    sc = make_scene(_far_quad(z=-100.0), [0, 0], [0.5])
+    s, r = np.array([0.0, 0.0, 1.0]), np.array([3.0, 4.0, 1.0])
+    ok, h, L = O.backward_one(sc, s, r, 0, 0, [], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    d = 5.0
+    assert ok and L == pytest.approx(d, rel=1e-12)
+    ref = WL / (4 * math.pi * d) * np.exp(-1j * 2 * math.pi * d / WL)
+    assert abs(h - ref) < 1e-9 * abs(ref)
+
+
+@pytest.mark.parametrize("phase", [0.0, math.pi])
+def test_single_mirror(phase):
+    sc = make_scene(_far_quad(z=0.0), [0, 0], [0.5], [phase])
+    plane = int(sc.plane_ids()[0])
+    assert plane == 0
+    s, r = np.array([0.0, 0.0, 1.0]), np.array([2.0, 0.0, 1.5])
+    I = np.array([0.0, 0.0, -1.0])
+    L = np.linalg.norm(I - r)
+    ok, h, Lo = O.backward_one(sc, s, r, 1, 1, [plane], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert ok and Lo == pytest.approx(L, rel=1e-12)
+    ph32 = float(np.float32(phase))  # the material table is f32
+    ref = 0.5 * WL / (4 * math.pi * L) * np.exp(-1j * (2 * math.pi * L / WL - ph32))
+    assert abs(h - ref) < 1e-9 * abs(ref)
+    # the direct path is unobstructed too
+    ok0, _, _ = O.backward_one(sc, s, r, 0, 0, [], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert ok0
+    # a plate at z = 0.5 under the floor's specular point (x = 1.2): the ray
+    # meets another plane first, so the floor path fails; the plate's own
+    # specular point (x = 2/3) is off the plate, so that path fails too
+    plate = quad((1.0, -0.5, 0.5), (2.0, -0.5, 0.5), (2.0, 0.5, 0.5), (1.0, 0.5, 0.5))
+    sc2 = make_scene(_far_quad(z=0.0) + plate, [0, 0, 1, 1], [0.5, 0.3])
+    okw, _, _ = O.backward_one(sc2, s, r, 1, 1, [0], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert not okw
+    okp, hp, Lp = O.backward_one(sc2, s, r, 1, 1, [2], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert not okp
+
+
+def test_two_parallel_walls_image_series():
+    """walls z=0 (Gamma 0.6) and z=3 (Gamma 0.4): images alternate."""
+    sc = make_scene(_far_quad(z=0.0) + _far_quad(z=3.0), [0, 0, 1, 1], [0.6, 0.4])
+    p_lo, p_hi = 0, 2
+    s, r = np.array([0.0, 0.0, 1.0]), np.array([4.0, 1.0, 2.0])
+    for first, seq_gamma in ((p_lo, None), (p_hi, None)):
+        planes = []
+        I = s.copy()
+        gam = 1.0
+        for k in range(1, 5):
+            pl = first if (k % 2 == 1) else (p_hi if first == p_lo else p_lo)
+            planes.append(pl)
+            z0 = 0.0 if pl == p_lo else 3.0
+            I = I.copy()
+            I[2] = 2 * z0 - I[2]
+            gam *= float(np.float32(0.6 if pl == p_lo else 0.4))  # f32 material table
+            L = np.linalg.norm(I - r)
+            ok, h, Lo = O.backward_one(sc, s, r, k, (1 << k) - 1, planes, WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+            assert ok, (first, k)
+            assert Lo == pytest.approx(L, rel=1e-11)
+            ref = gam * WL / (4 * math.pi * L) * np.exp(-1j * 2 * math.pi * L / WL)
+            assert abs(h - ref) < 1e-8 * abs(ref), (first, k)
+
+
+def test_penetrable_door_and_blackhole():
+    s, r = np.array([0.0, 0.0, 1.0]), np.array([4.0, 0.0, 1.0])
+    door = quad((2.0, -1.0, 0.0), (2.0, 1.0, 0.0), (2.0, 1.0, 2.0), (2.0, -1.0, 2.0))
+    sc = make_scene(door, [0, 0], [-0.7])
+    # the direct path is blocked by the door as an unplanned interaction...
+    ok0, _, _ = O.backward_one(sc, s, r, 0, 0, [], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert not ok0
+    # ...and succeeds as a one-interaction transmission (b = 0) keeping |-0.7|
+    ok, h, L = O.backward_one(sc, s, r, 1, 0, [], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert ok and L == pytest.approx(4.0)
+    ref = float(np.float32(0.7)) * WL / (4 * math.pi * 4.0) * np.exp(-1j * 2 * math.pi * 4.0 / WL)
+    assert abs(h - ref) < 1e-9 * abs(ref)
+    # a blackhole (0) plate in the mirror position: success, zero amplitude
+    bh = make_scene(_far_quad(z=0.0), [0, 0], [0.0])
+    ok, h, _ = O.backward_one(bh, s, np.array([2.0, 0.0, 1.5]), 1, 1, [0], WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert ok and h == 0
+
+
+def test_fraunhofer_split():
+    """Two 4 cm plates far apart: the far-field split of
+    optixPathTracer.cu:369-391 multiplies the two legs' spreading losses."""
+    e = 0.04
+    # plate A in plane x = 0 around (0, 0, 1); plate B in plane y = 5 around (3, 5, 1)
+    A = quad((0, -e / 2, 1 - e / 2), (0, e / 2, 1 - e / 2), (0, e / 2, 1 + e / 2), (0, -e / 2, 1 + e / 2))
+    B = quad((3 - e / 2, 5, 1 - e / 2), (3 + e / 2, 5, 1 - e / 2), (3 + e / 2, 5, 1 + e / 2), (3 - e / 2, 5, 1 + e / 2))
+    sc = make_scene(A + B, [0, 0, 1, 1], [0.9, 0.8])
+    # forward: Tx -> B -> A -> Rx.  choose points so both specular points are plate centres
+    pA, pB = np.array([0.0, 0.0, 1.0]), np.array([3.0, 5.0, 1.0])
+    # Rx on the mirror ray of (pB -> pA) about x = 0; Tx on the mirror ray of (pA -> pB) about y = 5
+    dAB = (pA - pB) / np.linalg.norm(pA - pB)
+    dA_out = dAB * np.array([-1, 1, 1])
+    r = pA + 2.0 * dA_out
+    dB_in = dAB * np.array([1, -1, 1])  # incoming direction at B: the B -> A leg mirrored about y = 5
+    s = pB - 3.0 * dB_in
+    d1, d2, d3 = 2.0, np.linalg.norm(pA - pB), 3.0
+    planes = [2, 0]  # forward order: B first, then A
+    ok, h, L = O.backward_one(sc, s, r, 2, 3, planes, WL, 0.5e-5, [1, 0, 0], [0, 0, 0], 0, 0)
+    assert ok and L == pytest.approx(d1 + d2 + d3, rel=1e-9)
+    fk = 2 * (e * math.sqrt(2)) ** 2 / WL  # 2 maxedge^2 / lambda of the plates
+    assert d1 > fk and d2 > fk  # the split happens at the second interaction
+    ref = float(np.float32(0.9)) * float(np.float32(0.8)) * WL / (4 * math.pi * d1) * WL / (4 * math.pi * (d2 + d3))
+    assert abs(abs(h) - ref) < 1e-9 * ref
+
+
+def test_full_trace_floor_los_and_image():
+    """Discovery + evaluation over an (effectively infinite) floor: every Rx
+    gets exactly LOS + one floor image."""
+    sc = make_scene(_far_quad(z=0.0), [0, 0], [0.5], [math.pi])
+    src = np.array([[[0.0, 0.0, 2.0]]], np.float32)
+    xs = np.linspace(1.0, 4.0, 6)
+    dst = np.stack(np.meshgrid(xs, np.linspace(-1.5, 1.5, 5), indexing="ij"), -1)
+    dst = np.concatenate([dst, np.full(dst.shape[:2] + (1,), 1.0)], -1).astype(np.float32)
+    H, st, paths = O.trace_channel(sc, src, dst, WL, 2, 0.5e-5, 0.06, [0, 0, 1], [0, 0, 0], 0, 0, 5, 1 << 16, True)
+    assert st["paths"] >= 2
+    s = src.reshape(3).astype(np.float64)
+    for j, r in enumerate(dst.reshape(-1, 3).astype(np.float64)):
+        d0 = np.linalg.norm(r - s)
+        d1 = np.linalg.norm(r - s * np.array([1, 1, -1]))
+        ref = WL / (4 * math.pi * d0) * np.exp(-1j * 2 * math.pi * d0 / WL) \
+            + 0.5 * WL / (4 * math.pi * d1) * np.exp(-1j * (2 * math.pi * d1 / WL - float(np.float32(math.pi))))
+        assert abs(H[0, j] - ref) < 1e-9 * abs(ref), j
+
+
+def test_discovery_is_deterministic_and_sorted():
+    from automs_amd import scenes
+    w = scenes.room_boxes(fib_n=1 << 8, n_rx=(16, 16))
+    sc = O.OracleScene.from_scene(w.scene)
+    k1, s1 = O.discover(sc, w.tx.pos, w.rx.pos, w.rx.radius, w.max_reflection, w.gridsize, w.fib_n, True)
+    k2, s2 = O.discover(sc, w.tx.pos, w.rx.pos, w.rx.radius, w.max_reflection, w.gridsize, w.fib_n, True)
+    assert np.array_equal(k1, k2)
+    keys = [(int(a), int(b)) for a, b in k1]
+    assert keys == sorted(keys) and len(set(keys)) == len(keys)
+    for hi, lo in keys[:200]:
+        h, b, pl, tile = O.key_fields(hi, lo)
+        assert 0 <= h <= w.max_reflection
+        assert b == (1 << h) - 1  # no penetrable material: every interaction reflects
+        assert all(p != 0xFFFFF for p in pl[:h]) and all(p == 0xFFFFF for p in pl[h:])
