@@ -249,6 +249,9 @@ class PhaseMapModel(nn.Module):
 
     def forward(self) -> torch.Tensor:
         Y = self.received()
+        with torch.no_grad():  # coverage side output: mean over Rx of best-code power (dB)
+            p = (Y.real ** 2 + Y.imag ** 2).reshape(self.n_maps, self.code_num, -1).amax(1)
+            self.last_cov = (10 * torch.log10(p + 1e-30)).mean(1)
         rowmask = None
         if self.mode == 1:
             # go.py:220: rows of cap zeroed where torch.rand(C) < 0.1 (CPU generator)
@@ -329,3 +332,58 @@ def optimize_model(model: PhaseMapModel, lr: float = 0.3, step: int = 200, fused
     if record and hist:
         model.loss_list.extend(torch.stack(hist).cpu().tolist())
     return model
+
+
+def channels_from_traces(H_d: torch.Tensor, G: torch.Tensor, F: torch.Tensor) -> Dict[str, torch.Tensor]:
+    """The channel dictionary of build_channels for one metasurface
+    (go.py:1172-1224 key names)."""
+    return {"src_to_dst": H_d, "src_to_ms{0}": G, "ms{0}_to_dst": F}
+
+
+def time_to_target(channels: Dict[str, torch.Tensor], ms: Metasurface, steps: int = 500, lr: float = 0.3,
+                   code_num: int = 30, seed: int = 0, tol_db: float = 0.1) -> dict:
+    """Optimise the phase map twice from the same start: plain FP32 (vector
+    ALU contraction, torch.optim.Adam) and the split-bf16 MFMA path with the
+    fused Adam.  Returns the plain optimum's coverage, and the wall seconds the
+    MFMA path needs to come within tol_db of it (step events on the stream)."""
+    K = channels["src_to_ms{0}"].shape[1]
+    M = channels["src_to_dst"].shape[0]
+    g = torch.Generator().manual_seed(seed)
+    th_ms = torch.rand(1, K, generator=g) * 2 * math.pi
+    th_ap = torch.rand(code_num, M, generator=g) * 2 * math.pi
+    out = {}
+    for name, plain in (("plain_fp32", True), ("mfma_split_bf16", False)):
+        model = PhaseMapModel(name, channels, [ms], code_num=code_num, ms_theta=[th_ms], antenna_theta=th_ap,
+                              plain=plain)
+        covs, evs = [], []
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+
+        def cb(it, m):
+            covs.append(m.last_cov.detach().clone())
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            evs.append(e)
+        optimize_model(model, lr=lr, step=steps, fused=not plain, callback=cb)
+        torch.cuda.synchronize()
+        cov = torch.stack(covs).reshape(-1).cpu()
+        t = [e0.elapsed_time(e) / 1e3 for e in evs]
+        best_loss_step = int(torch.tensor(model.loss_list).argmin())
+        out[name] = {"cov_db": cov, "t": t, "best_cov_db": float(cov[best_loss_step]),
+                     "final_loss": float(min(model.loss_list)), "seconds": t[-1]}
+    target = out["plain_fp32"]["best_cov_db"]
+    cov = out["mfma_split_bf16"]["cov_db"]
+    hit = (cov >= target - tol_db).nonzero()
+    k = int(hit[0]) if len(hit) else None
+    return {
+        "plain_optimum_cov_db": target,
+        "plain_seconds_500": out["plain_fp32"]["seconds"],
+        "mfma_seconds_500": out["mfma_split_bf16"]["seconds"],
+        "mfma_best_cov_db": out["mfma_split_bf16"]["best_cov_db"],
+        "mfma_steps_to_target": None if k is None else k + 1,
+        "seconds_to_target": None if k is None else out["mfma_split_bf16"]["t"][k],
+        "start_cov_db": float(cov[0]),
+        "final_loss_plain": out["plain_fp32"]["final_loss"],
+        "final_loss_mfma": out["mfma_split_bf16"]["final_loss"],
+    }

@@ -51,7 +51,7 @@ def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int) -> dict:
     if name == "room_boxes":
         traces = {"H": scenes.room_boxes(fib_n=sample_fib)}
     else:
-        traces = scenes.room_boxes_ms(fib_n=sample_fib, ms_fib_n=sample_ms_fib)
+        traces = scenes.room_boxes_ms(fib_n=sample_fib, ms_fib_n=sample_ms_fib, ms_size=(16, 16))
     q = 0
     t0 = time.perf_counter()
     for w in traces.values():
@@ -59,10 +59,11 @@ def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int) -> dict:
         q += st["fwd_queries"] + st["bwd_queries"]
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    what = f"{name}: every trace with {sample_fib} Fibonacci rays per AP element"
+    if name != "room_boxes":
+        what += f", a 16 x 16 metasurface and {sample_ms_fib} rays in total from it"
     return {"value": q / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"{name}: every trace with {sample_fib} Fibonacci rays per AP element"
-                      f"{'' if name == 'room_boxes' else f' and {sample_ms_fib} rays in total from the metasurface'}"
-                      f" ({q} closest-hit queries in {dt:.1f} s)"}
+            "sample": f"{what} ({q} closest-hit queries in {dt:.1f} s; FP64 oracle, OpenMP)"}
 
 
 def main():
@@ -70,9 +71,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="room_boxes", choices=["room_boxes", "room_boxes_ms"])
+    ap.add_argument("--workload", default="room_boxes_ms", choices=["room_boxes", "room_boxes_ms"])
+    ap.add_argument("--opt-steps", type=int, default=500)
+    ap.add_argument("--no-opt", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-fib", type=int, default=1 << 12)
+    ap.add_argument("--cpu-sample-fib", type=int, default=1 << 16)
     ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -156,6 +159,22 @@ def main():
             "algorithmic_bytes_per_launch": kb[dom] / max(1, kl[dom]),
         },
     }
+    if args.workload == "room_boxes_ms" and not args.no_opt:
+        from automs_amd.phasemap import Metasurface, channels_from_traces, time_to_target
+
+        def cplx(k):
+            return torch.complex(outs[k][..., 0], outs[k][..., 1])
+        ch = channels_from_traces(cplx("H_d"), cplx("G"), cplx("F"))
+        ms = Metasurface(None, traces["G"].ms[0]["size"], None, None, [], max_phase=360.0)
+        r = time_to_target(ch, ms, steps=args.opt_steps)
+        print(f"phase map: {r}", file=sys.stderr, flush=True)
+        line["phase_map"] = {
+            "metric": "seconds to within 0.1 dB of the plain FP32 optimum (mean best-code Rx power)",
+            "seconds_to_0p1dB": r["seconds_to_target"], "steps_to_0p1dB": r["mfma_steps_to_target"],
+            "adam_steps": args.opt_steps, "mfma_seconds_all_steps": r["mfma_seconds_500"],
+            "plain_fp32_seconds_all_steps": r["plain_seconds_500"],
+            "plain_optimum_cov_db": r["plain_optimum_cov_db"], "mfma_best_cov_db": r["mfma_best_cov_db"],
+            "start_cov_db": r["start_cov_db"], "codes": 30, "objective": "mean log2(1+SNR), N0 = -90 dBm"}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_fib, args.cpu_sample_ms_fib)
     print(json.dumps(line), flush=True)
