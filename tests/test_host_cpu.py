@@ -189,3 +189,35 @@ def test_contraction_oracle_gradient_matches_finite_differences():
             fd = (fn(R.received(A, ta, Hd, [G], [F], [p], [2 * math.pi])).item()
                   - fn(R.received(A, ta, Hd, [G], [F], [m], [2 * math.pi])).item()) / (2 * h)
             assert tmv.grad[k].item() == pytest.approx(fd, rel=1e-4, abs=1e-9), (name, k)
+
+
+def test_scene_folder_loader_4321():
+    """The reference's 4321 scene (data fixture): 2,689 triangles; material
+    from the file-name suffix; the 19-bin table; synthetic materials appended
+    in the order blackhole, reflector, door (go.py:679-737)."""
+    from automs_amd import channel
+    sc = channel.load_scene_folder("tests/golden/scene_4321")
+    assert sc.n_tri == 2689
+    assert sc.names == ["screen", "wood", "carpet", "wall3", "wall", "glass"]
+    assert np.bincount(sc.mat).tolist() == [780, 1740, 12, 44, 111, 2]
+    assert sc.mat_prop.shape == (6, 19) and sc.mat_prop[4, 0] == pytest.approx(0.7)
+    bh = [[[0, 0, 0], [1, 0, 0], [0, 1, 0]]]
+    door = [[[0, 0, 1], [1, 0, 1], [0, 1, 1]]]
+    sc2 = channel.load_scene_folder("tests/golden/scene_4321", blackhole_buffer=[bh], door_buffer=[door])
+    assert sc2.names[-2:] == ["blackhole", "door"] and sc2.n_tri == 2691
+    assert sc2.mat_prop[-2, 0] == 0.0 and sc2.mat_prop[-1, 0] == pytest.approx(-0.7)
+    walls = channel.get_walls("tests/golden/scene_4321")
+    assert len(walls) == 3 and walls[0][0] == [-3.9151, 0.3959, 1.6322]
+
+
+def test_torch_op_schema_registered():
+    import torch
+
+    from automs_amd import channel
+    channel.register_torch_op()
+    op = torch.ops.pathtrace.torch_launch_pathtrace
+    names = [a.name for a in op.default._schema.arguments]
+    assert names == ["result", "src", "dst", "src_aabb_buffer", "dst_aabb_buffer", "scene_triangle_buffer", "mat",
+                     "samples_per_launch", "launch_x", "launch_y", "wave_len", "max_reflection", "reserve_size",
+                     "mat_prop", "src_dir", "dst_dir", "optix_log_level", "src_bptype", "dst_bptype",
+                     "dst_girdsize", "src_r", "dst_r"]

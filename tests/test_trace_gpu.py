@@ -118,3 +118,44 @@ def test_end_to_end_coverage(small_room, oracle_scene, trace_scene):
     _flush(f"end-to-end: paths o={ost['paths']} g={st['paths']} tasks o={ost['tasks']} g={st['tasks']} "
            f"H rel={rel:.3e} dB>0.05: {(ddb > 0.05).sum()} of {m.sum()}")
     assert (ddb > 0.05).sum() <= max(5, int(0.01 * m.sum()))
+
+
+def test_torch_op_scene_4321_matches_oracle(gpu):
+    """The drop-in op torch.ops.pathtrace.torch_launch_pathtrace on the
+    reference's own 4321 scene and go.py's __main__ geometry (6x6 AP at
+    (-3, 3.4, -0.51) facing -x, 84 x 75 Rx at z = -0.51, 4 reflections,
+    100 x 100 launch), against the oracle on the same inputs."""
+    import torch
+
+    from automs_amd import channel, geometry
+    from automs_amd.scenes import ArraySpec, Workload
+    from oracle import oracle as O
+    _flush("case: torch op scene 4321")
+    channel.register_torch_op()
+    wl = 3e8 / 60e9
+    sc = channel.load_scene_folder("tests/golden/scene_4321")
+    src = geometry.gen_pos_matrix([-3, 3.4, -0.51], [-1, 0, 0], [0, 0, -1], (6, 6), wl / 2)
+    dst = geometry.gen_pos_from_endpoint([-6, 2.4, 84], [-1, 6.5, 75], -0.51)
+    result = torch.zeros(36, 84 * 75, 2, dtype=torch.float32, device=gpu)
+    tri = torch.tensor(sc.tri.reshape(-1, 3), device=gpu)
+    torch.ops.pathtrace.torch_launch_pathtrace(
+        result, src.to(gpu), dst.to(gpu), torch.cat([src - 0.5e-5, src + 0.5e-5], -1).to(gpu),
+        torch.cat([dst - 0.06, dst + 0.06], -1).to(gpu), tri, torch.tensor(sc.mat, device=gpu), 1, 100, 100, wl, 4,
+        100, torch.tensor(sc.mat_prop, device=gpu), torch.tensor([-1.0, 0, 0]), torch.tensor([0.0, 0, 0]), 0, 2, 0, 5,
+        0.5e-5, 0.06)
+    torch.cuda.synchronize()
+    Hg = result.double().cpu().numpy()
+    Hg = Hg[..., 0] + 1j * Hg[..., 1]
+    w = Workload("4321", sc, ArraySpec(src.numpy(), np.array([-1, 0, 0], np.float32), 2, 0.5e-5),
+                 ArraySpec(dst.numpy(), np.zeros(3, np.float32), 0, 0.06), wl, 4, 100 * 100, False)
+    Ho, ost, _ = O.trace_workload(w)
+    rel = np.linalg.norm(Hg - Ho) / np.linalg.norm(Ho)
+    co = 20 * np.log10(np.abs(Ho.sum(0)) + 1e-300)
+    cg = 20 * np.log10(np.abs(Hg.sum(0)) + 1e-300)
+    m = co > -120
+    ddb = np.abs(co - cg)[m]
+    _flush(f"  oracle paths={ost['paths']} tasks={ost['tasks']} ties={ost['bwd_ties']} H rel={rel:.3e} "
+           f"dB>0.05: {(ddb > 0.05).sum()} of {m.sum()}")
+    assert ost["paths"] > 10
+    assert rel < 1e-3
+    assert (ddb > 0.05).sum() <= max(5, int(0.01 * m.sum()))
