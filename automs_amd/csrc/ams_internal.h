@@ -45,6 +45,13 @@ struct Scratch {
     if (!b.first) set_error(std::string("device allocation failed: ") + name);
     return b.first;
   }
+  // take ownership of an allocation made with dmalloc under this name
+  void adopt(const char *name, void *p, size_t bytes) {
+    auto &b = bufs[name];
+    if (b.first && b.first != p) dfree(b.first);
+    b.first = p;
+    b.second = bytes;
+  }
   void release() {
     for (auto &kv : bufs)
       if (kv.second.first) dfree(kv.second.first);
@@ -65,6 +72,7 @@ struct ams_scene {
   float *d_mat_prop = nullptr;
   float *d_refl_phase = nullptr;
   ams::Scratch scratch;
+  long long perm_n = -1;  // fib_n of the cached ray permutation in scratch "perm"
 };
 
 namespace ams {

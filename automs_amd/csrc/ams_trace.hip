@@ -285,6 +285,60 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const
 // ------------------------------------------------------------------------
 // forward discovery
 // ------------------------------------------------------------------------
+// Fibonacci direction k of n in FP64 with the azimuth range-reduced in
+// cycles (the same formula the oracle uses).
+__device__ __forceinline__ void fib_dir(long long k, long long n, double &x, double &y, double &z) {
+  const double g = 0.38196601125010515179541316563436;  // (3 - sqrt 5) / 2
+  z = 1.0 - (2.0 * (double)k + 1.0) / (double)n;
+  double rr = sqrt(fmax(0.0, 1.0 - z * z));
+  double c = (double)k * g;
+  double f = c - floor(c);
+  double sn, cs;
+  sincos(2.0 * M_PI * f, &sn, &cs);
+  x = rr * cs;
+  y = rr * sn;
+}
+
+// Hilbert index of (x, y) on a 2^16 grid.
+__device__ __forceinline__ unsigned hilbert16(unsigned x, unsigned y) {
+  unsigned d = 0;
+  for (unsigned s = 1u << 15; s > 0; s >>= 1) {
+    unsigned rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = s - 1 - x;
+        y = s - 1 - y;
+      }
+      unsigned t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
+}
+
+// Sort key of every direction: Hilbert order of its octahedral map, so that
+// a wave's lanes get neighbouring directions (coherent traversal).
+__global__ void k_fib_keys(long long n, unsigned *__restrict__ keys, unsigned *__restrict__ idx) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double x, y, z;
+  fib_dir(k, n, x, y, z);
+  double l1 = fabs(x) + fabs(y) + fabs(z);
+  double u = x / l1, v = y / l1;
+  if (z < 0) {
+    double uu = (1.0 - fabs(v)) * (u >= 0 ? 1.0 : -1.0);
+    double vv = (1.0 - fabs(u)) * (v >= 0 ? 1.0 : -1.0);
+    u = uu;
+    v = vv;
+  }
+  unsigned qx = (unsigned)fmin(65535.0, fmax(0.0, (u * 0.5 + 0.5) * 65536.0));
+  unsigned qy = (unsigned)fmin(65535.0, fmax(0.0, (v * 0.5 + 0.5) * 65536.0));
+  keys[k] = hilbert16(qx, qy);
+  idx[k] = (unsigned)k;
+}
+
 struct DiscoverArgs {
   const AmsNode *tnodes;
   int n_tnodes;
@@ -302,6 +356,7 @@ struct DiscoverArgs {
   long long fib_n;
   int per_element;
   long long ray_begin, ray_end;
+  const unsigned *perm;  // slot -> direction (per-element) or ray (distributed) index
   uint64_t *out_hi, *out_lo;
   unsigned long long *out_count;
   unsigned long long cap;
@@ -322,21 +377,22 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
   bool active = r < a.ray_end;
   unsigned long long nrec = 0;
   if (active) {
-    long long e = a.per_element ? r / a.fib_n : r % a.n_src;
-    long long k = a.per_element ? r % a.fib_n : r;
-    // Fibonacci direction in FP64, the azimuth range-reduced in cycles
-    const double g = 0.38196601125010515179541316563436;
-    double z = 1.0 - (2.0 * (double)k + 1.0) / (double)a.fib_n;
-    double rr = sqrt(fmax(0.0, 1.0 - z * z));
-    double x = (double)k * g;
-    double f = x - floor(x);
-    double sn, cs;
-    sincos(2.0 * M_PI * f, &sn, &cs);
+    long long e, k;
+    if (a.per_element) {
+      e = r / a.fib_n;
+      k = a.perm[r % a.fib_n];
+    } else {
+      long long rr2 = a.perm[r];
+      e = rr2 % a.n_src;
+      k = rr2;
+    }
+    double dx, dy, dz;
+    fib_dir(k, a.fib_n, dx, dy, dz);
     Branch st[AMS_MAX_HIT];
     int sp = 0;
     Branch cur;
     cur.o = f3(a.src[3 * e], a.src[3 * e + 1], a.src[3 * e + 2]);
-    cur.d = f3((float)(rr * cs), (float)(rr * sn), (float)z);
+    cur.d = f3((float)dx, (float)dy, (float)dz);
     cur.ttl = a.K + 1;
     cur.b = 0;
     cur.nref = 0;
@@ -489,18 +545,36 @@ __global__ void __launch_bounds__(512) k_evaluate(EvalArgs a) {
       int ttl = h + 1, ri = 0, last_pl = -1;
       nchain++;
       while (ttl > 0) {
-        Ray ray;
-        ray_setup(ray, o, d);
-        float tt = INFINITY;
-        int torig;
-        int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
-        ct.queries++;
         // own Tx sphere (other Tx spheres are pass-through: optixPathTracer.cu:396)
         float3 O = s3 - o;
         float bs = dot3(O, d);
         float3 Pp = O - d * bs;
         bool sh = dot3(Pp, Pp) < a.src_acc2 && bs >= AMS_TMIN && dot3(O, O) > a.src_r2;
-        if (sh && bs < tt) {
+        // Bound the query by what the path allows: on the last leg nothing may
+        // lie before the Tx; before a reflection the hit must lie on the key's
+        // plane, so anything beyond that plane cannot be the interaction.
+        // Both bounds leave the outcome of the unbounded query unchanged.
+        float tlim = INFINITY;
+        if (ttl == 1) {
+          if (!sh) break;  // the ray passes the Tx by: no success is possible
+          tlim = bs;
+        } else if ((bm >> (h + 1 - ttl)) & 1) {
+          if (ri >= nref) break;
+          const AmsPlane P = a.planes[key_get_plane(khi, klo, nref - 1 - ri)];
+          float3 pn = f3((float)P.n[0], (float)P.n[1], (float)P.n[2]);
+          float nd = dot3(pn, d);
+          float tp = nd != 0.0f ? ((float)P.d - dot3(pn, o)) / nd : -1.0f;
+          if (!(tp > 0.0f)) break;  // the plane is not ahead
+          tlim = tp * 1.001f + 1e-3f;
+        }
+        Ray ray;
+        ray_setup(ray, o, d);
+        float tt = tlim;
+        int torig;
+        int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
+        ct.queries++;
+        if (slot < 0) tt = INFINITY;
+        if (sh && bs < tt && bs <= tlim) {
           if (ttl == 1) {
             // optixPathTracer.cu:398-406
             D += bs;
@@ -790,6 +864,23 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   da.out_count = d_cnt;
   da.cap = cap;
   da.counters = d_ctr;
+  // coherent ray order: a permutation of the Fibonacci indices, cached per fib_n
+  if (!o->seed_keys && s->perm_n != o->fib_n) {
+    long long n = o->fib_n;
+    unsigned *keys = (unsigned *)sc.get("perm_keys", 4 * (size_t)n), *idx = (unsigned *)sc.get("perm_idx", 4 * (size_t)n);
+    unsigned *k2 = (unsigned *)sc.get("perm_k2", 4 * (size_t)n), *perm = (unsigned *)sc.get("perm", 4 * (size_t)n);
+    if (!keys || !idx || !k2 || !perm) return AMS_ERR_ALLOC;
+    k_fib_keys<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, keys, idx);
+    HIPCHK(hipGetLastError());
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, k2, idx, perm, (int)n, 0, 32, st));
+    void *tmp = sc.get("perm_tmp", tb);
+    if (!tmp) return AMS_ERR_ALLOC;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, k2, idx, perm, (int)n, 0, 32, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s->perm_n = n;
+  }
+  da.perm = (const unsigned *)sc.get("perm", 4 * (size_t)o->fib_n);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -823,8 +914,16 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
       }
     }
   }
+  // Running unique set on the device (acc_hi/acc_lo); each chunk's records
+  // are sorted and deduplicated, appended, and the set is compacted when it
+  // outgrows its buffer.  A chunk whose records overflow is split and redone
+  // (the kernel is deterministic, so the split gives the same records).
+  size_t acc_n = 0, acc_cap = 1 << 20;
+  uint64_t *a_hi = (uint64_t *)sc.get("acc_hi", acc_cap * 8), *a_lo = (uint64_t *)sc.get("acc_lo", acc_cap * 8);
+  if (!a_hi || !a_lo) return AMS_ERR_ALLOC;
+  long long cur_chunk = chunk;
   for (long long rb = o->seed_keys ? n_rays : 0; rb < n_rays;) {
-    long long re = std::min(n_rays, rb + chunk);
+    long long re = std::min(n_rays, rb + cur_chunk);
     HIPCHK(hipMemsetAsync(d_cnt, 0, 8, st));
     da.ray_begin = rb;
     da.ray_end = re;
@@ -841,38 +940,79 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     disc_ms += ms;
     if (hcnt > cap) {
-      set_error("forward records exceed the per-chunk buffer");
-      S.overflow += (long long)(hcnt - cap);
-      return AMS_ERR_OVERFLOW;
+      if (re - rb <= 1) {
+        set_error("forward records of a single ray exceed the record buffer");
+        return AMS_ERR_OVERFLOW;
+      }
+      S.overflow++;
+      cur_chunk = std::max<long long>(1, (re - rb) / 2);
+      continue;  // redo [rb, rb + cur_chunk)
     }
     size_t nu = 0;
     int rc = sort_unique(sc, c_hi, c_lo, (size_t)hcnt, st, &nu);
     if (rc) return rc;
-    size_t old = uh.size();
-    uh.resize(old + nu);
-    ul.resize(old + nu);
-    HIPCHK(hipMemcpyAsync(uh.data() + old, c_hi, nu * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(ul.data() + old, c_lo, nu * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
     S.records_raw += (long long)hcnt;
+    if (acc_n + nu > acc_cap) {
+      // compact the running set, then grow its buffers if still needed
+      size_t m = 0;
+      rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m);
+      if (rc) return rc;
+      acc_n = m;
+      if (acc_n + nu > acc_cap) {
+        size_t ncap = std::max(2 * acc_cap, acc_n + nu);
+        uint64_t *nh = (uint64_t *)dmalloc(ncap * 8), *nl = (uint64_t *)dmalloc(ncap * 8);
+        if (!nh || !nl) {
+          dfree(nh);
+          dfree(nl);
+          set_error("device allocation failed: record set");
+          return AMS_ERR_ALLOC;
+        }
+        HIPCHK(hipMemcpyAsync(nh, a_hi, acc_n * 8, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(nl, a_lo, acc_n * 8, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        sc.adopt("acc_hi", nh, ncap * 8);
+        sc.adopt("acc_lo", nl, ncap * 8);
+        a_hi = nh;
+        a_lo = nl;
+        acc_cap = ncap;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(a_hi + acc_n, c_hi, nu * 8, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(a_lo + acc_n, c_lo, nu * 8, hipMemcpyDeviceToDevice, st));
+    acc_n += nu;
     rb = re;
+    cur_chunk = chunk;
   }
-  // merge chunk-unique sets on the host: sort (hi, lo) and unique
-  {
-    std::vector<size_t> ix(uh.size());
+  if (o->seed_keys) {
+    acc_hi.swap(uh);
+    acc_lo.swap(ul);
+    // seeds: sort + unique on the host (small)
+    std::vector<size_t> ix(acc_hi.size());
     for (size_t k = 0; k < ix.size(); k++) ix[k] = k;
     std::sort(ix.begin(), ix.end(), [&](size_t a0, size_t b0) {
-      return uh[a0] != uh[b0] ? uh[a0] < uh[b0] : ul[a0] < ul[b0];
+      return acc_hi[a0] != acc_hi[b0] ? acc_hi[a0] < acc_hi[b0] : acc_lo[a0] < acc_lo[b0];
     });
-    acc_hi.reserve(ix.size());
-    acc_lo.reserve(ix.size());
+    std::vector<uint64_t> h2, l2;
     for (size_t k : ix) {
-      if (!acc_hi.empty() && acc_hi.back() == uh[k] && acc_lo.back() == ul[k]) continue;
-      acc_hi.push_back(uh[k]);
-      acc_lo.push_back(ul[k]);
+      if (!h2.empty() && h2.back() == acc_hi[k] && l2.back() == acc_lo[k]) continue;
+      h2.push_back(acc_hi[k]);
+      l2.push_back(acc_lo[k]);
     }
-    n_unique_total = acc_hi.size();
+    acc_hi.swap(h2);
+    acc_lo.swap(l2);
+  } else {
+    size_t m = 0;
+    int rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m);
+    if (rc) return rc;
+    acc_hi.resize(m);
+    acc_lo.resize(m);
+    if (m) {
+      HIPCHK(hipMemcpyAsync(acc_hi.data(), a_hi, m * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(acc_lo.data(), a_lo, m * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
   }
+  n_unique_total = acc_hi.size();
   S.rays = n_rays;
   S.records_unique = (long long)n_unique_total;
   S.t_discover_ms = disc_ms;

@@ -164,3 +164,127 @@ def room_boxes_ms(seed: int = 0, fib_n: int = 1 << 20, ms_fib_n: int = 1 << 24, 
                         rx=base.rx, wave_len=wl, max_reflection=max_reflection, fib_n=ms_fib_n,
                         per_element=False, ms=[ms])
     return out
+
+
+def _tess_quad(origin, u, v, nu: int, nv: int) -> np.ndarray:
+    """Planar rectangle origin + s u + t v (s, t in [0, 1]) as nu x nv cells of
+    two triangles each: [2 nu nv, 9] f32 (all exactly coplanar)."""
+    o, u, v = (np.asarray(x, np.float64) for x in (origin, u, v))
+    s = np.arange(nu + 1) / nu
+    t = np.arange(nv + 1) / nv
+    P = o[None, None, :] + s[:, None, None] * u[None, None, :] + t[None, :, None] * v[None, None, :]
+    a, b = P[:-1, :-1], P[1:, :-1]
+    c, d = P[1:, 1:], P[:-1, 1:]
+    t1 = np.concatenate([a, b, c], -1).reshape(-1, 9)
+    t2 = np.concatenate([a, c, d], -1).reshape(-1, 9)
+    return np.concatenate([t1, t2]).astype(np.float32)
+
+
+def _box(cx, cy, cz, sx, sy, sz) -> np.ndarray:
+    return _faces(create_vertex_buffer([cx, cy, cz], sx, sy, sz))
+
+
+def office_scene(seed: int = 0, size=(24.0, 16.0, 3.0), cell: float = 0.15, refl_phase: float = np.pi):
+    """Seeded office: tessellated floor / ceiling / outer walls, a corridor
+    along y with 6 rooms on each side (interior walls 0.12 m thick with door
+    gaps), and per room a grid of desk clusters (desk top, two side panels, a
+    monitor, a chair of seat + back + four legs) and a bookshelf of
+    shelves.  Materials: floor, ceiling, outer wall, inner wall, desk, chair,
+    monitor, shelf, each |Gamma| ~ U[0.3, 0.7], constant over the 19 bins.
+    About 1e5 triangles at cell = 0.15 m."""
+    rng = SplitMix64(seed)
+    X, Y, Z = size
+    groups = {}  # material name -> list of triangle arrays
+
+    def add(name, t):
+        groups.setdefault(name, []).append(t)
+
+    nx, ny, nz = int(round(X / cell)), int(round(Y / cell)), int(round(Z / cell))
+    add("floor", _tess_quad([0, 0, 0], [X, 0, 0], [0, Y, 0], nx, ny))
+    add("ceiling", _tess_quad([0, 0, Z], [X, 0, 0], [0, Y, 0], nx, ny))
+    add("outer_wall", _tess_quad([0, 0, 0], [X, 0, 0], [0, 0, Z], nx, nz))
+    add("outer_wall", _tess_quad([0, Y, 0], [X, 0, 0], [0, 0, Z], nx, nz))
+    add("outer_wall", _tess_quad([0, 0, 0], [0, Y, 0], [0, 0, Z], ny, nz))
+    add("outer_wall", _tess_quad([X, 0, 0], [0, Y, 0], [0, 0, Z], ny, nz))
+    cy0, cy1 = Y / 2 - 1.0, Y / 2 + 1.0  # corridor
+    th = 0.12
+    rooms = 6
+    rw = X / rooms
+    for side, (y0, y1) in enumerate(((0.0, cy0), (cy1, Y))):
+        wall_y = cy0 if side == 0 else cy1
+        for r in range(rooms):
+            x0 = r * rw
+            # corridor wall of this room with a 1 m door gap at a seeded spot
+            gx = x0 + rng.uniform(0.5, rw - 1.5)
+            for a, b in ((x0, gx), (gx + 1.0, x0 + rw)):
+                if b - a > 0.05:
+                    add("inner_wall", _box((a + b) / 2, wall_y, Z / 2, b - a, th, Z))
+            if r > 0:  # wall between rooms
+                add("inner_wall", _box(x0, (y0 + y1) / 2, Z / 2, th, y1 - y0, Z))
+            # desk clusters on a 2 x 3 grid
+            for i in range(2):
+                for j in range(3):
+                    dx = x0 + 0.8 + i * (rw - 1.6) + rng.uniform(-0.2, 0.2)
+                    dy = y0 + (j + 0.5) * (y1 - y0) / 3 + rng.uniform(-0.2, 0.2)
+                    w_, d_ = rng.uniform(1.2, 1.6), rng.uniform(0.6, 0.8)
+                    add("desk", _box(dx, dy, 0.74, w_ * 0.5, d_, 0.03))
+                    add("desk", _box(dx - w_ * 0.23, dy, 0.36, 0.03, d_, 0.72))
+                    add("desk", _box(dx + w_ * 0.23, dy, 0.36, 0.03, d_, 0.72))
+                    add("monitor", _box(dx, dy + d_ * 0.3, 0.98, 0.55, 0.04, 0.35))
+                    cxh, cyh = dx, dy - d_ * 0.5 - 0.3
+                    add("chair", _box(cxh, cyh, 0.46, 0.45, 0.45, 0.05))
+                    add("chair", _box(cxh, cyh - 0.21, 0.75, 0.45, 0.04, 0.55))
+                    for lx in (-0.19, 0.19):
+                        for ly in (-0.19, 0.19):
+                            add("chair", _box(cxh + lx, cyh + ly, 0.22, 0.03, 0.03, 0.44))
+            # bookshelf against the outer wall: frame + 5 shelves
+            bx = x0 + rng.uniform(0.8, rw - 0.8)
+            by = (0.25 if side == 0 else Y - 0.25)
+            add("shelf", _box(bx - 0.45, by, 1.0, 0.03, 0.35, 2.0))
+            add("shelf", _box(bx + 0.45, by, 1.0, 0.03, 0.35, 2.0))
+            for k in range(6):
+                add("shelf", _box(bx, by, 0.05 + 0.38 * k, 0.9, 0.35, 0.03))
+    names = list(groups.keys())
+    tri = np.concatenate([np.concatenate(groups[n]) for n in names]).astype(np.float32)
+    mat = np.concatenate([np.full(sum(len(t) for t in groups[n]), i, np.int32) for i, n in enumerate(names)])
+    gam = np.asarray([rng.uniform(0.3, 0.7) for _ in names], np.float32)
+    mat_prop = np.repeat(gam[:, None], 19, axis=1).astype(np.float32)
+    return Scene(tri=tri, mat=mat, mat_prop=mat_prop, refl_phase=np.full(len(names), refl_phase, np.float32),
+                 names=names)
+
+
+def office(seed: int = 0, fib_n: int = 1 << 24, ms_fib_n: int = 1 << 24, n_rx=(256, 256), ap=(8, 8),
+           ms_size=(48, 48), max_reflection: int = 5, cell: float = 0.15) -> Dict[str, Workload]:
+    """Office workload: 8x8 AP (beam pattern 2) on the corridor's west end,
+    three 48 x 48 metasurfaces on room walls, 256 x 256 Rx at z = 1.2 m.
+    fib_n rays in total per AP trace (distributed over the elements) and
+    ms_fib_n per metasurface trace.  Returns H_d and G{i}, F{i} traces."""
+    wl = WAVE_LEN_60GHZ
+    scene = office_scene(seed, cell=cell)
+    X, Y, Z = 24.0, 16.0, 3.0
+    tx = gen_pos_matrix([0.3, Y / 2, 2.4], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0], ap, wl / 2).numpy()
+    rx = gen_pos_from_endpoint([0.1, X - 0.1, n_rx[0]], [0.1, Y - 0.1, n_rx[1]], 1.2).numpy()
+    tx_spec = ArraySpec(tx.astype(np.float32), np.array([1, 0, 0], np.float32), 2, 0.5e-5)
+    rx_spec = ArraySpec(rx.astype(np.float32), np.array([0, 0, 0], np.float32), 0, 0.06)
+    # metasurfaces: on the corridor-facing wall of two rooms and on the east wall
+    cy0 = Y / 2 - 1.0
+    placements = [([6.0, cy0 - 0.06, 2.0], [0.0, -1.0, 0.0]),
+                  ([14.0, Y / 2 + 1.0 + 0.06, 2.0], [0.0, 1.0, 0.0]),
+                  ([X, Y / 2, 1.8], [-1.0, 0.0, 0.0])]
+    ms_list = []
+    for c, k in placements:
+        pos, bh, nrm = gen_ms(c, k, [0.0, 0.0, -1.0], ms_size, wl / 2)
+        ms_list.append(dict(pos=pos.numpy().astype(np.float32), blackhole=bh, normal=nrm.numpy().astype(np.float32),
+                            size=ms_size))
+    out = {}
+    all_bh = [t for m in ms_list for t in m["blackhole"]]
+    out["H_d"] = Workload("office/H_d", scene.with_extra(all_bh, 0.0, "blackhole"), tx_spec, rx_spec, wl,
+                          max_reflection, fib_n, False, ms=ms_list)
+    for i, m in enumerate(ms_list):
+        others = [t for j, mm in enumerate(ms_list) if j != i for t in mm["blackhole"]]
+        sc = scene.with_extra(others, 0.0, "blackhole")
+        out[f"G{i}"] = Workload(f"office/G{i}", sc, tx_spec, ArraySpec(m["pos"], m["normal"], 1, 0.06), wl,
+                                max_reflection, fib_n, False, ms=ms_list)
+        out[f"F{i}"] = Workload(f"office/F{i}", sc, ArraySpec(m["pos"], m["normal"], 1, 0.5e-5), rx_spec, wl,
+                                max_reflection, ms_fib_n, False, ms=ms_list)
+    return out
