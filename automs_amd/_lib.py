@@ -103,6 +103,12 @@ def _bind_contract(L):
         "ams_objective_bwd": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp, vp], C.c_int),
         "ams_adam_step": ([vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp], C.c_int),
         "ams_host_bvh_build": ([vp, vp, i32, i32, vp, i32, vp, vp], C.c_int),
+        "ams_conv3x3_fwd": ([vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp], C.c_int),
+        "ams_conv3x3_bwd_weight": ([vp, vp, vp, vp, i32, i32, i32, i32, i32, vp], C.c_int),
+        "ams_relu_bwd": ([vp, vp, i64, vp], C.c_int),
+        "ams_mse_partials": ([i64], i64),
+        "ams_mse": ([vp, vp, vp, i64, vp, vp, vp], C.c_int),
+        "ams_to_bf16": ([vp, vp, i64, vp], C.c_int),
         "ams_host_plane_ids": ([vp, i32, vp], C.c_int),
     }
     for name, (args, res) in sigs.items():
@@ -123,4 +129,5 @@ EXPORTED = [
     "ams_frag_bytes", "ams_frag_pack", "ams_cgemm_workspace_bytes", "ams_cgemm_frag", "ams_cgemm_f32",
     "ams_objective_workspace_bytes", "ams_objective_fwd", "ams_objective_bwd", "ams_adam_step",
     "ams_host_bvh_build", "ams_host_plane_ids",
+    "ams_conv3x3_fwd", "ams_conv3x3_bwd_weight", "ams_relu_bwd", "ams_mse_partials", "ams_mse", "ams_to_bf16",
 ]

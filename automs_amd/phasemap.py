@@ -387,3 +387,28 @@ def time_to_target(channels: Dict[str, torch.Tensor], ms: Metasurface, steps: in
         "final_loss_plain": out["plain_fp32"]["final_loss"],
         "final_loss_mfma": out["mfma_split_bf16"]["final_loss"],
     }
+
+
+def dft_beams(n0: int, n1: int, n_beams: int) -> torch.Tensor:
+    """Phases [n_beams, n0*n1] of steering vectors of an n0 x n1 half-wave
+    array towards n_beams directions on a regular (u, v) grid in the visible
+    region (u^2 + v^2 < 1); the codebook of the batched optimiser."""
+    side = int(math.ceil(math.sqrt(n_beams)))
+    us = torch.linspace(-0.8, 0.8, side, dtype=torch.float64)
+    dirs = [(u, v) for u in us for v in us][:n_beams]
+    i = torch.arange(n0, dtype=torch.float64).repeat_interleave(n1)
+    j = torch.arange(n1, dtype=torch.float64).repeat(n0)
+    ph = torch.stack([math.pi * (u * i + v * j) for u, v in dirs])
+    return torch.remainder(ph, 2 * math.pi).to(torch.float32)
+
+
+def batched_model(channels: Dict[str, torch.Tensor], ms: Metasurface, ap_shape=(8, 8), n_beams: int = 64,
+                  n_maps: int = 4, seed: int = 0, plain: bool = False) -> PhaseMapModel:
+    """n_maps metasurface phase maps x n_beams fixed AP beams in one GEMM:
+    the rows of A are (map, beam); only the phase maps are optimised."""
+    K = channels["src_to_ms{0}"].shape[1]
+    g = torch.Generator().manual_seed(seed)
+    th_ms = torch.rand(n_maps, K, generator=g) * 2 * math.pi
+    th_ap = dft_beams(ap_shape[0], ap_shape[1], n_beams)
+    return PhaseMapModel("batched", channels, [ms], code_num=n_beams, n_maps=n_maps, ms_theta=[th_ms],
+                         antenna_theta=th_ap, optimize_codebook=False, plain=plain)

@@ -1,0 +1,428 @@
+"""Learned placement surrogate and the annealing search it steers.
+
+The reference tunes the metasurface placement (wall, offset) with NNI's
+simulated-annealing tuner, every trial retracing and re-optimising from
+scratch (physical_exp_nni_main.py:12-29, physical_exp_nni_model.py:59-70).
+Here an in-process annealing loop searches placements, a small conv net
+predicts each candidate's per-Rx gain map, and the ray tracer re-scores every
+accepted candidate (the tracer stays the judge and the fallback).
+
+Network: 4 conv3x3 layers, channels 9 -> 32 -> 32 -> 32 -> 1, ReLU between,
+bf16 weights with FP32 master copies, all forward/backward math in
+libams.so (ams_conv3x3_fwd / ams_conv3x3_bwd_weight / ams_relu_bwd /
+ams_mse), Adam via ams_adam_step.
+
+Target of a placement: the per-Rx gain (dB) a phase-aligned metasurface can
+add to the AP's broadside beam,
+    h_d = sum_tx H_d[tx, rx],  g_k = sum_tx G[tx, k],  m = sum_k |g_k| |F[k, rx]|,
+    gain = 20 log10((|h_d| + m + eps) / (|h_d| + eps)), clamped to [0, 40] dB,
+and the placement score is its mean over the Rx grid (coverage gain, dB).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .geometry import gen_ms
+from .scenes import ArraySpec, SplitMix64, Workload, room_boxes, room_boxes_scene, WAVE_LEN_60GHZ
+from .tracer import TraceScene, trace
+
+ROOM = (10.0, 8.0, 3.0)
+GRID = 64
+EPS = 1e-7
+MS_SIZE = (32, 32)
+
+
+def _p(t: torch.Tensor) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+def _s() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+# ----------------------------------------------------------------- network --
+
+class SurrogateNet:
+    """Conv net on the GPU with hand-written kernels."""
+
+    def __init__(self, channels=(9, 32, 32, 32, 1), seed: int = 0, device="cuda"):
+        self.ch = list(channels)
+        self.dev = torch.device(device)
+        g = torch.Generator().manual_seed(seed)
+        shapes = []
+        for i in range(len(self.ch) - 1):
+            shapes.append((self.ch[i + 1], self.ch[i], 3, 3))
+            shapes.append((self.ch[i + 1],))
+        n = sum(int(np.prod(s)) for s in shapes)
+        self.flat = torch.zeros(n, dtype=torch.float32)
+        off = 0
+        self.views = []
+        for s in shapes:
+            k = int(np.prod(s))
+            if len(s) == 4:  # He initialisation
+                fan_in = s[1] * 9
+                self.flat[off:off + k] = torch.randn(k, generator=g) * math.sqrt(2.0 / fan_in)
+            self.views.append((off, s))
+            off += k
+        self.flat = self.flat.to(self.dev)
+        self.grad = torch.zeros_like(self.flat)
+        self.m = torch.zeros_like(self.flat)
+        self.v = torch.zeros_like(self.flat)
+        self.t = 0
+        self.refresh()
+
+    def param(self, i: int) -> torch.Tensor:
+        off, s = self.views[i]
+        return self.flat[off:off + int(np.prod(s))].view(s)
+
+    def grad_of(self, i: int) -> torch.Tensor:
+        off, s = self.views[i]
+        return self.grad[off:off + int(np.prod(s))].view(s)
+
+    def _bf16(self, x: torch.Tensor) -> torch.Tensor:
+        x = x.contiguous()
+        out = torch.empty(x.numel(), dtype=torch.int16, device=self.dev)
+        _lib.check(_lib.lib().ams_to_bf16(_p(x), _p(out), x.numel(), _s()), "ams_to_bf16")
+        return out
+
+    def refresh(self):
+        """bf16 weight copies from the FP32 masters, and the flipped /
+        channel-transposed copies the input-gradient pass uses."""
+        self.w_bf, self.wt_bf = [], []
+        for l in range(len(self.ch) - 1):
+            W = self.param(2 * l)
+            self.w_bf.append(self._bf16(W))
+            self.wt_bf.append(self._bf16(W.flip(2, 3).transpose(0, 1).contiguous()))
+
+    def weights_f64(self) -> List[torch.Tensor]:
+        """The bf16-rounded weights the kernels use, as float64 CPU tensors."""
+        out = []
+        for l in range(len(self.ch) - 1):
+            W = self.param(2 * l).detach().float().cpu().contiguous()
+            u = W.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+            u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).to(torch.int64)
+            u = torch.where(u >= 2 ** 31, u - 2 ** 32, u).to(torch.int32)
+            out.append(u.view(torch.float32).double())
+            out.append(self.param(2 * l + 1).detach().cpu().double())
+        return out
+
+    def forward(self, X: torch.Tensor, keep: bool = False) -> torch.Tensor:
+        L = _lib.lib()
+        B, _, H, W = X.shape
+        acts = [X.contiguous()]
+        a = acts[0]
+        nl = len(self.ch) - 1
+        for l in range(nl):
+            y = torch.empty(B, self.ch[l + 1], H, W, dtype=torch.float32, device=self.dev)
+            _lib.check(L.ams_conv3x3_fwd(_p(a), _p(self.w_bf[l]), _p(self.param(2 * l + 1)), _p(y), B, self.ch[l],
+                                         self.ch[l + 1], H, W, 1 if l < nl - 1 else 0, _s()), "ams_conv3x3_fwd")
+            acts.append(y)
+            a = y
+        if keep:
+            self.acts = acts
+        return a
+
+    def backward(self, gY: torch.Tensor):
+        """Fills self.grad from dL/dY of the last forward(keep=True)."""
+        L = _lib.lib()
+        acts = self.acts
+        B, _, H, W = gY.shape
+        g = gY.contiguous()
+        nl = len(self.ch) - 1
+        for l in reversed(range(nl)):
+            _lib.check(L.ams_conv3x3_bwd_weight(_p(acts[l]), _p(g), _p(self.grad_of(2 * l)),
+                                                _p(self.grad_of(2 * l + 1)), B, self.ch[l], self.ch[l + 1], H, W,
+                                                _s()), "ams_conv3x3_bwd_weight")
+            if l == 0:
+                break
+            gx = torch.empty(B, self.ch[l], H, W, dtype=torch.float32, device=self.dev)
+            _lib.check(L.ams_conv3x3_fwd(_p(g), _p(self.wt_bf[l]), None, _p(gx), B, self.ch[l + 1], self.ch[l], H, W,
+                                         0, _s()), "ams_conv3x3_fwd (input gradient)")
+            _lib.check(L.ams_relu_bwd(_p(acts[l]), _p(gx), gx.numel(), _s()), "ams_relu_bwd")
+            g = gx
+
+    def mse(self, pred: torch.Tensor, target: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        L = _lib.lib()
+        n = pred.numel()
+        grad = torch.empty_like(pred)
+        part = torch.empty(L.ams_mse_partials(n), dtype=torch.float32, device=self.dev)
+        _lib.check(L.ams_mse(_p(pred), _p(target.contiguous()), None, n, _p(grad), _p(part), _s()), "ams_mse")
+        return part.sum() / n, grad
+
+    def adam_step(self, lr: float = 2e-3):
+        self.t += 1
+        _lib.check(_lib.lib().ams_adam_step(_p(self.flat), _p(self.grad), _p(self.m), _p(self.v), self.flat.numel(),
+                                            lr, 0.9, 0.999, 1e-8, self.t, _s()), "ams_adam_step")
+        self.refresh()
+
+
+# --------------------------------------------------------------- placements --
+
+@dataclass
+class Placement:
+    wall: int   # 0: x = 0 (normal +x), 1: x = 10 (-x), 2: y = 0 (+y), 3: y = 8 (-y)
+    s: float    # position along the wall, 0..1
+    z: float    # centre height (m)
+
+    def pose(self):
+        X, Y, _ = ROOM
+        if self.wall == 0:
+            return 0.0, self.s * Y, self.z, 0.0
+        if self.wall == 1:
+            return X, self.s * Y, self.z, math.pi
+        if self.wall == 2:
+            return self.s * X, 0.0, self.z, math.pi / 2
+        return self.s * X, Y, self.z, -math.pi / 2
+
+    def ms(self, size=MS_SIZE):
+        x, y, z, yaw = self.pose()
+        k = [math.cos(yaw), math.sin(yaw), 0.0]
+        pos, bh, nrm = gen_ms([x, y, z], k, [0.0, 0.0, -1.0], size, WAVE_LEN_60GHZ / 2)
+        return pos.numpy().astype(np.float32), bh, nrm.numpy().astype(np.float32)
+
+
+def random_placement(rng: SplitMix64) -> Placement:
+    return Placement(int(rng.uniform(0, 4)) % 4, rng.uniform(0.1, 0.9), rng.uniform(0.6, 2.4))
+
+
+def neighbour(p: Placement, rng: SplitMix64) -> Placement:
+    if rng.uniform() < 0.1:
+        return Placement(int(rng.uniform(0, 4)) % 4, rng.uniform(0.1, 0.9), rng.uniform(0.6, 2.4))
+    u1, u2 = max(rng.uniform(), 1e-12), rng.uniform()
+    n1 = math.sqrt(-2 * math.log(u1)) * math.cos(2 * math.pi * u2)
+    n2 = math.sqrt(-2 * math.log(u1)) * math.sin(2 * math.pi * u2)
+    return Placement(p.wall, min(0.9, max(0.1, p.s + 0.06 * n1)), min(2.4, max(0.6, p.z + 0.2 * n2)))
+
+
+def occupancy(scene_seed: int) -> np.ndarray:
+    """Top view of the room at the Rx grid: height of the tallest box over
+    each Rx position / room height."""
+    sc = room_boxes_scene(scene_seed)
+    boxes = sc.tri[12:].reshape(-1, 12, 3, 3)
+    xs = np.linspace(0.1, 9.9, GRID)
+    ys = np.linspace(0.1, 7.9, GRID)
+    occ = np.zeros((GRID, GRID), np.float32)
+    for b in boxes:
+        lo, hi = b.reshape(-1, 3).min(0), b.reshape(-1, 3).max(0)
+        ix = (xs >= lo[0]) & (xs <= hi[0])
+        iy = (ys >= lo[1]) & (ys <= hi[1])
+        occ[np.ix_(ix, iy)] = np.maximum(occ[np.ix_(ix, iy)], hi[2] / ROOM[2])
+    return occ
+
+
+def features(occ: np.ndarray, p: Placement) -> np.ndarray:
+    """9 input channels [9, 64, 64]: occupancy, Rx x / y, offsets to the
+    metasurface, distance, cosine to its normal, its height and size."""
+    x, y, z, yaw = p.pose()
+    xs = np.linspace(0.1, 9.9, GRID)[:, None] * np.ones((1, GRID))
+    ys = np.ones((GRID, 1)) * np.linspace(0.1, 7.9, GRID)[None, :]
+    dx, dy, dz = xs - x, ys - y, 1.2 - z
+    r = np.sqrt(dx * dx + dy * dy + dz * dz)
+    cosn = (dx * math.cos(yaw) + dy * math.sin(yaw)) / np.maximum(r, 1e-6)
+    f = np.stack([occ, xs / 10, ys / 8, dx / 10, dy / 8, r / 10, cosn, np.full_like(xs, z / 3),
+                  np.full_like(xs, MS_SIZE[0] * WAVE_LEN_60GHZ / 2 / 0.1)]).astype(np.float32)
+    return f
+
+
+# ------------------------------------------------------------------ tracing --
+
+class PlacementTracer:
+    """Traces the gain map of placements in one scene: H_d once, then G and
+    F per placement (reduced ray counts; the scene handle is reused)."""
+
+    def __init__(self, scene_seed: int, fib_ap: int = 1 << 18, fib_ms: int = 1 << 18, max_reflection: int = 3):
+        self.base = room_boxes(scene_seed, fib_n=fib_ap, max_reflection=max_reflection)
+        self.ts = TraceScene(self.base.scene)
+        self.fib_ap, self.fib_ms = fib_ap, fib_ms
+        w = self.base
+        res, _, _ = trace(self.ts, w.tx, w.rx, w.wave_len, w.max_reflection, fib_ap, True, w.gridsize)
+        self.hd = torch.complex(res[..., 0], res[..., 1]).sum(0)  # broadside beam
+        self.n_traces = 0
+        self.gpu_seconds = 0.0
+
+    def gain_map(self, p: Placement) -> torch.Tensor:
+        w = self.base
+        pos, _, nrm = p.ms()
+        t0 = time.perf_counter()
+        resG, _, _ = trace(self.ts, w.tx, ArraySpec(pos, nrm, 1, 0.06), w.wave_len, w.max_reflection, self.fib_ap,
+                           True, w.gridsize)
+        resF, _, _ = trace(self.ts, ArraySpec(pos, nrm, 1, 0.5e-5), w.rx, w.wave_len, w.max_reflection, self.fib_ms,
+                           False, w.gridsize)
+        g = torch.complex(resG[..., 0], resG[..., 1]).sum(0)
+        F = torch.complex(resF[..., 0], resF[..., 1])
+        m = (g.abs()[:, None] * F.abs()).sum(0)
+        a = self.hd.abs()
+        gain = (20 * torch.log10((a + m + EPS) / (a + EPS))).clamp(0, 40).reshape(GRID, GRID)
+        torch.cuda.synchronize()
+        self.gpu_seconds += time.perf_counter() - t0
+        self.n_traces += 1
+        return gain
+
+
+def generate_dataset(n_scenes: int, per_scene: int, seed: int = 0, fib_ap: int = 1 << 18, fib_ms: int = 1 << 18,
+                     log=None):
+    """[N, 9, 64, 64] features and [N, 1, 64, 64] traced gain maps (GPU)."""
+    rng = SplitMix64(seed)
+    Xs, Ys = [], []
+    for sidx in range(n_scenes):
+        occ = occupancy(sidx)
+        pt = PlacementTracer(sidx, fib_ap, fib_ms)
+        for _ in range(per_scene):
+            p = random_placement(rng)
+            Xs.append(torch.from_numpy(features(occ, p)))
+            Ys.append(pt.gain_map(p).unsqueeze(0))
+        if log:
+            log(f"scene {sidx + 1}/{n_scenes}: {pt.n_traces} placements traced in {pt.gpu_seconds:.1f} s")
+        del pt
+    return torch.stack(Xs).cuda(), torch.stack(Ys).float()
+
+
+def train(net: SurrogateNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20, batch: int = 64, lr: float = 2e-3,
+          seed: int = 0, log=None) -> List[float]:
+    """Minibatch Adam on the MSE of the gain map (scaled by 1/10)."""
+    g = torch.Generator().manual_seed(seed)
+    n = X.shape[0]
+    hist = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g)
+        tot = torch.zeros((), device=X.device)
+        nb = 0
+        lr_ep = lr * (0.5 * (1 + math.cos(math.pi * ep / epochs)))
+        for i in range(0, n - batch + 1, batch):
+            idx = perm[i:i + batch].to(X.device)
+            pred = net.forward(X[idx], keep=True)
+            loss, gpred = net.mse(pred, Y[idx] / 10)
+            net.backward(gpred)
+            net.adam_step(lr_ep)
+            tot += loss.detach()
+            nb += 1
+        hist.append(float(tot / max(nb, 1)))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: mse {hist[-1]:.4f} (gain/10)^2")
+    return hist
+
+
+def predict_score(net: SurrogateNet, occ: np.ndarray, ps: List[Placement]) -> torch.Tensor:
+    X = torch.from_numpy(np.stack([features(occ, p) for p in ps])).cuda()
+    return net.forward(X).mean((1, 2, 3)) * 10
+
+
+# ---------------------------------------------------------------- annealing --
+
+def anneal(score_fn, steps: int, seed: int, t0: float = 1.0, t1: float = 0.01, rescore=None, start=None):
+    """Simulated annealing maximising a placement score.
+
+    Plain search (rescore None): score_fn is the traced score of a candidate.
+    Surrogate search: score_fn is the predicted score; the Metropolis test
+    compares predictions, and every accepted candidate is re-scored by
+    rescore (the tracer); the best placement is chosen on traced scores."""
+    rng = SplitMix64(seed)
+    cur = start or random_placement(rng)
+    cur_val = score_fn(cur)
+    best = cur
+    best_val = rescore(cur) if rescore else cur_val
+    accepted = 0
+    for k in range(steps):
+        T = t0 * (t1 / t0) ** (k / max(1, steps - 1))
+        cand = neighbour(cur, rng)
+        val = score_fn(cand)
+        if val >= cur_val or rng.uniform() < math.exp((val - cur_val) / T):
+            cur, cur_val = cand, val
+            accepted += 1
+            traced = rescore(cand) if rescore else val
+            if traced > best_val:
+                best, best_val = cand, traced
+    return best, best_val, accepted
+
+
+def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 2000, search_scene: int = 1000,
+                   epochs: int = 20, fib: int = 1 << 18, seed: int = 0, log=print) -> dict:
+    """Config-4 experiment: traced dataset -> trained surrogate -> placement
+    search three ways on an unseen scene: plain annealing (every candidate
+    traced), surrogate-guided annealing (accepted candidates traced), and
+    plain annealing cut to the surrogate search's number of traces."""
+    out: Dict = {"scenes": n_scenes, "per_scene": per_scene, "sa_steps": sa_steps, "search_scene": search_scene}
+    t0 = time.perf_counter()
+    X, Y = generate_dataset(n_scenes, per_scene, seed=seed, fib_ap=fib, fib_ms=fib, log=log)
+    torch.cuda.synchronize()
+    out["dataset_seconds"] = time.perf_counter() - t0
+    n = X.shape[0]
+    n_val = max(64, n // 20)
+    Xtr, Ytr, Xva, Yva = X[:-n_val], Y[:-n_val], X[-n_val:], Y[-n_val:]
+    net = SurrogateNet(seed=seed)
+    t0 = time.perf_counter()
+    hist = train(net, Xtr, Ytr, epochs=epochs, seed=seed, log=log)
+    torch.cuda.synchronize()
+    out["train_seconds"] = time.perf_counter() - t0
+    with torch.no_grad():
+        pv = torch.cat([net.forward(Xva[i:i + 64]) for i in range(0, n_val, 64)]) * 10
+        mse_val = float(((pv - Yva) ** 2).mean())
+        base = float(((Ytr.mean() - Yva) ** 2).mean())
+        sv, tv = pv.mean((1, 2, 3)), Yva.mean((1, 2, 3))
+        r2_score = 1 - float(((sv - tv) ** 2).sum() / ((tv - tv.mean()) ** 2).sum())
+    out.update(train_mse_final=hist[-1] * 100, val_mse_db2=mse_val, val_mse_mean_predictor_db2=base,
+               val_score_r2=r2_score)
+    log(f"validation: map mse {mse_val:.2f} dB^2 (mean predictor {base:.2f}); placement-score R^2 {r2_score:.3f}")
+
+    occ = occupancy(search_scene)
+    pt = PlacementTracer(search_scene, fib, fib)
+
+    def traced(p):
+        return float(pt.gain_map(p).mean())
+
+    # plain search: every candidate traced
+    pt.gpu_seconds, pt.n_traces = 0.0, 0
+    t0 = time.perf_counter()
+    b1, v1, a1 = anneal(traced, sa_steps, seed=seed + 1)
+    out["plain"] = {"best_gain_db": v1, "seconds": time.perf_counter() - t0, "traces": pt.n_traces,
+                    "accepted": a1, "best": b1.__dict__}
+    log(f"plain search: {v1:.3f} dB in {out['plain']['seconds']:.1f} s ({pt.n_traces} traces)")
+    # surrogate-guided search
+    pt.gpu_seconds, pt.n_traces = 0.0, 0
+    t0 = time.perf_counter()
+    b2, v2, a2 = anneal(lambda p: float(predict_score(net, occ, [p])[0]), sa_steps, seed=seed + 1, rescore=traced)
+    torch.cuda.synchronize()
+    out["surrogate"] = {"best_gain_db": v2, "seconds": time.perf_counter() - t0, "traces": pt.n_traces,
+                        "accepted": a2, "best": b2.__dict__}
+    log(f"surrogate search: {v2:.3f} dB in {out['surrogate']['seconds']:.1f} s ({pt.n_traces} traces)")
+    # plain search with the surrogate search's trace budget
+    budget = max(1, pt.n_traces - 1)
+    pt.gpu_seconds, pt.n_traces = 0.0, 0
+    t0 = time.perf_counter()
+    b3, v3, a3 = anneal(traced, budget, seed=seed + 1)
+    out["plain_same_budget"] = {"best_gain_db": v3, "seconds": time.perf_counter() - t0, "traces": pt.n_traces,
+                                "steps": budget}
+    log(f"plain search, same trace budget: {v3:.3f} dB in {out['plain_same_budget']['seconds']:.1f} s")
+    out["gain_gap_db"] = v1 - v2
+    out["time_ratio"] = out["surrogate"]["seconds"] / out["plain"]["seconds"]
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description="placement surrogate experiment (traced data, training, search)")
+    ap.add_argument("--scenes", type=int, default=20)
+    ap.add_argument("--per-scene", type=int, default=1000)
+    ap.add_argument("--sa-steps", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--fib", type=int, default=1 << 18)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    res = run_experiment(a.scenes, a.per_scene, a.sa_steps, epochs=a.epochs, fib=a.fib,
+                         log=lambda m: print(m, file=sys.stderr, flush=True))
+    txt = json.dumps(res)
+    print(txt, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(txt + "\n")

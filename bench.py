@@ -66,6 +66,61 @@ def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int) -> dict:
             "sample": f"{what} ({q} closest-hit queries in {dt:.1f} s; FP64 oracle, OpenMP)"}
 
 
+def contraction_office(iters: int = 20) -> dict:
+    """Config-3 shape on traced office channels: 4 phase maps x 64 AP beams
+    (M = 256) against [F0 ; H_d] (K = 2304 + 64, N = 65,536).  Times the
+    forward split-bf16 MFMA GEMM alone (HIP events on the current stream) and
+    a full optimiser step (forward, objective, hand-written backward, Adam)."""
+    import torch
+
+    from automs_amd import scenes
+    from automs_amd.phasemap import Contract, FusedAdam, Metasurface, batched_model, cgemm_frag, frag_pack
+    from automs_amd.tracer import TraceScene, trace
+    tr = scenes.office(fib_n=1 << 20, ms_fib_n=1 << 20)
+    ch = {}
+    for key, name in (("src_to_dst", "H_d"), ("src_to_ms{0}", "G0"), ("ms{0}_to_dst", "F0")):
+        w = tr[name]
+        res, _, _ = trace(TraceScene(w.scene), w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
+                          w.gridsize)
+        ch[key] = torch.complex(res[..., 0], res[..., 1])
+    model = batched_model(ch, Metasurface(None, (48, 48), None, None, [], max_phase=360.0))
+    op = model.op
+    with torch.no_grad():
+        A = model.stacked_operand().contiguous()
+    M, Kc, N = A.shape[0], op.Kc, op.N
+    fa = frag_pack(A, M, Kc, Kc, 1)
+    for _ in range(3):
+        cgemm_frag(fa, op.frag_fwd, M, N, Kc, conj_b=False)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        cgemm_frag(fa, op.frag_fwd, M, N, Kc, conj_b=False)
+    e1.record()
+    torch.cuda.synchronize()
+    t_fwd = e0.elapsed_time(e1) / iters * 1e-3
+    useful = 8.0 * M * Kc * N
+    opt = FusedAdam([p for p in model.parameters() if p.requires_grad], lr=0.3)
+    for _ in range(2):
+        model().backward()
+        opt.step()
+        opt.zero_grad()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(iters):
+        model().backward()
+        opt.step()
+        opt.zero_grad()
+    e1.record()
+    torch.cuda.synchronize()
+    t_step = e0.elapsed_time(e1) / iters * 1e-3
+    return {"shape": {"M": M, "K": Kc, "N": N}, "fwd_gemm_ms": t_fwd * 1e3,
+            "useful_tflops": useful / t_fwd / 1e12, "issued_tflops": 3 * useful / t_fwd / 1e12,
+            "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": 3 * useful / t_fwd / 2.5e15,
+            "f_operand_gb_per_s": 16.0 * Kc * N / t_fwd / 1e9,
+            "optimizer_step_ms": t_step * 1e3, "maps": 4, "beams": 64}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +129,7 @@ def main():
     ap.add_argument("--workload", default="room_boxes_ms", choices=["room_boxes", "room_boxes_ms"])
     ap.add_argument("--opt-steps", type=int, default=500)
     ap.add_argument("--no-opt", action="store_true")
+    ap.add_argument("--no-contract", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-fib", type=int, default=1 << 16)
     ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
@@ -175,6 +231,10 @@ def main():
             "plain_fp32_seconds_all_steps": r["plain_seconds_500"],
             "plain_optimum_cov_db": r["plain_optimum_cov_db"], "mfma_best_cov_db": r["mfma_best_cov_db"],
             "start_cov_db": r["start_cov_db"], "codes": 30, "objective": "mean log2(1+SNR), N0 = -90 dBm"}
+    if not args.no_contract:
+        r = contraction_office()
+        print(f"contraction: {r}", file=sys.stderr, flush=True)
+        line["contraction_office_batched"] = r
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_fib, args.cpu_sample_ms_fib)
     print(json.dumps(line), flush=True)

@@ -109,6 +109,23 @@ int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t m
 int ams_adam_step(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
                   float beta2, float eps, int32_t step, void *stream);
 
+/* ---- placement surrogate network (3x3 convolutions, bf16 weights) --------
+ * X [B][Cin][H][W] f32, W [Cout][Cin][3][3] bf16, bias [Cout] f32, padding 1;
+ * channels <= 32, H and W multiples of 16.  The input gradient is
+ * ams_conv3x3_fwd with weights flipped in (ky, kx) and Cin/Cout swapped. */
+int ams_conv3x3_fwd(const float *X, const void *Wbf16, const float *bias, float *Y, int32_t B, int32_t Cin,
+                    int32_t Cout, int32_t H, int32_t W, int32_t relu, void *stream);
+int ams_conv3x3_bwd_weight(const float *X, const float *dY, float *dW, float *db, int32_t B, int32_t Cin,
+                           int32_t Cout, int32_t H, int32_t W, void *stream);
+/* grad *= (Y > 0) */
+int ams_relu_bwd(const float *Y, float *grad, int64_t n, void *stream);
+/* mean((pred - target)^2 mask): grad [n] and per-block partial sums
+ * [ams_mse_partials(n)] of the squared errors (sum them and divide by n) */
+int64_t ams_mse_partials(int64_t n);
+int ams_mse(const float *pred, const float *target, const float *mask, int64_t n, float *grad, float *partial,
+            void *stream);
+int ams_to_bf16(const float *x, void *y, int64_t n, void *stream);
+
 /* ---- host-only helpers (no device calls; used by the CPU test suite) ---- */
 /* the binned-SAH threaded BVH of ams_scene_create: nodes_out gets 32-byte
  * nodes {lo[3] f32, miss i32, hi[3] f32, info i32}, order_out the leaf order */
