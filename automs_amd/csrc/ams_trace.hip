@@ -110,8 +110,8 @@ __device__ __forceinline__ bool box_test(const Ray &r, float4 lo, float4 hi, flo
   return t0 <= t1 * 1.0000004f;
 }
 
-struct Counters {
-  unsigned long long nodes = 0, tris = 0, sph = 0, queries = 0;
+struct Counters {  // per-lane counts of one launch (32 bits per lane suffice)
+  unsigned nodes = 0, tris = 0, sph = 0, queries = 0;
 };
 
 // Closest scene triangle in [tmin, tbest], triangles of plane skip_pl are
@@ -273,7 +273,8 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long *co
 }
 
 __device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const Counters &c) {
-  unsigned long long v[4] = {c.nodes, c.tris, c.sph, c.queries};
+  unsigned long long v[4] = {(unsigned long long)c.nodes, (unsigned long long)c.tris, (unsigned long long)c.sph,
+                             (unsigned long long)c.queries};
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     unsigned long long x = v[k];
@@ -369,13 +370,11 @@ struct Branch {
   uint64_t khi, klo;
 };
 
-constexpr int kDedup = 4;
-
 __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
   long long r = a.ray_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   Counters ct;
   bool active = r < a.ray_end;
-  unsigned long long nrec = 0;
+  unsigned nrec = 0;
   if (active) {
     long long e, k;
     if (a.per_element) {
@@ -399,8 +398,7 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
     cur.last_pl = -1;
     cur.khi = (0xFFFFFull << 36) | (0xFFFFFull << 16) | 0xFFFFull;
     cur.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
-    uint64_t seen_hi[kDedup], seen_lo[kDedup];
-    int n_seen = 0;
+    uint64_t last_hi = ~0ull, last_lo = ~0ull;  // previous record of this ray
     bool more = true;
     while (more) {
       while (cur.ttl > 0) {
@@ -418,13 +416,9 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
           uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
           uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
           uint64_t lo = (cur.klo & ~0xFFFFFull) | tile;
-          bool dup = false;
-          for (int q = 0; q < kDedup; q++)
-            if (q < n_seen && seen_hi[q] == hi && seen_lo[q] == lo) dup = true;
-          if (!dup) {
-            seen_hi[n_seen % kDedup] = hi;
-            seen_lo[n_seen % kDedup] = lo;
-            n_seen++;
+          if (hi != last_hi || lo != last_lo) {
+            last_hi = hi;
+            last_lo = lo;
             unsigned long long idx = wave_append(a.out_count);
             nrec++;
             if (idx < a.cap) {
@@ -469,6 +463,120 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
   if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
 }
 
+// Wavefront-persistent form of the same discovery: the grid is sized to the
+// machine, every loop iteration performs one closest-hit query per lane, and
+// a lane whose ray has ended takes the next ray index from a global counter
+// (one atomic per wave per refill).  Lanes therefore stay busy however many
+// events their rays have; each wave leaves the loop once the counter is past
+// the end of the range and none of its lanes holds a ray.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
+  Counters ct;
+  unsigned nrec = 0;
+  Branch st[AMS_MAX_HIT];
+  int sp = 0;
+  Branch cur;
+  bool have = false;
+  uint64_t last_hi = ~0ull, last_lo = ~0ull;  // the lane's previous record (consecutive duplicates)
+  const int lane = __lane_id();
+  while (true) {
+    unsigned long long need = __ballot(!have);
+    if (need) {
+      int leader = __ffsll((long long)need) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(next_ray, (unsigned long long)__popcll(need));
+      base = __shfl(base, leader);
+      if (!have) {
+        long long r = a.ray_begin + (long long)(base + __popcll(need & ((1ull << lane) - 1ull)));
+        if (r < a.ray_end) {
+          long long e, k;
+          if (a.per_element) {
+            e = (r / a.fib_n);
+            k = a.perm[r % a.fib_n];
+          } else {
+            long long rr2 = a.perm[r];
+            e = rr2 % a.n_src;
+            k = rr2;
+          }
+          double dx, dy, dz;
+          fib_dir(k, a.fib_n, dx, dy, dz);
+          cur.o = f3(a.src[3 * e], a.src[3 * e + 1], a.src[3 * e + 2]);
+          cur.d = f3((float)dx, (float)dy, (float)dz);
+          cur.ttl = a.K + 1;
+          cur.b = 0;
+          cur.nref = 0;
+          cur.last_pl = -1;
+          cur.khi = (0xFFFFFull << 36) | (0xFFFFFull << 16) | 0xFFFFull;
+          cur.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
+          last_hi = ~0ull;
+          last_lo = ~0ull;
+          have = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) break;
+    if (have) {
+      Ray ray;
+      ray_setup(ray, cur.o, cur.d);
+      float tt = INFINITY;
+      int torig;
+      int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+      float ts = tt;
+      int rx = closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
+      ct.queries++;
+      bool ended = false;
+      if (rx >= 0 && ts < tt) {
+        int h = a.K + 1 - cur.ttl;
+        uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
+        uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
+        uint64_t lo = (cur.klo & ~0xFFFFFull) | tile;
+        if (hi != last_hi || lo != last_lo) {
+          last_hi = hi;
+          last_lo = lo;
+          unsigned long long idx = wave_append(a.out_count);
+          nrec++;
+          if (idx < a.cap) {
+            a.out_hi[idx] = hi;
+            a.out_lo[idx] = lo;
+          }
+        }
+        cur.o = cur.o + cur.d * ts;
+        cur.last_pl = -1;
+      } else if (slot < 0) {
+        ended = true;  // miss, optixPathTracer.cu:276-279
+      } else {
+        const AmsTri tr = a.tris[slot];
+        int pl = __float_as_int(tr.b.w);
+        cur.o = cur.o + cur.d * tt;
+        cur.last_pl = pl;
+        if (cur.ttl > 1) {
+          float4 nm = a.aux[torig].n_maxedge;
+          float3 N0 = f3(nm.x, nm.y, nm.z);
+          float3 N = dot3(N0, cur.d) > 0.0f ? N0 * -1.0f : N0;
+          int m = __float_as_int(tr.c.w);
+          int th = theta_bin(cur.d, N);
+          bool pen = a.mat_prop[th + 19 * m] < 0.0f;
+          cur.ttl--;
+          int h = a.K + 1 - cur.ttl;
+          if (pen && sp < AMS_MAX_HIT) st[sp++] = cur;
+          cur.b |= 1 << (h - 1);
+          cur.d = norm3(cur.d - N * (2.0f * dot3(cur.d, N)));
+          if (cur.nref < AMS_MAX_HIT) key_set_plane(cur.khi, cur.klo, cur.nref, (uint32_t)pl);
+          cur.nref++;
+        }
+      }
+      if (ended || cur.ttl <= 0) {
+        if (sp > 0) cur = st[--sp];
+        else have = false;
+      }
+    }
+  }
+  wave_add_counters(a.counters, ct);
+  unsigned long long x = nrec;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+  if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
+}
+
 // ------------------------------------------------------------------------
 // backward evaluation
 // ------------------------------------------------------------------------
@@ -500,7 +608,7 @@ struct EvalArgs {
   unsigned long long *counters;  // nodes, tris, spheres, queries, chains, successes
 };
 
-__global__ void __launch_bounds__(512) k_evaluate(EvalArgs a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) k_evaluate(EvalArgs a) {
   int unit = blockIdx.y;
   int item = blockIdx.x * blockDim.x + threadIdx.x;
   int g = a.unit_tile[unit];
@@ -514,10 +622,12 @@ __global__ void __launch_bounds__(512) k_evaluate(EvalArgs a) {
   Counters ct;
   unsigned long long nchain = 0, nsucc = 0;
   double acc_re = 0.0, acc_im = 0.0;
-  double3 si = make_double3(0, 0, 0), rj = make_double3(0, 0, 0);
+  // element positions are f32 inputs: kept in f32 registers, widened exactly
+  // to FP64 where the image and the path length are formed
+  float3 s3 = f3(0, 0, 0), r3 = f3(0, 0, 0);
   if (active) {
-    si = make_double3(a.src[3 * i], a.src[3 * i + 1], a.src[3 * i + 2]);
-    rj = make_double3(a.dst[3 * j], a.dst[3 * j + 1], a.dst[3 * j + 2]);
+    s3 = f3(a.src[3 * i], a.src[3 * i + 1], a.src[3 * i + 2]);
+    r3 = f3(a.dst[3 * j], a.dst[3 * j + 1], a.dst[3 * j + 2]);
   }
   for (int t = tb; t < te; t++) {
     bool ok = false;
@@ -527,7 +637,7 @@ __global__ void __launch_bounds__(512) k_evaluate(EvalArgs a) {
       int h = (int)(khi >> 61), bm = (int)((khi >> 56) & 31);
       int nref = __popc(bm & ((1 << h) - 1));
       // image of the Tx element across the reflection planes (FP64)
-      double3 I = si;
+      double3 I = make_double3(s3.x, s3.y, s3.z);
       for (int k = 0; k < nref; k++) {
         const AmsPlane P = a.planes[key_get_plane(khi, klo, k)];
         double s = 2.0 * (I.x * P.n[0] + I.y * P.n[1] + I.z * P.n[2] - P.d);
@@ -535,11 +645,10 @@ __global__ void __launch_bounds__(512) k_evaluate(EvalArgs a) {
         I.y -= s * P.n[1];
         I.z -= s * P.n[2];
       }
-      double ex = I.x - rj.x, ey = I.y - rj.y, ez = I.z - rj.z;
+      double ex = I.x - (double)r3.x, ey = I.y - (double)r3.y, ez = I.z - (double)r3.z;
       double L = sqrt(ex * ex + ey * ey + ez * ez);
       float3 d = f3((float)(ex / L), (float)(ey / L), (float)(ez / L));
-      float3 o = f3((float)rj.x, (float)rj.y, (float)rj.z);
-      float3 s3 = f3((float)si.x, (float)si.y, (float)si.z);
+      float3 o = r3;
       float A = beam_pattern(d, a.dst_dir, a.dst_bptype);
       float D = 0.0f, Dres = 0.0f, far = 1e6f, psi = 0.0f;
       int ttl = h + 1, ri = 0, last_pl = -1;
@@ -839,7 +948,11 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   const unsigned long long cap = (unsigned long long)chunk * 6 + 1024;
   uint64_t *c_hi = (uint64_t *)sc.get("c_hi", cap * 8), *c_lo = (uint64_t *)sc.get("c_lo", cap * 8);
   unsigned long long *d_cnt = (unsigned long long *)sc.get("c_cnt", 8);
-  if (!c_hi || !c_lo || !d_cnt) return AMS_ERR_ALLOC;
+  unsigned long long *d_next = (unsigned long long *)sc.get("c_next", 8);
+  if (!c_hi || !c_lo || !d_cnt || !d_next) return AMS_ERR_ALLOC;
+  int dev = 0, n_cu = 256;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
   std::vector<uint64_t> acc_hi, acc_lo;  // running unique set (host)
   DiscoverArgs da;
   da.tnodes = s->d_tnodes;
@@ -929,7 +1042,13 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     da.ray_end = re;
     unsigned blocks = (unsigned)((re - rb + 255) / 256);
     HIPCHK(hipEventRecord(e0, st));
-    k_discover<<<blocks, 256, 0, st>>>(da);
+    if (o->flags & 4) {
+      HIPCHK(hipMemsetAsync(d_next, 0, 8, st));
+      unsigned pblocks = std::min<unsigned>(blocks, (unsigned)(n_cu * 8));
+      k_discover_persistent<<<pblocks, 256, 0, st>>>(da, d_next);
+    } else {
+      k_discover<<<blocks, 256, 0, st>>>(da);
+    }
     HIPCHK(hipGetLastError());
     S.n_discover_launches++;
     HIPCHK(hipEventRecord(e1, st));
@@ -1098,7 +1217,9 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ea.counters = d_ctr;
     const int bpt = (ea.items + 511) / 512;                       // blocks per tile slice
     const int block = ((ea.items + bpt - 1) / bpt + 63) / 64 * 64;  // multiple of the wave size
-    const int slice = o->tasks_per_unit > 0 ? o->tasks_per_unit : 16;
+    // tasks per workgroup: longer slices when a tile spans many workgroups
+    // (many Tx elements) to cut the FP64 atomics into H; short ones otherwise
+    const int slice = o->tasks_per_unit > 0 ? o->tasks_per_unit : (bpt >= 8 ? 64 : 16);
     float eval_ms = 0.0f;
     while (!tp.empty()) {
       S.rounds++;

@@ -25,6 +25,15 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (calibrated against a 1 GiB device copy); the table and the commands that
+# produced it are in profiles/hbm_traffic.md.
+MEASURED_TRAFFIC = {
+    ("room_boxes_ms", "k_discover"): {"bytes": 22.1e6 + 114.2e6, "l2_hit": 0.927},
+    ("office_batched", "k_cgemm"): {"bytes": 1596.6e6 + 134.2e6, "l2_hit": 0.859, "mfma_util": 0.446},
+}
 
 
 def kernel_bytes(stats: dict) -> dict:
@@ -114,7 +123,11 @@ def contraction_office(iters: int = 20) -> dict:
     e1.record()
     torch.cuda.synchronize()
     t_step = e0.elapsed_time(e1) / iters * 1e-3
+    tr_meas = MEASURED_TRAFFIC[("office_batched", "k_cgemm")]
     return {"shape": {"M": M, "K": Kc, "N": N}, "fwd_gemm_ms": t_fwd * 1e3,
+            "roofline": {"bound": "mfma", "achieved": 3 * useful / t_fwd / 1e12, "peak": MFMA_BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": 3 * useful / t_fwd / 2.5e15, "traffic": tr_meas["bytes"],
+                         "mfma_busy_pmc": tr_meas["mfma_util"]},
             "useful_tflops": useful / t_fwd / 1e12, "issued_tflops": 3 * useful / t_fwd / 1e12,
             "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": 3 * useful / t_fwd / 2.5e15,
             "f_operand_gb_per_s": 16.0 * Kc * N / t_fwd / 1e9,
@@ -130,6 +143,9 @@ def main():
     ap.add_argument("--opt-steps", type=int, default=500)
     ap.add_argument("--no-opt", action="store_true")
     ap.add_argument("--no-contract", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="short run for rocprof counter passes: 1 traced step, 5 contraction GEMMs and a "
+                         "1 GiB device copy that calibrates FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-fib", type=int, default=1 << 16)
     ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
@@ -144,6 +160,14 @@ def main():
     from automs_amd.tracer import TraceScene, trace
 
     dev = torch.device("cuda:0")
+    if args.profile_pass:
+        args.steps, args.warmup, args.no_opt, args.no_cpu_baseline = 1, 0, True, True
+        x = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)  # calibration: 1 GiB read + 1 GiB written per launch
+        torch.cuda.synchronize()
+        del x, y
     traces = workload_traces(args.workload)
     handles = {k: TraceScene(w.scene) for k, w in traces.items()}
     outs = {k: torch.zeros(w.tx.n, w.rx.n, 2, dtype=torch.float32, device=dev) for k, w in traces.items()}
@@ -210,7 +234,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.workload, dom), {}).get("bytes"),
+            "l2_hit_rate": MEASURED_TRAFFIC.get((args.workload, dom), {}).get("l2_hit"),
             "avg_launch_ms": kt[dom] / max(1, kl[dom]),
             "algorithmic_bytes_per_launch": kb[dom] / max(1, kl[dom]),
         },
@@ -232,7 +257,7 @@ def main():
             "plain_optimum_cov_db": r["plain_optimum_cov_db"], "mfma_best_cov_db": r["mfma_best_cov_db"],
             "start_cov_db": r["start_cov_db"], "codes": 30, "objective": "mean log2(1+SNR), N0 = -90 dBm"}
     if not args.no_contract:
-        r = contraction_office()
+        r = contraction_office(iters=5 if args.profile_pass else 20)
         print(f"contraction: {r}", file=sys.stderr, flush=True)
         line["contraction_office_batched"] = r
     if not args.no_cpu_baseline:

@@ -43,7 +43,9 @@ typedef struct {
   int32_t per_element;    /* 1: fib_n rays from every Tx element; 0: fib_n rays in total,
                              ray r leaving element r % n_tx */
   int32_t flags;          /* bit 0: keep the reference's row wrap of the last tile column;
-                             bit 1: evaluate the round-1 tasks only (no neighbour expansion) */
+                             bit 1: evaluate the round-1 tasks only (no neighbour expansion);
+                             bit 2: wavefront-persistent discovery (ray refill) instead of
+                                    one thread per ray */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
   void *stream;           /* hipStream_t; NULL = default stream */
   uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */
