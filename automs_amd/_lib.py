@@ -38,6 +38,11 @@ class AmsTraceOpts(C.Structure):
         ("keys_cap", C.c_int64),
         ("seed_keys", C.POINTER(C.c_uint64)),
         ("n_seed", C.c_int64),
+        ("discovery_mode", C.c_int32),
+        ("launch_x", C.c_int32),
+        ("launch_y", C.c_int32),
+        ("samples_per_launch", C.c_int32),
+        ("reserve_size", C.c_int32),
     ]
 
 

@@ -144,11 +144,11 @@ def torch_launch_pathtrace(result: torch.Tensor, src: torch.Tensor, dst: torch.T
         raise ValueError("src and dst must be [n0, n1, 3] (optixPathTracer.cpp:814-821)")
     tx = ArraySpec(s, _np(src_dir).reshape(3), int(src_bptype), float(src_r))
     rx = ArraySpec(d, _np(dst_dir).reshape(3), int(dst_bptype), float(dst_r))
-    n_rays = int(samples_per_launch) * int(launch_x) * int(launch_y)
     ts = TraceScene(scene)
     try:
-        _, st, _ = trace(ts, tx, rx, wave_len, int(max_reflection), n_rays, per_element=False,
-                         gridsize=int(dst_girdsize), result=result)
+        _, st, _ = trace(ts, tx, rx, wave_len, int(max_reflection), 0, per_element=False,
+                         gridsize=int(dst_girdsize), result=result,
+                         reference_launch=(launch_x, launch_y, samples_per_launch, reserve_size))
     finally:
         ts.close()
     if optix_log_level > 0:

@@ -142,7 +142,17 @@ int ams_trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, co
     ams::set_error("ams_trace_channel: max_reflection must be in 1..5");
     return AMS_ERR_ARG;
   }
-  if (o->gridsize < 1 || o->fib_n < 1 || !(o->wave_len > 0)) {
+  if (o->discovery_mode == 1 &&
+      (o->launch_x < 1 || o->launch_y < 1 || o->samples_per_launch < 1 || o->reserve_size < 0 ||
+       (long long)o->launch_x * o->launch_y > (1LL << 26))) {
+    ams::set_error("ams_trace_channel: reference-mode launch_x, launch_y, samples_per_launch must be positive");
+    return AMS_ERR_ARG;
+  }
+  if (o->discovery_mode != 0 && o->discovery_mode != 1) {
+    ams::set_error("ams_trace_channel: discovery_mode must be 0 or 1");
+    return AMS_ERR_ARG;
+  }
+  if (o->gridsize < 1 || (o->discovery_mode == 0 && o->fib_n < 1) || !(o->wave_len > 0)) {
     ams::set_error("ams_trace_channel: gridsize, fib_n and wave_len must be positive");
     return AMS_ERR_ARG;
   }

@@ -577,6 +577,127 @@ k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
   if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
 }
 
+// ---- reference-compatible discovery (optixPathTracer.cu:142-190) ----------
+// TEA with 4 rounds (Wheeler and Needham 1994; key words as GPU samplers use
+// them) seeds a per-thread Numerical-Recipes LCG; 24-bit uniforms.
+__device__ __forceinline__ unsigned tea4(unsigned v0, unsigned v1) {
+  unsigned s0 = 0;
+  for (int n = 0; n < 4; n++) {
+    s0 += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+__device__ __forceinline__ float lcg_uniform(unsigned &st) {
+  st = 1664525u * st + 1013904223u;
+  return (float)(st & 0x00FFFFFFu) / (float)0x01000000u;
+}
+
+struct RefArgs {
+  DiscoverArgs d;
+  int launch_x, launch_y, samples, reserve, n_grid;
+  int *min_hc;  // [threads][n_grid], zeroed
+};
+
+// One launch index per thread; the 32-slot branch tree is walked in slot
+// order (a reflection child lands in a higher slot than its parent), so the
+// order of Rx hits -- and with it the per-tile min_hc gating and the
+// reserve cap -- follows the reference.
+__global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
+  const DiscoverArgs &a = ra.d;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  Counters ct;
+  unsigned nrec = 0;
+  if (t < ra.launch_x * ra.launch_y) {
+    const int ix = t / ra.launch_y, iy = t % ra.launch_y;
+    unsigned seed = tea4((unsigned)t, 0u);
+    int *min_hc = ra.min_hc + (size_t)t * ra.n_grid;
+    int res_cnt = 0;
+    Branch slot[1 << AMS_MAX_HIT];
+    for (int smp = 0; smp < ra.samples; smp++) {
+      int src = (int)(lcg_uniform(seed) * (float)a.n_src);
+      if (src >= a.n_src) src = a.n_src - 1;
+      // rand_dir, optixPathTracer.cu:56-68
+      float r1 = (lcg_uniform(seed) + (float)ix) / (float)ra.launch_x;
+      float r2 = (lcg_uniform(seed) + (float)iy) / (float)ra.launch_y;
+      float th = 2.0f * (float)M_PI * r1, rr = sqrtf(r2);
+      float u = rr * cosf(th), v = rr * sinf(th), q = sqrtf(fmaxf(0.0f, 1.0f - rr * rr));
+      for (int k = 0; k < (1 << AMS_MAX_HIT); k++) slot[k].ttl = 0;
+      Branch &s0 = slot[0];
+      s0.o = f3(a.src[3 * src], a.src[3 * src + 1], a.src[3 * src + 2]);
+      s0.d = f3(2.0f * u * q, 2.0f * v * q, 1.0f - 2.0f * rr * rr);
+      s0.ttl = a.K + 1;
+      s0.b = 0;
+      s0.nref = 0;
+      s0.last_pl = -1;
+      s0.khi = (0xFFFFFull << 36) | (0xFFFFFull << 16) | 0xFFFFull;
+      s0.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
+      for (int i = 0; i < (1 << AMS_MAX_HIT); i++) {
+        Branch cur = slot[i];
+        while (cur.ttl > 0) {
+          Ray ray;
+          ray_setup(ray, cur.o, cur.d);
+          float tt = INFINITY;
+          int torig;
+          int sl = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+          float ts = tt;
+          int rx = closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
+          ct.queries++;
+          if (rx >= 0 && ts < tt) {
+            if (res_cnt < ra.reserve) {  // optixPathTracer.cu:173-185
+              int grid = (int)ams_tile_of(rx, a.n1_dst, a.gs);
+              int h = a.K + 1 - cur.ttl;
+              if (min_hc[grid] <= h) {
+                min_hc[grid] = h + 1;
+                uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
+                uint64_t lo = (cur.klo & ~0xFFFFFull) | (uint64_t)grid;
+                unsigned long long idx = wave_append(a.out_count);
+                nrec++;
+                if (idx < a.cap) {
+                  a.out_hi[idx] = hi;
+                  a.out_lo[idx] = lo;
+                }
+                res_cnt++;
+              }
+            }
+            cur.o = cur.o + cur.d * ts;
+            cur.last_pl = -1;
+            continue;
+          }
+          if (sl < 0) break;
+          const AmsTri tr = a.tris[sl];
+          int pl = __float_as_int(tr.b.w);
+          cur.o = cur.o + cur.d * tt;
+          cur.last_pl = pl;
+          if (cur.ttl > 1) {
+            float4 nm = a.aux[torig].n_maxedge;
+            float3 N0 = f3(nm.x, nm.y, nm.z);
+            float3 N = dot3(N0, cur.d) > 0.0f ? N0 * -1.0f : N0;
+            int m = __float_as_int(tr.c.w);
+            int thb = theta_bin(cur.d, N);
+            bool pen = a.mat_prop[thb + 19 * m] < 0.0f;
+            cur.ttl--;
+            int h = a.K + 1 - cur.ttl;
+            int cs = cur.b | (1 << (h - 1));  // optixPathTracer.cu:313
+            Branch ch = cur;
+            ch.b = cs;
+            ch.d = norm3(cur.d - N * (2.0f * dot3(cur.d, N)));
+            if (ch.nref < AMS_MAX_HIT) key_set_plane(ch.khi, ch.klo, ch.nref, (uint32_t)pl);
+            ch.nref++;
+            slot[cs] = ch;
+            if (!pen) break;
+          }
+        }
+      }
+    }
+  }
+  wave_add_counters(a.counters, ct);
+  unsigned long long x = nrec;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+  if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
+}
+
 // ------------------------------------------------------------------------
 // backward evaluation
 // ------------------------------------------------------------------------
@@ -943,7 +1064,9 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
 
   // ---------------- forward discovery in chunks of rays -----------------
-  const long long n_rays = o->per_element ? (long long)n_src * o->fib_n : o->fib_n;
+  const long long n_rays = o->discovery_mode == 1
+                               ? (long long)o->launch_x * o->launch_y * o->samples_per_launch
+                               : (o->per_element ? (long long)n_src * o->fib_n : o->fib_n);
   const long long chunk = std::min<long long>(n_rays, 1LL << 22);
   const unsigned long long cap = (unsigned long long)chunk * 6 + 1024;
   uint64_t *c_hi = (uint64_t *)sc.get("c_hi", cap * 8), *c_lo = (uint64_t *)sc.get("c_lo", cap * 8);
@@ -978,7 +1101,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   da.cap = cap;
   da.counters = d_ctr;
   // coherent ray order: a permutation of the Fibonacci indices, cached per fib_n
-  if (!o->seed_keys && s->perm_n != o->fib_n) {
+  if (!o->seed_keys && o->discovery_mode == 0 && s->perm_n != o->fib_n) {
     long long n = o->fib_n;
     unsigned *keys = (unsigned *)sc.get("perm_keys", 4 * (size_t)n), *idx = (unsigned *)sc.get("perm_idx", 4 * (size_t)n);
     unsigned *k2 = (unsigned *)sc.get("perm_k2", 4 * (size_t)n), *perm = (unsigned *)sc.get("perm", 4 * (size_t)n);
@@ -1035,7 +1158,64 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   uint64_t *a_hi = (uint64_t *)sc.get("acc_hi", acc_cap * 8), *a_lo = (uint64_t *)sc.get("acc_lo", acc_cap * 8);
   if (!a_hi || !a_lo) return AMS_ERR_ALLOC;
   long long cur_chunk = chunk;
-  for (long long rb = o->seed_keys ? n_rays : 0; rb < n_rays;) {
+  if (o->discovery_mode == 1 && !o->seed_keys) {
+    // reference-compatible rays: one launch over launch_x x launch_y threads
+    long long nthr = (long long)o->launch_x * o->launch_y;
+    unsigned long long rcap = (unsigned long long)nthr * (unsigned long long)std::max(0, o->reserve_size) + 1024;
+    int *d_minhc = (int *)sc.get("min_hc", sizeof(int) * (size_t)nthr * (size_t)n_grid);
+    uint64_t *r_hi = (uint64_t *)sc.get("r_hi", rcap * 8), *r_lo = (uint64_t *)sc.get("r_lo", rcap * 8);
+    if (!d_minhc || !r_hi || !r_lo) return AMS_ERR_ALLOC;
+    HIPCHK(hipMemsetAsync(d_minhc, 0, sizeof(int) * (size_t)nthr * (size_t)n_grid, st));
+    HIPCHK(hipMemsetAsync(d_cnt, 0, 8, st));
+    RefArgs ra;
+    ra.d = da;
+    ra.d.out_hi = r_hi;
+    ra.d.out_lo = r_lo;
+    ra.d.cap = rcap;
+    ra.launch_x = o->launch_x;
+    ra.launch_y = o->launch_y;
+    ra.samples = o->samples_per_launch;
+    ra.reserve = o->reserve_size;
+    ra.n_grid = (int)n_grid;
+    ra.min_hc = d_minhc;
+    HIPCHK(hipEventRecord(e0, st));
+    k_discover_ref<<<(unsigned)((nthr + 255) / 256), 256, 0, st>>>(ra);
+    HIPCHK(hipGetLastError());
+    S.n_discover_launches++;
+    HIPCHK(hipEventRecord(e1, st));
+    unsigned long long hcnt = 0;
+    HIPCHK(hipMemcpyAsync(&hcnt, d_cnt, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    disc_ms += ms;
+    if (hcnt > rcap) {
+      set_error("reference-mode records exceed launch_x * launch_y * reserve_size");
+      return AMS_ERR_OVERFLOW;
+    }
+    size_t nu = 0;
+    int rc = sort_unique(sc, r_hi, r_lo, (size_t)hcnt, st, &nu);
+    if (rc) return rc;
+    S.records_raw += (long long)hcnt;
+    if (nu > acc_cap) {
+      uint64_t *nh = (uint64_t *)dmalloc(nu * 8), *nl = (uint64_t *)dmalloc(nu * 8);
+      if (!nh || !nl) {
+        dfree(nh);
+        dfree(nl);
+        set_error("device allocation failed: record set");
+        return AMS_ERR_ALLOC;
+      }
+      sc.adopt("acc_hi", nh, nu * 8);
+      sc.adopt("acc_lo", nl, nu * 8);
+      a_hi = nh;
+      a_lo = nl;
+      acc_cap = nu;
+    }
+    HIPCHK(hipMemcpyAsync(a_hi, r_hi, nu * 8, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(a_lo, r_lo, nu * 8, hipMemcpyDeviceToDevice, st));
+    acc_n = nu;
+  }
+  for (long long rb = (o->seed_keys || o->discovery_mode == 1) ? n_rays : 0; rb < n_rays;) {
     long long re = std::min(n_rays, rb + cur_chunk);
     HIPCHK(hipMemsetAsync(d_cnt, 0, 8, st));
     da.ray_begin = rb;

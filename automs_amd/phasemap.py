@@ -195,8 +195,10 @@ class PhaseMapModel(nn.Module):
     def __init__(self, name: str, channels: Dict[str, torch.Tensor], ms_list: List[Metasurface],
                  code_num: int = 30, w: float = 1.0, objective: str = "logsnr", n0: float = N0_90DBM,
                  n_maps: int = 1, optimize_codebook: bool = True, antenna_theta: Optional[torch.Tensor] = None,
-                 ms_theta: Optional[List[torch.Tensor]] = None, plain: bool = False, seed: int = 0):
+                 ms_theta: Optional[List[torch.Tensor]] = None, plain: bool = False, seed: int = 0,
+                 max_ms_reflex: int = 1):
         super().__init__()
+        self.max_ms_reflex = max_ms_reflex
         dev = channels["src_to_dst"].device
         self.name = name
         self.ms_list = ms_list
@@ -223,24 +225,50 @@ class PhaseMapModel(nn.Module):
             t0 = ms_theta[i] if ms_theta is not None else torch.zeros(n_maps, K)
             self.ms_theta.append(nn.Parameter(t0.to(dev, torch.float32).reshape(n_maps, K).clone()))
         self.G_list = [channels["src_to_ms{%d}" % i].to(torch.complex64) for i in range(len(ms_list))]
+        # metasurface-to-metasurface channels; ms{b}->ms{a} for b > a is the
+        # transpose (not the conjugate) of ms{a}->ms{b}, as go.py:147-153 uses it
+        self.T = {}
+        for a in range(len(ms_list)):
+            for b in range(len(ms_list)):
+                if a < b and ("ms{%d}_to_ms{%d}" % (a, b)) in channels:
+                    self.T[(a, b)] = channels["ms{%d}_to_ms{%d}" % (a, b)].to(torch.complex64)
+                    self.T[(b, a)] = self.T[(a, b)].T
         F_list = [channels["ms{%d}_to_dst" % i] for i in range(len(ms_list))]
         self.op = ChannelOperand(F_list, channels["src_to_dst"], keep_plain=plain)
         self.loss_list: List[float] = []
         self.rowmask_fn = None
 
     def stacked_operand(self) -> torch.Tensor:
-        """A [n_maps * C, Kc] = [ (W G_i) diag(u_i) ... | W ] per map."""
+        """A [n_maps * C, Kc] = [ A_0 | A_1 | ... | W ] per map, where A_b
+        collects every metasurface chain ending at b (go.py:107-157):
+        P1_b = (W G_b) diag(u_b),  Pr_b = sum_{a != b} (P(r-1)_a T_ab) diag(u_b),
+        A_b = sum_{r <= max_ms_reflex} Pr_b; so y = A . [F ; H_d]."""
         W = self.antenna_A * torch.exp(-1j * self.antenna_theta)  # go.py:163
         rows = []
+        n = len(self.ms_list)
         for m in range(self.n_maps):
-            parts = []
+            u = []
             for i, ms in enumerate(self.ms_list):
                 phase = torch.remainder(self.ms_theta[i][m], ms.max_phase_rad)  # go.py:131
-                u = self.ms_A[i] * torch.exp(1j * phase)
-                parts.append((W @ self.G_list[i]) * u)
-            parts.append(W.to(torch.complex64))
-            rows.append(torch.cat(parts, 1))
+                u.append(self.ms_A[i] * torch.exp(1j * phase))
+            P = [(W @ self.G_list[i]) * u[i] for i in range(n)]
+            acc = list(P)
+            for _ in range(2, self.max_ms_reflex + 1):
+                P = [sum(P[a] @ self.T[(a, b)] for a in range(n) if a != b and (a, b) in self.T) * u[b]
+                     if any(a != b and (a, b) in self.T for a in range(n)) else torch.zeros_like(P[b])
+                     for b in range(n)]
+                acc = [acc[b] + P[b] for b in range(n)]
+            rows.append(torch.cat(acc + [W.to(torch.complex64)], 1))
         return torch.cat(rows, 0)
+
+    @torch.no_grad()
+    def postprocess(self):
+        """go.py:89-97: AP phases rounded to multiples of pi/2, metasurface
+        phases wrapped to [0, max_phase)."""
+        self.antenna_theta.copy_(torch.round(self.antenna_theta / (math.pi / 2)) * (math.pi / 2))
+        for i, ms in enumerate(self.ms_list):
+            self.ms_theta[i].copy_(torch.remainder(self.ms_theta[i], ms.max_phase_rad))
+        return self.forward()
 
     def received(self) -> torch.Tensor:
         A = self.stacked_operand()

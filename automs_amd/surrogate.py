@@ -268,21 +268,47 @@ class PlacementTracer:
 
 
 def generate_dataset(n_scenes: int, per_scene: int, seed: int = 0, fib_ap: int = 1 << 18, fib_ms: int = 1 << 18,
-                     log=None):
-    """[N, 9, 64, 64] features and [N, 1, 64, 64] traced gain maps (GPU)."""
-    rng = SplitMix64(seed)
-    Xs, Ys = [], []
-    for sidx in range(n_scenes):
-        occ = occupancy(sidx)
+                     log=None, first_scene: int = 0):
+    """Traced gain maps of random placements: returns (placements [N, 3]
+    (wall, s, z), scene seeds [N], gains [N, 64, 64] f32 on the GPU)."""
+    rng = SplitMix64(seed * 1000003 + first_scene)
+    P, S_, Ys = [], [], []
+    for sidx in range(first_scene, first_scene + n_scenes):
         pt = PlacementTracer(sidx, fib_ap, fib_ms)
         for _ in range(per_scene):
             p = random_placement(rng)
-            Xs.append(torch.from_numpy(features(occ, p)))
-            Ys.append(pt.gain_map(p).unsqueeze(0))
+            P.append([p.wall, p.s, p.z])
+            S_.append(sidx)
+            Ys.append(pt.gain_map(p))
         if log:
-            log(f"scene {sidx + 1}/{n_scenes}: {pt.n_traces} placements traced in {pt.gpu_seconds:.1f} s")
+            log(f"scene {sidx + 1 - first_scene}/{n_scenes}: {pt.n_traces} placements traced in {pt.gpu_seconds:.1f} s")
         del pt
-    return torch.stack(Xs).cuda(), torch.stack(Ys).float()
+    return np.asarray(P, np.float32), np.asarray(S_, np.int32), torch.stack(Ys).float()
+
+
+def build_inputs(placements: np.ndarray, scene_ids: np.ndarray) -> torch.Tensor:
+    """[N, 9, 64, 64] network inputs on the GPU."""
+    occ = {int(s): occupancy(int(s)) for s in np.unique(scene_ids)}
+    X = np.stack([features(occ[int(s)], Placement(int(p[0]), float(p[1]), float(p[2])))
+                  for p, s in zip(placements, scene_ids)])
+    return torch.from_numpy(X).cuda()
+
+
+def save_dataset(path: str, placements, scene_ids, gains: torch.Tensor) -> None:
+    """Gains quantised to uint8 in 40/255 dB steps (the clamp range)."""
+    q = torch.round(gains.clamp(0, 40) * (255 / 40)).to(torch.uint8).cpu().numpy()
+    np.savez_compressed(path, placements=placements, scene_ids=scene_ids, gains_u8=q)
+
+
+def load_dataset(paths: List[str]):
+    P, S_, G = [], [], []
+    for p in paths:
+        d = np.load(p)
+        P.append(d["placements"])
+        S_.append(d["scene_ids"])
+        G.append(d["gains_u8"])
+    g = torch.from_numpy(np.concatenate(G)).float() * (40 / 255)
+    return np.concatenate(P), np.concatenate(S_), g.cuda()
 
 
 def train(net: SurrogateNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20, batch: int = 64, lr: float = 2e-3,
@@ -344,31 +370,39 @@ def anneal(score_fn, steps: int, seed: int, t0: float = 1.0, t1: float = 0.01, r
 
 
 def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 2000, search_scene: int = 1000,
-                   epochs: int = 20, fib: int = 1 << 18, seed: int = 0, log=print) -> dict:
+                   epochs: int = 20, fib: int = 1 << 18, seed: int = 0, log=print, datasets=None,
+                   t0: float = 0.3, t1: float = 0.003) -> dict:
     """Config-4 experiment: traced dataset -> trained surrogate -> placement
     search three ways on an unseen scene: plain annealing (every candidate
     traced), surrogate-guided annealing (accepted candidates traced), and
     plain annealing cut to the surrogate search's number of traces."""
-    out: Dict = {"scenes": n_scenes, "per_scene": per_scene, "sa_steps": sa_steps, "search_scene": search_scene}
-    t0 = time.perf_counter()
-    X, Y = generate_dataset(n_scenes, per_scene, seed=seed, fib_ap=fib, fib_ms=fib, log=log)
-    torch.cuda.synchronize()
-    out["dataset_seconds"] = time.perf_counter() - t0
+    out: Dict = {"sa_steps": sa_steps, "search_scene": search_scene, "t0_db": t0, "t1_db": t1}
+    t_start = time.perf_counter()
+    if datasets:
+        P, S_, Y = load_dataset(datasets)
+        out["dataset"] = {"files": len(datasets), "seconds": None}
+    else:
+        P, S_, Y = generate_dataset(n_scenes, per_scene, seed=seed, fib_ap=fib, fib_ms=fib, log=log)
+        torch.cuda.synchronize()
+        out["dataset"] = {"seconds": time.perf_counter() - t_start}
+    X = build_inputs(P, S_)
+    Y = Y.reshape(-1, 1, GRID, GRID)
+    out["dataset"].update(samples=int(X.shape[0]), scenes=int(len(np.unique(S_))))
     n = X.shape[0]
     n_val = max(64, n // 20)
     Xtr, Ytr, Xva, Yva = X[:-n_val], Y[:-n_val], X[-n_val:], Y[-n_val:]
     net = SurrogateNet(seed=seed)
-    t0 = time.perf_counter()
+    tt = time.perf_counter()
     hist = train(net, Xtr, Ytr, epochs=epochs, seed=seed, log=log)
     torch.cuda.synchronize()
-    out["train_seconds"] = time.perf_counter() - t0
+    out["train_seconds"] = time.perf_counter() - tt
     with torch.no_grad():
         pv = torch.cat([net.forward(Xva[i:i + 64]) for i in range(0, n_val, 64)]) * 10
         mse_val = float(((pv - Yva) ** 2).mean())
         base = float(((Ytr.mean() - Yva) ** 2).mean())
         sv, tv = pv.mean((1, 2, 3)), Yva.mean((1, 2, 3))
         r2_score = 1 - float(((sv - tv) ** 2).sum() / ((tv - tv.mean()) ** 2).sum())
-    out.update(train_mse_final=hist[-1] * 100, val_mse_db2=mse_val, val_mse_mean_predictor_db2=base,
+    out.update(train_mse_final_db2=hist[-1] * 100, val_mse_db2=mse_val, val_mse_mean_predictor_db2=base,
                val_score_r2=r2_score)
     log(f"validation: map mse {mse_val:.2f} dB^2 (mean predictor {base:.2f}); placement-score R^2 {r2_score:.3f}")
 
@@ -378,31 +412,28 @@ def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 20
     def traced(p):
         return float(pt.gain_map(p).mean())
 
-    # plain search: every candidate traced
-    pt.gpu_seconds, pt.n_traces = 0.0, 0
-    t0 = time.perf_counter()
-    b1, v1, a1 = anneal(traced, sa_steps, seed=seed + 1)
-    out["plain"] = {"best_gain_db": v1, "seconds": time.perf_counter() - t0, "traces": pt.n_traces,
-                    "accepted": a1, "best": b1.__dict__}
-    log(f"plain search: {v1:.3f} dB in {out['plain']['seconds']:.1f} s ({pt.n_traces} traces)")
-    # surrogate-guided search
-    pt.gpu_seconds, pt.n_traces = 0.0, 0
-    t0 = time.perf_counter()
-    b2, v2, a2 = anneal(lambda p: float(predict_score(net, occ, [p])[0]), sa_steps, seed=seed + 1, rescore=traced)
-    torch.cuda.synchronize()
-    out["surrogate"] = {"best_gain_db": v2, "seconds": time.perf_counter() - t0, "traces": pt.n_traces,
-                        "accepted": a2, "best": b2.__dict__}
-    log(f"surrogate search: {v2:.3f} dB in {out['surrogate']['seconds']:.1f} s ({pt.n_traces} traces)")
-    # plain search with the surrogate search's trace budget
-    budget = max(1, pt.n_traces - 1)
-    pt.gpu_seconds, pt.n_traces = 0.0, 0
-    t0 = time.perf_counter()
-    b3, v3, a3 = anneal(traced, budget, seed=seed + 1)
-    out["plain_same_budget"] = {"best_gain_db": v3, "seconds": time.perf_counter() - t0, "traces": pt.n_traces,
-                                "steps": budget}
-    log(f"plain search, same trace budget: {v3:.3f} dB in {out['plain_same_budget']['seconds']:.1f} s")
+    def timed(fn):
+        torch.cuda.synchronize()
+        pt.gpu_seconds, pt.n_traces = 0.0, 0
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t, pt.n_traces
+
+    (b1, v1, a1), s1, n1 = timed(lambda: anneal(traced, sa_steps, seed=seed + 1, t0=t0, t1=t1))
+    out["plain"] = {"best_gain_db": v1, "seconds": s1, "traces": n1, "accepted": a1, "best": b1.__dict__}
+    log(f"plain search: {v1:.3f} dB in {s1:.1f} s ({n1} traces)")
+    (b2, v2, a2), s2, n2 = timed(lambda: anneal(lambda p: float(predict_score(net, occ, [p])[0]), sa_steps,
+                                                seed=seed + 1, t0=t0, t1=t1, rescore=traced))
+    out["surrogate"] = {"best_gain_db": v2, "seconds": s2, "traces": n2, "accepted": a2, "best": b2.__dict__}
+    log(f"surrogate search: {v2:.3f} dB in {s2:.1f} s ({n2} traces)")
+    budget = max(1, n2 - 1)
+    (b3, v3, a3), s3, n3 = timed(lambda: anneal(traced, budget, seed=seed + 1, t0=t0, t1=t1))
+    out["plain_same_budget"] = {"best_gain_db": v3, "seconds": s3, "traces": n3, "steps": budget}
+    log(f"plain search with the same trace budget: {v3:.3f} dB in {s3:.1f} s")
     out["gain_gap_db"] = v1 - v2
-    out["time_ratio"] = out["surrogate"]["seconds"] / out["plain"]["seconds"]
+    out["time_ratio"] = s2 / s1
+    out["surrogate_gain_over_same_budget_db"] = v2 - v3
     return out
 
 
@@ -418,9 +449,20 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--fib", type=int, default=1 << 18)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--make-dataset", default=None, help="only trace a dataset and save it (npz)")
+    ap.add_argument("--first-scene", type=int, default=0)
+    ap.add_argument("--datasets", default=None, help="comma-separated npz files to train on")
     a = ap.parse_args()
-    res = run_experiment(a.scenes, a.per_scene, a.sa_steps, epochs=a.epochs, fib=a.fib,
-                         log=lambda m: print(m, file=sys.stderr, flush=True))
+    logf = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.make_dataset:
+        t = time.perf_counter()
+        P, S_, Y = generate_dataset(a.scenes, a.per_scene, fib_ap=a.fib, fib_ms=a.fib, log=logf,
+                                    first_scene=a.first_scene)
+        save_dataset(a.make_dataset, P, S_, Y)
+        print(json.dumps({"samples": len(P), "seconds": time.perf_counter() - t}), flush=True)
+        sys.exit(0)
+    res = run_experiment(a.scenes, a.per_scene, a.sa_steps, epochs=a.epochs, fib=a.fib, log=logf,
+                         datasets=a.datasets.split(",") if a.datasets else None)
     txt = json.dumps(res)
     print(txt, flush=True)
     if a.out:

@@ -66,9 +66,15 @@ def _array(spec: ArraySpec, keep: list) -> _lib.AmsArray:
 
 def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_reflection: int, fib_n: int,
           per_element: bool = True, gridsize: int = 5, flags: int = 0, result: Optional[torch.Tensor] = None,
-          keys_cap: int = 0, tasks_per_unit: int = 0, seed_keys=None, device: str = "cuda"):
+          keys_cap: int = 0, tasks_per_unit: int = 0, seed_keys=None, device: str = "cuda",
+          reference_launch: Optional[tuple] = None):
     """Trace Tx -> Rx and accumulate into result (f32 [n_tx, n_rx, 2] on the
-    GPU, zero-initialised if None).  Returns (result, stats dict, keys or None)."""
+    GPU, zero-initialised if None).  Returns (result, stats dict, keys or None).
+
+    reference_launch = (launch_x, launch_y, samples_per_launch, reserve_size)
+    selects the reference's own discovery (optixPathTracer.cu:142-190: TEA/LCG
+    random rays per launch index, per-tile min-hit gating, reserve cap)
+    instead of the Fibonacci ray set; fib_n is then ignored."""
     if result is None:
         result = torch.zeros(tx.n, rx.n, 2, dtype=torch.float32, device=device)
     if result.dtype != torch.float32 or not result.is_contiguous() or not result.is_cuda or \
@@ -84,6 +90,10 @@ def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_ref
     o.per_element = 1 if per_element else 0
     o.flags = int(flags)
     o.tasks_per_unit = int(tasks_per_unit)
+    if reference_launch is not None:
+        lx, ly, spl, res = (int(v) for v in reference_launch)
+        o.discovery_mode = 1
+        o.launch_x, o.launch_y, o.samples_per_launch, o.reserve_size = lx, ly, spl, res
     o.stream = C.c_void_p(torch.cuda.current_stream(result.device).cuda_stream)
     keys = None
     if keys_cap > 0:

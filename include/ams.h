@@ -53,6 +53,11 @@ typedef struct {
   const uint64_t *seed_keys; /* optional HOST [n_seed][2] (path, tile) keys used instead of
                                 forward discovery */
   int64_t n_seed;
+  int32_t discovery_mode;     /* 0: Fibonacci rays (fib_n, per_element); 1: the reference's
+                                 rays: launch_x x launch_y threads x samples_per_launch, TEA/LCG
+                                 random Tx and stratified directions, per-thread min_hc gating
+                                 and reserve_size records (optixPathTracer.cu:142-190) */
+  int32_t launch_x, launch_y, samples_per_launch, reserve_size;
 } ams_trace_opts;
 
 typedef struct {

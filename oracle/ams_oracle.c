@@ -1069,3 +1069,141 @@ int or_trace_channel(void *scene, const float *src, int n0_src, int n1_src, cons
   (void)n_dst;
   return 0;
 }
+
+/* ------------------------------------ reference-compatible discovery ---- */
+/* optixPathTracer.cu:142-190 with random.h's generators: TEA (4 rounds) seeds
+ * a Numerical-Recipes LCG per launch index; the ray's Tx element and the
+ * stratified direction of rand_dir are computed in f32 as the reference does,
+ * then traced in FP64; the 32-slot branch tree is walked in slot order with
+ * per-tile min_hc gating and the reserve cap. */
+static unsigned or_tea4(unsigned v0, unsigned v1) {
+  unsigned s0 = 0;
+  for (int n = 0; n < 4; n++) {
+    s0 += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+static float or_rnd(unsigned *st) {
+  *st = 1664525u * *st + 1013904223u;
+  return (float)(*st & 0x00FFFFFFu) / (float)0x01000000u;
+}
+
+long or_discover_ref(void *scene, const float *src, int n_src, const float *dst, int n0_dst, int n1_dst,
+                     double dst_r, int max_reflection, int gridsize, int launch_x, int launch_y, int samples,
+                     int reserve, uint64_t *out, long cap, long *stats) {
+  or_scene *s = (or_scene *)scene;
+  or_spheres *rx = spheres_create(n0_dst * n1_dst, dst, dst_r);
+  int n1g = (n1_dst - 1) / gridsize + 1, n0g = (n0_dst - 1) / gridsize + 1, n_grid = n0g * n1g;
+  long nthr = (long)launch_x * launch_y;
+  int K = max_reflection;
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  kbuf *bufs = calloc(nt, sizeof(kbuf));
+  long nq = 0;
+#pragma omp parallel reduction(+ : nq)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    kbuf *kb = &bufs[tid];
+    int *min_hc = malloc(sizeof(int) * n_grid);
+    fbranch slot[1 << OR_MAX_HIT];
+#pragma omp for schedule(dynamic, 16)
+    for (long t = 0; t < nthr; t++) {
+      int ix = (int)(t / launch_y), iy = (int)(t % launch_y);
+      unsigned seed = or_tea4((unsigned)t, 0u);
+      memset(min_hc, 0, sizeof(int) * n_grid);
+      int res_cnt = 0;
+      for (int smp = 0; smp < samples; smp++) {
+        int e = (int)(or_rnd(&seed) * (float)n_src);
+        if (e >= n_src) e = n_src - 1;
+        float r1 = (or_rnd(&seed) + (float)ix) / (float)launch_x;
+        float r2 = (or_rnd(&seed) + (float)iy) / (float)launch_y;
+        float th = 2.0f * (float)M_PI * r1, rr = sqrtf(r2);
+        float u = rr * cosf(th), v = rr * sinf(th), q = sqrtf(fmaxf(0.0f, 1.0f - rr * rr));
+        for (int k = 0; k < (1 << OR_MAX_HIT); k++) slot[k].ttl = 0;
+        slot[0].o = mk(src[3 * e], src[3 * e + 1], src[3 * e + 2]);
+        slot[0].d = nrm(mk(2.0f * u * q, 2.0f * v * q, 1.0f - 2.0f * rr * rr));
+        slot[0].ttl = K + 1;
+        slot[0].b = 0;
+        slot[0].nref = 0;
+        slot[0].last_pl = -1;
+        for (int k = 0; k < OR_MAX_HIT; k++) slot[0].pl[k] = OR_NO_PLANE;
+        for (int i = 0; i < (1 << OR_MAX_HIT); i++) {
+          fbranch cur = slot[i];
+          while (cur.ttl > 0) {
+            tri_q tq = closest_tri(s, cur.o, cur.d, OR_TMIN, INFINITY, cur.last_pl);
+            sph_q sq = closest_sphere(rx, cur.o, cur.d, OR_TMIN, tq.t);
+            nq++;
+            if (sq.idx >= 0 && sq.t < tq.t) {
+              if (res_cnt < reserve) {
+                int grid = (int)tile_of(sq.idx, n1_dst, gridsize);
+                int h = K + 1 - cur.ttl;
+                if (min_hc[grid] <= h) {
+                  min_hc[grid] = h + 1;
+                  uint64_t hi, lo;
+                  key_pack(h, cur.b, cur.pl, (unsigned)grid, &hi, &lo);
+                  kbuf_push(kb, hi, lo);
+                  res_cnt++;
+                }
+              }
+              cur.o = add(cur.o, scl(cur.d, sq.t));
+              cur.last_pl = -1;
+              continue;
+            }
+            if (tq.tri < 0) break;
+            int tri = tq.tri;
+            cur.o = add(cur.o, scl(cur.d, tq.t));
+            cur.last_pl = s->plane[tri];
+            if (cur.ttl > 1) {
+              v3 N0 = s->n0[tri];
+              v3 N = dot(N0, scl(cur.d, -1.0)) < 0 ? scl(N0, -1.0) : N0;
+              int m = s->mat[tri];
+              int thb = theta_bin(cur.d, N);
+              int pen = s->mat_prop[thb + 19 * m] < 0;
+              cur.ttl--;
+              int h = K + 1 - cur.ttl;
+              int cs = cur.b | (1 << (h - 1));
+              fbranch ch = cur;
+              ch.b = cs;
+              ch.d = sub(cur.d, scl(N, 2.0 * dot(cur.d, N)));
+              if (ch.nref < OR_MAX_HIT) ch.pl[ch.nref] = (unsigned)s->plane[tri];
+              ch.nref++;
+              slot[cs] = ch;
+              if (!pen) break;
+            }
+          }
+        }
+      }
+    }
+    free(min_hc);
+  }
+  long total = 0;
+  for (int t = 0; t < nt; t++) total += bufs[t].n;
+  key128 *all = malloc(sizeof(key128) * (total + 1));
+  long off = 0;
+  for (int t = 0; t < nt; t++) {
+    memcpy(all + off, bufs[t].k, sizeof(key128) * bufs[t].n);
+    off += bufs[t].n;
+    free(bufs[t].k);
+  }
+  free(bufs);
+  long m = key_unique(all, total);
+  if (stats) {
+    stats[0] = nthr * samples;
+    stats[1] = nq;
+    stats[2] = m;
+    stats[3] = total;
+  }
+  long ret = m <= cap ? m : -1;
+  if (ret >= 0)
+    for (long i = 0; i < m; i++) { out[2 * i] = all[i].hi; out[2 * i + 1] = all[i].lo; }
+  free(all);
+  spheres_destroy(rx);
+  return ret;
+}

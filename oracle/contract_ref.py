@@ -38,6 +38,33 @@ def received(antenna_A: torch.Tensor, antenna_theta: torch.Tensor, H_d: torch.Te
     return torch.cat(out, 0)
 
 
+def received_chains(antenna_A, antenna_theta, channels: dict, ms_theta, max_phase, max_ms_reflex: int):
+    """go.py:107-164 literally: every chain of metasurfaces of length
+    1..max_ms_reflex without consecutive repeats (itertools.product), with
+    ms{b}->ms{a} for b > a taken as the transpose of ms{a}->ms{b}."""
+    import itertools
+    W = antenna_A.to(torch.float64) * torch.exp(-1j * antenna_theta.to(torch.float64))
+    ch = channels["src_to_dst"].to(torch.complex128)
+    n = len(ms_theta)
+    for L in range(1, max_ms_reflex + 1):
+        for path in itertools.product(range(n), repeat=L):
+            if any(path[i] == path[i + 1] for i in range(len(path) - 1)):
+                continue
+            m = channels["src_to_ms{%d}" % path[0]].to(torch.complex128)
+            for i, ms_id in enumerate(path):
+                u = torch.exp(1j * torch.remainder(ms_theta[ms_id].to(torch.float64), max_phase[ms_id]))
+                m = m * u
+                if i == len(path) - 1:
+                    m = m @ channels["ms{%d}_to_dst" % ms_id].to(torch.complex128)
+                else:
+                    nxt = path[i + 1]
+                    T = channels["ms{%d}_to_ms{%d}" % (ms_id, nxt)] if ms_id < nxt else \
+                        channels["ms{%d}_to_ms{%d}" % (nxt, ms_id)].T
+                    m = m @ T.to(torch.complex128)
+            ch = ch + m
+    return W @ ch
+
+
 def loss_logsnr(Y: torch.Tensor, groups: int, n0: float) -> torch.Tensor:
     """-(1/N) sum_r log2(1 + max_c |y|^2 / n0) per group, summed."""
     GC, N = Y.shape

@@ -58,6 +58,9 @@ def lib():
                                        C.c_double, C.c_double, fp, fp, C.c_int, C.c_int, C.c_int, C.c_long, C.c_int,
                                        C.c_int, u64p, C.c_long, dp, lp, u64p, C.c_long, lp]
         L.or_trace_channel.restype = C.c_int
+        L.or_discover_ref.argtypes = [vp, fp, C.c_int, fp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, u64p, C.c_long, lp]
+        L.or_discover_ref.restype = C.c_long
         _lib = L
     return _lib
 
@@ -151,6 +154,23 @@ def discover(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, fib_n, 
     if m < 0:
         raise RuntimeError("oracle discovery cap exceeded")
     return out[:m].copy(), dict(rays=int(st[0]), queries=int(st[1]), unique=int(st[2]), ties=int(st[3]))
+
+
+def discover_ref(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, launch_x, launch_y, samples,
+                 reserve, cap=1 << 24):
+    """Reference-compatible discovery (TEA/LCG rays, min_hc gating, reserve)."""
+    src_a, sp = _f(np.asarray(src).reshape(-1, 3))
+    dst = np.asarray(dst)
+    n0, n1 = dst.shape[0], dst.shape[1]
+    dst_a, dp = _f(dst.reshape(-1, 3))
+    out = np.zeros((cap, 2), np.uint64)
+    st = np.zeros(4, np.int64)
+    m = lib().or_discover_ref(sc.h, sp, src_a.shape[0], dp, n0, n1, dst_r, max_reflection, gridsize, launch_x,
+                              launch_y, samples, reserve, out.ctypes.data_as(C.POINTER(C.c_uint64)), cap,
+                              st.ctypes.data_as(C.POINTER(C.c_long)))
+    if m < 0:
+        raise RuntimeError("oracle discovery cap exceeded")
+    return out[:m].copy(), dict(rays=int(st[0]), queries=int(st[1]), unique=int(st[2]), raw=int(st[3]))
 
 
 def trace_ray_log(sc: OracleScene, o, d, dst, dst_r, max_reflection, gridsize, cap=256):

@@ -175,3 +175,31 @@ def test_batched_maps_match_single(gpu):
         ls += one().item()
     _flush(f"case: batched maps {lb:.6f} vs {ls:.6f}")
     assert abs(lb - ls) / abs(ls) < 1e-5
+
+
+@pytest.mark.parametrize("reflex", [1, 2, 3])
+def test_metasurface_cascades_vs_reference_enumeration(gpu, reflex):
+    """Three metasurfaces with ms-to-ms channels: the recursive stacked
+    operand equals go.py's literal chain enumeration (max_ms_reflex 1..3)."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    from oracle import contract_ref as R
+    _flush(f"case: metasurface cascades max_ms_reflex={reflex}")
+    M, N, Ks = 16, 300, [40, 56, 48]
+    ch64 = {"src_to_dst": _crand(M, N, scale=3e-5, seed=30)}
+    for i, K in enumerate(Ks):
+        ch64["src_to_ms{%d}" % i] = _crand(M, K, scale=3e-4, seed=31 + i)
+        ch64["ms{%d}_to_dst" % i] = _crand(K, N, scale=3e-4, seed=41 + i)
+    for a in range(3):
+        for b in range(a + 1, 3):
+            ch64["ms{%d}_to_ms{%d}" % (a, b)] = _crand(Ks[a], Ks[b], scale=1e-2, seed=50 + 3 * a + b)
+    ch = {k: v.to(torch.complex64).to(gpu) for k, v in ch64.items()}
+    ms = [Metasurface(None, (1, K), None, None, [], max_phase=300.0) for K in Ks]
+    g = torch.Generator().manual_seed(11)
+    th = [torch.rand(1, K, generator=g) * 2 * math.pi for K in Ks]
+    model = PhaseMapModel("c", ch, ms, code_num=6, ms_theta=th, max_ms_reflex=reflex)
+    Y = model.received().cpu().to(torch.complex128)
+    ref = R.received_chains(model.antenna_A.cpu(), model.antenna_theta.detach().cpu(), ch64,
+                            [t.reshape(-1) for t in th], [300 / 180 * math.pi] * 3, reflex)
+    rel = (Y - ref).norm() / ref.norm()
+    _flush(f"  rel err {rel:.2e}")
+    assert rel < 3e-5

@@ -221,3 +221,30 @@ def test_torch_op_schema_registered():
                      "samples_per_launch", "launch_x", "launch_y", "wave_len", "max_reflection", "reserve_size",
                      "mat_prop", "src_dir", "dst_dir", "optix_log_level", "src_bptype", "dst_bptype",
                      "dst_girdsize", "src_r", "dst_r"]
+
+
+def test_surrogate_placements_and_features():
+    """Placement poses on the four walls, neighbour moves stay in range, and
+    the 9 input channels follow their definitions."""
+    from automs_amd import surrogate as S
+    from automs_amd.scenes import SplitMix64
+    poses = [S.Placement(w, 0.5, 1.5).pose() for w in range(4)]
+    assert poses[0][:2] == (0.0, 4.0) and poses[1][:2] == (10.0, 4.0)
+    assert poses[2][:2] == (5.0, 0.0) and poses[3][:2] == (5.0, 8.0)
+    rng = SplitMix64(0)
+    p = S.Placement(1, 0.5, 1.5)
+    for _ in range(200):
+        p = S.neighbour(p, rng)
+        assert p.wall in (0, 1, 2, 3) and 0.1 <= p.s <= 0.9 and 0.6 <= p.z <= 2.4
+    pos, bh, nrm = S.Placement(3, 0.25, 2.0).ms()
+    assert pos.shape == (32, 32, 3) and np.allclose(nrm, [0, -1, 0], atol=1e-6)
+    assert np.allclose(pos[..., 1], 8.0 - 0.001, atol=1e-5)
+    occ = S.occupancy(0)
+    assert occ.shape == (64, 64) and 0 < occ.max() <= 0.5 + 1e-6
+    f = S.features(occ, S.Placement(1, 0.5, 1.5))
+    assert f.shape == (9, 64, 64)
+    assert np.allclose(f[0], occ) and np.isclose(f[7].mean(), 0.5)
+    # the Rx closest to the metasurface (x = 9.9, y ~ 4) has the largest cosine to its normal
+    r = f[5]
+    i, j = np.unravel_index(np.argmin(r), r.shape)
+    assert i == 63 and abs(j - 32) <= 1

@@ -49,7 +49,9 @@ def test_conv_net_forward_backward_vs_oracle(gpu):
 def test_surrogate_learns_traced_gain_maps(gpu):
     from automs_amd import surrogate as S
     _flush("case: surrogate training on a small traced dataset")
-    X, Y = S.generate_dataset(2, 96, seed=5, fib_ap=1 << 16, fib_ms=1 << 16, log=_flush)
+    P, Sid, Y = S.generate_dataset(2, 96, seed=5, fib_ap=1 << 16, fib_ms=1 << 16, log=_flush)
+    X = S.build_inputs(P, Sid)
+    Y = Y.reshape(-1, 1, 64, 64)
     Xtr, Ytr, Xva, Yva = X[:160], Y[:160], X[160:], Y[160:]
     net = S.SurrogateNet(seed=0)
     hist = S.train(net, Xtr, Ytr, epochs=40, batch=32, lr=3e-3)

@@ -124,7 +124,8 @@ def test_torch_op_scene_4321_matches_oracle(gpu):
     """The drop-in op torch.ops.pathtrace.torch_launch_pathtrace on the
     reference's own 4321 scene and go.py's __main__ geometry (6x6 AP at
     (-3, 3.4, -0.51) facing -x, 84 x 75 Rx at z = -0.51, 4 reflections,
-    100 x 100 launch), against the oracle on the same inputs."""
+    100 x 100 launch with the reference's ray set), against the oracle on the
+    same inputs and rays."""
     import torch
 
     from automs_amd import channel, geometry
@@ -148,7 +149,9 @@ def test_torch_op_scene_4321_matches_oracle(gpu):
     Hg = Hg[..., 0] + 1j * Hg[..., 1]
     w = Workload("4321", sc, ArraySpec(src.numpy(), np.array([-1, 0, 0], np.float32), 2, 0.5e-5),
                  ArraySpec(dst.numpy(), np.zeros(3, np.float32), 0, 0.06), wl, 4, 100 * 100, False)
-    Ho, ost, _ = O.trace_workload(w)
+    osc = O.OracleScene.from_scene(sc)
+    keys, _ = O.discover_ref(osc, w.tx.pos, w.rx.pos, w.rx.radius, 4, 5, 100, 100, 1, 100)
+    Ho, ost, _ = O.trace_workload(w, seed=keys, sc=osc)
     rel = np.linalg.norm(Hg - Ho) / np.linalg.norm(Ho)
     co = 20 * np.log10(np.abs(Ho.sum(0)) + 1e-300)
     cg = 20 * np.log10(np.abs(Hg.sum(0)) + 1e-300)
@@ -159,3 +162,44 @@ def test_torch_op_scene_4321_matches_oracle(gpu):
     assert ost["paths"] > 10
     assert rel < 1e-3
     assert (ddb > 0.05).sum() <= max(5, int(0.01 * m.sum()))
+
+
+def test_reference_discovery_mode(small_room, oracle_scene, trace_scene):
+    """The reference's own ray set (TEA/LCG per launch index, per-tile
+    min-hit gating, reserve cap): record sets within 1% of the oracle's, and
+    end-to-end coverage from those records within 0.05 dB on 99% of tiles."""
+    import torch
+
+    from automs_amd.tracer import trace
+    from oracle import oracle as O
+    w = small_room
+    launch = (100, 100, 2, 100)
+    _flush(f"case: reference-mode discovery room_boxes launch={launch}")
+    ok, ost = O.discover_ref(oracle_scene, w.tx.pos, w.rx.pos, w.rx.radius, w.max_reflection, w.gridsize,
+                             *launch)
+    res, st, gk = trace(trace_scene, w.tx, w.rx, w.wave_len, w.max_reflection, 0, False, w.gridsize,
+                        keys_cap=len(ok) * 2 + 1024, reference_launch=launch)
+    torch.cuda.synchronize()
+    a, b = _keyset(ok), _keyset(gk)
+    diff = a ^ b
+    _flush(f"oracle unique={len(a)} raw={ost['raw']} gpu unique={len(b)} raw={st['records_raw']} "
+           f"symmetric diff={len(diff)}")
+    assert st["rays"] == ost["rays"] == 100 * 100 * 2
+    assert len(diff) <= max(2, 0.01 * len(a))
+    Ho, _, _ = O.trace_workload(w, seed=ok, sc=oracle_scene)
+    Hg = res.double().cpu().numpy()
+    Hg = Hg[..., 0] + 1j * Hg[..., 1]
+    co = 20 * np.log10(np.abs(Ho.sum(0)) + 1e-300)
+    cg = 20 * np.log10(np.abs(Hg.sum(0)) + 1e-300)
+    m = co > -120
+    ddb = np.abs(co - cg)[m]
+    _flush(f"coverage dB>0.05: {(ddb > 0.05).sum()} of {m.sum()}")
+    assert (ddb > 0.05).sum() <= max(5, int(0.01 * m.sum()))
+
+
+def test_reference_mode_rejects_bad_launch(trace_scene, small_room):
+    from automs_amd.tracer import trace
+    w = small_room
+    with pytest.raises(RuntimeError):
+        trace(trace_scene, w.tx, w.rx, w.wave_len, w.max_reflection, 0, False, w.gridsize,
+              reference_launch=(0, 10, 1, 10))
