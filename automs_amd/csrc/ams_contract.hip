@@ -185,6 +185,128 @@ __global__ void __launch_bounds__(256) k_cgemm(const u32x4 *__restrict__ A, cons
   }
 }
 
+// Workgroup-tiled variant for Ra >= 128.  Four waves (2 x 2) own a 128 x 64
+// output tile, each wave 64 x 32 (4 x 2 MFMA tiles).  Per 32-deep step the
+// workgroup stages its 8 A blocks and 4 B blocks (48 KB) in LDS once, so a
+// fragment crosses L2 -> CU once per workgroup instead of once per wave (the
+// per-wave kernel needs ~62 B/clk/CU at full MFMA rate, the CU's load limit).
+// The next step's blocks are loaded into registers while the current step
+// computes.  Workgroups are numbered so that the row groups of one column
+// group run on the same XCD (blockIdx.x mod 8) and share B through its L2.
+constexpr int kWgA = 8, kWgB = 4, kWgBlocks = kWgA + kWgB;
+
+template <bool CONJB>
+__global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, const u32x4 *__restrict__ B, int Ra,
+                                                  int Rb, int n_kb, int row_groups, int col_groups,
+                                                  float *__restrict__ C, long long ldc, float *__restrict__ part) {
+  __shared__ u32x4 lds[kWgBlocks * 256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int xcd = (int)(blockIdx.x & 7), s = (int)(blockIdx.x >> 3);
+  const int rg = s % row_groups;
+  const int cg = (s / row_groups) * 8 + xcd;
+  if (cg >= col_groups) return;  // uniform over the workgroup, before any barrier
+  const int ra_blocks = (Ra + 15) >> 4, rb_blocks = (Rb + 15) >> 4;
+  const int splits = gridDim.z;
+  const int kb0 = (int)((long long)n_kb * blockIdx.z / splits);
+  const int kb1 = (int)((long long)n_kb * (blockIdx.z + 1) / splits);
+  long long base[kWgBlocks];  // first fragment (kb = 0) of each staged block; clamped blocks are never stored
+#pragma unroll
+  for (int q = 0; q < kWgA; q++) {
+    int rb = rg * kWgA + q;
+    base[q] = (long long)(rb < ra_blocks ? rb : ra_blocks - 1) * n_kb * 256;
+  }
+#pragma unroll
+  for (int q = 0; q < kWgB; q++) {
+    int cb = cg * kWgB + q;
+    base[kWgA + q] = (long long)(cb < rb_blocks ? cb : rb_blocks - 1) * n_kb * 256;
+  }
+  u32x4 pf[kWgBlocks];
+  if (kb0 < kb1) {
+#pragma unroll
+    for (int q = 0; q < kWgBlocks; q++) pf[q] = (q < kWgA ? A : B)[base[q] + (long long)kb0 * 256 + tid];
+  }
+  f32x4 acc_re[4][2], acc_im[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc_re[i][j] = (f32x4){0, 0, 0, 0};
+      acc_im[i][j] = (f32x4){0, 0, 0, 0};
+    }
+  const int wr = w & 1, wc = w >> 1;
+  for (int kb = kb0; kb < kb1; kb++) {
+    __syncthreads();  // every wave is done reading the previous step
+#pragma unroll
+    for (int q = 0; q < kWgBlocks; q++) lds[q * 256 + tid] = pf[q];
+    if (kb + 1 < kb1) {
+#pragma unroll
+      for (int q = 0; q < kWgBlocks; q++) pf[q] = (q < kWgA ? A : B)[base[q] + (long long)(kb + 1) * 256 + tid];
+    }
+    __syncthreads();
+    u32x4 b[2][4];
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) b[j][q] = lds[(kWgA + wc * 2 + j) * 256 + q * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      u32x4 a[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[q] = lds[(wr * 4 + i) * 256 + q * 64 + lane];
+      bf16x8 arh = as_bf(a[0]), arl = as_bf(a[1]), aih = as_bf(a[2]), ail = as_bf(a[3]);
+      bf16x8 narh = neg_bf(a[0]), narl = neg_bf(a[1]), naih = neg_bf(a[2]), nail = neg_bf(a[3]);
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        bf16x8 brh = as_bf(b[j][0]), brl = as_bf(b[j][1]), bih = as_bf(b[j][2]), bil = as_bf(b[j][3]);
+        f32x4 re = acc_re[i][j], im = acc_im[i][j];
+        re = MFMA(arh, brh, re);
+        re = MFMA(arh, brl, re);
+        re = MFMA(arl, brh, re);
+        if (CONJB) {
+          re = MFMA(aih, bih, re);
+          re = MFMA(aih, bil, re);
+          re = MFMA(ail, bih, re);
+        } else {
+          re = MFMA(naih, bih, re);
+          re = MFMA(naih, bil, re);
+          re = MFMA(nail, bih, re);
+        }
+        im = MFMA(aih, brh, im);
+        im = MFMA(aih, brl, im);
+        im = MFMA(ail, brh, im);
+        if (CONJB) {
+          im = MFMA(narh, bih, im);
+          im = MFMA(narh, bil, im);
+          im = MFMA(narl, bih, im);
+        } else {
+          im = MFMA(arh, bih, im);
+          im = MFMA(arh, bil, im);
+          im = MFMA(arl, bih, im);
+        }
+        acc_re[i][j] = re;
+        acc_im[i][j] = im;
+      }
+    }
+  }
+  float *out = splits > 1 ? part + (long long)blockIdx.z * 2 * (long long)Ra * Rb : C;
+  long long ld = splits > 1 ? Rb : ldc;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      int col = (cg * kWgB + wc * 2 + j) * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row = (rg * kWgA + wr * 4 + i) * 16 + 4 * (lane >> 4) + v;
+        if (row < Ra && col < Rb) {
+          float2 val = make_float2(acc_re[i][j][v], acc_im[i][j][v]);
+          *reinterpret_cast<float2 *>(out + 2 * ((long long)row * ld + col)) = val;
+        }
+      }
+    }
+  }
+}
+
 // Ordered sum of split partials: C = sum_z part[z]  (deterministic).
 __global__ void k_reduce_parts(const float *__restrict__ part, int splits, int Ra, int Rb, float *__restrict__ C,
                                long long ldc) {
@@ -255,6 +377,15 @@ void launch_cgemm(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, int 
                                                                part);
 }
 
+template <bool CJ>
+void launch_cgemm_wg(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, int splits, float *C, long long ldc,
+                     float *part, hipStream_t st) {
+  int row_groups = ((Ra + 15) / 16 + kWgA - 1) / kWgA;
+  int col_groups = ((Rb + 15) / 16 + kWgB - 1) / kWgB;
+  unsigned blocks = (unsigned)(8LL * row_groups * ((col_groups + 7) / 8));
+  k_cgemm_wg<CJ><<<dim3(blocks, 1, splits), 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, C, ldc, part);
+}
+
 }  // namespace
 
 extern "C" {
@@ -294,6 +425,8 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
   }
   int n_kb = (Kd + 31) / 32;
   if (splits > n_kb) splits = n_kb;
+  const bool per_wave = (conj_b & 2) != 0;
+  conj_b &= 1;
   hipStream_t st = (hipStream_t)stream;
   const u32x4 *a = (const u32x4 *)A, *b = (const u32x4 *)B;
   float *part = (float *)workspace;
@@ -304,9 +437,12 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
   } else if (ra_blocks == 2) {
     if (conj_b) launch_cgemm<2, 4, true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
     else launch_cgemm<2, 4, false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
-  } else {
+  } else if (ra_blocks < kWgA || per_wave) {
     if (conj_b) launch_cgemm<4, 2, true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
     else launch_cgemm<4, 2, false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+  } else {
+    if (conj_b) launch_cgemm_wg<true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+    else launch_cgemm_wg<false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && splits > 1) {

@@ -369,14 +369,31 @@ def anneal(score_fn, steps: int, seed: int, t0: float = 1.0, t1: float = 0.01, r
     return best, best_val, accepted
 
 
+def calibrate_temperature(batch_score, seed: int, pairs: int = 32, chi0: float = 0.5, ratio: float = 100.0):
+    """Start temperature by Kirkpatrick's rule: a typical downhill move (the
+    median |score change| over random (placement, neighbour) pairs, scored by
+    the search's own score function) is accepted with probability chi0 at
+    the start; the schedule ends at t0 / ratio."""
+    rng = SplitMix64(seed)
+    ps = [random_placement(rng) for _ in range(pairs)]
+    qs = [neighbour(p, rng) for p in ps]
+    d = float(np.median(np.abs(np.asarray(batch_score(ps)) - np.asarray(batch_score(qs)))))
+    t0 = max(d, 1e-4) / math.log(1.0 / chi0)
+    return t0, t0 / ratio
+
+
 def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 2000, search_scene: int = 1000,
                    epochs: int = 20, fib: int = 1 << 18, seed: int = 0, log=print, datasets=None,
-                   t0: float = 0.3, t1: float = 0.003) -> dict:
+                   t0: Optional[float] = None, t1: Optional[float] = None) -> dict:
     """Config-4 experiment: traced dataset -> trained surrogate -> placement
     search three ways on an unseen scene: plain annealing (every candidate
     traced), surrogate-guided annealing (accepted candidates traced), and
-    plain annealing cut to the surrogate search's number of traces."""
-    out: Dict = {"sa_steps": sa_steps, "search_scene": search_scene, "t0_db": t0, "t1_db": t1}
+    plain annealing cut to the surrogate search's number of traces.
+
+    With t0 / t1 None each search calibrates its own schedule with
+    calibrate_temperature on its own score function (plain: traced pairs,
+    counted in its time and traces; surrogate: predicted pairs)."""
+    out: Dict = {"sa_steps": sa_steps, "search_scene": search_scene}
     t_start = time.perf_counter()
     if datasets:
         P, S_, Y = load_dataset(datasets)
@@ -420,15 +437,28 @@ def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 20
         torch.cuda.synchronize()
         return r, time.perf_counter() - t, pt.n_traces
 
-    (b1, v1, a1), s1, n1 = timed(lambda: anneal(traced, sa_steps, seed=seed + 1, t0=t0, t1=t1))
-    out["plain"] = {"best_gain_db": v1, "seconds": s1, "traces": n1, "accepted": a1, "best": b1.__dict__}
-    log(f"plain search: {v1:.3f} dB in {s1:.1f} s ({n1} traces)")
-    (b2, v2, a2), s2, n2 = timed(lambda: anneal(lambda p: float(predict_score(net, occ, [p])[0]), sa_steps,
-                                                seed=seed + 1, t0=t0, t1=t1, rescore=traced))
-    out["surrogate"] = {"best_gain_db": v2, "seconds": s2, "traces": n2, "accepted": a2, "best": b2.__dict__}
-    log(f"surrogate search: {v2:.3f} dB in {s2:.1f} s ({n2} traces)")
+    def predicted_batch(ps):
+        return predict_score(net, occ, ps).cpu().numpy()
+
+    def search(score_fn, batch_score, steps, rescore=None):
+        if t0 is None:
+            ta, tb = calibrate_temperature(batch_score, seed + 7)
+        else:
+            ta, tb = t0, t1
+        best, val, acc = anneal(score_fn, steps, seed=seed + 1, t0=ta, t1=tb, rescore=rescore)
+        return best, val, acc, ta, tb
+
+    (b1, v1, a1, ta1, tb1), s1, n1 = timed(lambda: search(traced, lambda ps: [traced(p) for p in ps], sa_steps))
+    out["plain"] = {"best_gain_db": v1, "seconds": s1, "traces": n1, "accepted": a1, "t0_db": ta1, "t1_db": tb1,
+                    "best": b1.__dict__}
+    log(f"plain search: {v1:.3f} dB in {s1:.1f} s ({n1} traces, {a1} accepted, t0 {ta1:.3f} dB)")
+    (b2, v2, a2, ta2, tb2), s2, n2 = timed(lambda: search(lambda p: float(predict_score(net, occ, [p])[0]),
+                                                          predicted_batch, sa_steps, rescore=traced))
+    out["surrogate"] = {"best_gain_db": v2, "seconds": s2, "traces": n2, "accepted": a2, "t0_db": ta2,
+                        "t1_db": tb2, "best": b2.__dict__}
+    log(f"surrogate search: {v2:.3f} dB in {s2:.1f} s ({n2} traces, {a2} accepted, t0 {ta2:.3f} dB)")
     budget = max(1, n2 - 1)
-    (b3, v3, a3), s3, n3 = timed(lambda: anneal(traced, budget, seed=seed + 1, t0=t0, t1=t1))
+    (b3, v3, a3), s3, n3 = timed(lambda: anneal(traced, budget, seed=seed + 1, t0=ta1, t1=tb1))
     out["plain_same_budget"] = {"best_gain_db": v3, "seconds": s3, "traces": n3, "steps": budget}
     log(f"plain search with the same trace budget: {v3:.3f} dB in {s3:.1f} s")
     out["gain_gap_db"] = v1 - v2

@@ -53,7 +53,7 @@ def workload_traces(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int) -> dict:
+def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int, office: bool = False) -> dict:
     """FP64 oracle (OpenMP) on the same scene with fewer rays per trace."""
     from automs_amd import scenes
     from oracle import oracle as O
@@ -71,8 +71,19 @@ def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int) -> dict:
     what = f"{name}: every trace with {sample_fib} Fibonacci rays per AP element"
     if name != "room_boxes":
         what += f", a 16 x 16 metasurface and {sample_ms_fib} rays in total from it"
-    return {"value": q / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"{what} ({q} closest-hit queries in {dt:.1f} s; FP64 oracle, OpenMP)"}
+    out = {"value": q / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+           "sample": f"{what} ({q} closest-hit queries in {dt:.1f} s; FP64 oracle, OpenMP)"}
+    if office:
+        w = scenes.office(fib_n=1 << 10, ms_fib_n=1 << 12)["H_d"]
+        t0 = time.perf_counter()
+        _, st, _ = O.trace_workload(w)
+        dto = time.perf_counter() - t0
+        qo = st["fwd_queries"] + st["bwd_queries"]
+        out["office_H_d"] = {"mrays_per_s": qo / dto / 1e6, "seconds": dto, "queries": qo,
+                             "rx_channels_per_s": w.tx.n * w.rx.n / dto,
+                             "sample": f"office H_d ({w.scene.n_tri} triangles, 64 x 65,536) with {w.fib_n} "
+                                       "discovery rays; the neighbour expansion then traces every reachable tile"}
+    return out
 
 
 def contraction_office(iters: int = 20) -> dict:
@@ -87,11 +98,16 @@ def contraction_office(iters: int = 20) -> dict:
     from automs_amd.tracer import TraceScene, trace
     tr = scenes.office(fib_n=1 << 20, ms_fib_n=1 << 20)
     ch = {}
+    traced = {}
     for key, name in (("src_to_dst", "H_d"), ("src_to_ms{0}", "G0"), ("ms{0}_to_dst", "F0")):
         w = tr[name]
-        res, _, _ = trace(TraceScene(w.scene), w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
-                          w.gridsize)
+        res, st, _ = trace(TraceScene(w.scene), w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
+                           w.gridsize)
         ch[key] = torch.complex(res[..., 0], res[..., 1])
+        q = st["fwd_queries"] + st["bwd_queries"]
+        traced[name] = {"n_tx": w.tx.n, "n_rx": w.rx.n, "rays": st["rays"], "queries": q, "paths": st["paths"],
+                        "ms": st["t_total_ms"], "mrays_per_s": q / (st["t_total_ms"] * 1e-3) / 1e6,
+                        "rx_channels_per_s": w.tx.n * w.rx.n / (st["t_total_ms"] * 1e-3)}
     model = batched_model(ch, Metasurface(None, (48, 48), None, None, [], max_phase=360.0))
     op = model.op
     with torch.no_grad():
@@ -124,14 +140,40 @@ def contraction_office(iters: int = 20) -> dict:
     torch.cuda.synchronize()
     t_step = e0.elapsed_time(e1) / iters * 1e-3
     tr_meas = MEASURED_TRAFFIC[("office_batched", "k_cgemm")]
+    # parity against complex128 on a column sample, and the host baseline:
+    # the same contraction in complex64 with torch on all host cores
+    with torch.no_grad():
+        Y = cgemm_frag(fa, op.frag_fwd, M, N, Kc, conj_b=False)
+        cols = torch.arange(0, N, 32, device=Y.device)
+        Fcat = torch.cat([ch["ms{0}_to_dst"], ch["src_to_dst"]], 0)
+        Fs = Fcat[:, cols].cpu()
+        A_h = A.cpu()
+        ref = A_h.to(torch.complex128) @ Fs.to(torch.complex128)
+        err = (Y[:, cols].cpu().to(torch.complex128) - ref).abs().pow(2).sum(0).sqrt() / \
+            ref.abs().pow(2).sum(0).sqrt().clamp_min(1e-300)
+        Fcpu = Fcat[:, : min(N, 8192)].cpu().contiguous()
+        A_h @ Fcpu
+        t0 = time.perf_counter()
+        A_h @ Fcpu
+        t_cpu = (time.perf_counter() - t0) * N / Fcpu.shape[1]
+    parity = {"rx_sampled": int(cols.numel()), "per_rx_rel_err_median": float(err.median()),
+              "per_rx_rel_err_max": float(err.max()),
+              "fro_rel_err": float((Y[:, cols].cpu().to(torch.complex128) - ref).norm() / ref.norm()),
+              "oracle": "complex128 product of the same operands on the host"}
+    host = {"seconds_per_contraction": t_cpu, "threads": torch.get_num_threads(),
+            "kind": "torch complex64 matmul on the host (FP32), timed on 8192 Rx columns and scaled to N",
+            "gpu_speedup": t_cpu / t_fwd}
     return {"shape": {"M": M, "K": Kc, "N": N}, "fwd_gemm_ms": t_fwd * 1e3,
+            "rx_channels_per_s": M * N / t_fwd, "parity_vs_fp64": parity, "host_baseline": host,
             "roofline": {"bound": "mfma", "achieved": 3 * useful / t_fwd / 1e12, "peak": MFMA_BF16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": 3 * useful / t_fwd / 2.5e15, "traffic": tr_meas["bytes"],
                          "mfma_busy_pmc": tr_meas["mfma_util"]},
             "useful_tflops": useful / t_fwd / 1e12, "issued_tflops": 3 * useful / t_fwd / 1e12,
             "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": 3 * useful / t_fwd / 2.5e15,
             "f_operand_gb_per_s": 16.0 * Kc * N / t_fwd / 1e9,
-            "optimizer_step_ms": t_step * 1e3, "maps": 4, "beams": 64}
+            "optimizer_step_ms": t_step * 1e3, "maps": 4, "beams": 64,
+            "office_traces": {"scene_tris": tr["H_d"].scene.n_tri, "max_reflection": tr["H_d"].max_reflection,
+                              "traces": traced}}
 
 
 def main():
@@ -225,7 +267,8 @@ def main():
         "rx_channels_per_s": n_channels * args.steps / dt,
         "primary_rays_per_s": tot["rays"] / dt,
         "per_step": {k: tot[k] / args.steps for k in ("rays", "paths", "tasks", "rounds", "chains", "successes",
-                                                      "fwd_queries", "bwd_queries", "t_discover_ms",
+                                                      "fwd_queries", "bwd_queries", "fwd_nodes", "fwd_tris",
+                                                      "fwd_spheres", "bwd_nodes", "bwd_tris", "t_discover_ms",
                                                       "t_evaluate_ms", "t_total_ms")},
         "roofline": {
             "kernel": dom,
@@ -261,7 +304,8 @@ def main():
         print(f"contraction: {r}", file=sys.stderr, flush=True)
         line["contraction_office_batched"] = r
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_fib, args.cpu_sample_ms_fib)
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_fib, args.cpu_sample_ms_fib,
+                                            office=not args.no_contract)
     print(json.dumps(line), flush=True)
 
 

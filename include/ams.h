@@ -91,9 +91,11 @@ int ams_trace_channel(ams_scene *scene, const ams_array *tx, const ams_array *rx
 int64_t ams_frag_bytes(int32_t R, int32_t Kd);
 /* pack X with element (r, k) at X + 2*(r*sr + k*sk) floats */
 int ams_frag_pack(const float *X, int32_t R, int32_t Kd, int64_t sr, int64_t sk, void *frag, void *stream);
-/* C[Ra][Rb] = A . B^T (conj_b: A . conj(B)^T), split-bf16 MFMA, FP32
+/* C[Ra][Rb] = A . B^T (conj_b bit 0: A . conj(B)^T), split-bf16 MFMA, FP32
  * accumulation; splits > 1 splits the reduction and needs
- * ams_cgemm_workspace_bytes(Ra, Rb, splits) of workspace. */
+ * ams_cgemm_workspace_bytes(Ra, Rb, splits) of workspace.  Ra >= 128 uses
+ * the LDS-staged workgroup kernel unless conj_b bit 1 asks for the per-wave
+ * kernel (kept for comparison; identical results up to FP32 rounding). */
 int64_t ams_cgemm_workspace_bytes(int32_t Ra, int32_t Rb, int32_t splits);
 int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b, float *C,
                    int64_t ldc, int32_t splits, void *workspace, void *stream);

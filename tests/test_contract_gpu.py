@@ -25,7 +25,8 @@ def _crand(*shape, scale=1.0, seed=0):
             + 1j * torch.randn(*shape, generator=g, dtype=torch.float64)) * scale
 
 
-@pytest.mark.parametrize("Ra,Rb,Kd", [(16, 64, 32), (30, 4096, 1040), (37, 333, 70), (256, 512, 2304)])
+@pytest.mark.parametrize("Ra,Rb,Kd", [(16, 64, 32), (30, 4096, 1040), (37, 333, 70), (256, 512, 2304),
+                                      (130, 333, 70), (128, 64, 32)])
 @pytest.mark.parametrize("conj_b", [False, True])
 def test_cgemm_frag_matches_complex128(gpu, Ra, Rb, Kd, conj_b):
     """Split-bf16 MFMA GEMM: relative Frobenius error < 3e-5 (hi/lo split
@@ -42,6 +43,24 @@ def test_cgemm_frag_matches_complex128(gpu, Ra, Rb, Kd, conj_b):
         rel = (out - ref).norm() / ref.norm()
         _flush(f"  splits={splits} rel={rel:.3e}")
         assert rel < 3e-5
+
+
+@pytest.mark.parametrize("Ra,Rb,Kd", [(256, 1000, 700), (130, 77, 300)])
+@pytest.mark.parametrize("conj_b", [False, True])
+def test_cgemm_workgroup_kernel_equals_per_wave(gpu, Ra, Rb, Kd, conj_b):
+    """The LDS-staged workgroup kernel (Ra >= 128) issues the same MFMAs in
+    the same order per output tile as the per-wave kernel: bit-identical."""
+    from automs_amd.phasemap import cgemm_frag, frag_pack
+    _flush(f"case: cgemm wg vs per-wave Ra={Ra} Rb={Rb} Kd={Kd} conj={conj_b}")
+    A = _crand(Ra, Kd, seed=5).to(torch.complex64).to(gpu)
+    B = _crand(Rb, Kd, seed=6).to(torch.complex64).to(gpu)
+    fa = frag_pack(A, Ra, Kd, Kd, 1)
+    fb = frag_pack(B, Rb, Kd, Kd, 1)
+    for splits in (1, 3):
+        x = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits)
+        y = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits, per_wave=True)
+        torch.cuda.synchronize()
+        assert torch.equal(torch.view_as_real(x), torch.view_as_real(y))
 
 
 def test_cgemm_f32_plain(gpu):
