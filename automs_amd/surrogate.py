@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import time
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Tuple
@@ -77,6 +78,16 @@ class SurrogateNet:
         self.m = torch.zeros_like(self.flat)
         self.v = torch.zeros_like(self.flat)
         self.t = 0
+        self.refresh()
+
+    def save(self, path: str) -> None:
+        np.save(path, self.flat.detach().cpu().numpy())
+
+    def load(self, path: str) -> None:
+        w = torch.from_numpy(np.load(path)).to(self.dev)
+        if w.shape != self.flat.shape:
+            raise ValueError(f"{path}: {tuple(w.shape)} parameters, the network has {tuple(self.flat.shape)}")
+        self.flat.copy_(w)
         self.refresh()
 
     def param(self, i: int) -> torch.Tensor:
@@ -343,7 +354,8 @@ def predict_score(net: SurrogateNet, occ: np.ndarray, ps: List[Placement]) -> to
 
 # ---------------------------------------------------------------- annealing --
 
-def anneal(score_fn, steps: int, seed: int, t0: float = 1.0, t1: float = 0.01, rescore=None, start=None):
+def anneal(score_fn, steps: int, seed: int, t0: float = 1.0, t1: float = 0.01, rescore=None, start=None,
+           stats: Optional[dict] = None):
     """Simulated annealing maximising a placement score.
 
     Plain search (rescore None): score_fn is the traced score of a candidate.
@@ -356,11 +368,15 @@ def anneal(score_fn, steps: int, seed: int, t0: float = 1.0, t1: float = 0.01, r
     best = cur
     best_val = rescore(cur) if rescore else cur_val
     accepted = 0
+    if stats is not None:
+        stats.update(uphill=[0, 0, 0, 0], downhill=[0, 0, 0, 0])
     for k in range(steps):
         T = t0 * (t1 / t0) ** (k / max(1, steps - 1))
         cand = neighbour(cur, rng)
         val = score_fn(cand)
         if val >= cur_val or rng.uniform() < math.exp((val - cur_val) / T):
+            if stats is not None:
+                stats["uphill" if val >= cur_val else "downhill"][min(3, 4 * k // max(1, steps))] += 1
             cur, cur_val = cand, val
             accepted += 1
             traced = rescore(cand) if rescore else val
@@ -384,7 +400,8 @@ def calibrate_temperature(batch_score, seed: int, pairs: int = 32, chi0: float =
 
 def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 2000, search_scene: int = 1000,
                    epochs: int = 20, fib: int = 1 << 18, seed: int = 0, log=print, datasets=None,
-                   t0: Optional[float] = None, t1: Optional[float] = None) -> dict:
+                   t0: Optional[float] = None, t1: Optional[float] = None, chi0: float = 0.5,
+                   t_ratio: float = 100.0, net_path: Optional[str] = None, searches=("plain", "surrogate")) -> dict:
     """Config-4 experiment: traced dataset -> trained surrogate -> placement
     search three ways on an unseen scene: plain annealing (every candidate
     traced), surrogate-guided annealing (accepted candidates traced), and
@@ -410,7 +427,13 @@ def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 20
     Xtr, Ytr, Xva, Yva = X[:-n_val], Y[:-n_val], X[-n_val:], Y[-n_val:]
     net = SurrogateNet(seed=seed)
     tt = time.perf_counter()
-    hist = train(net, Xtr, Ytr, epochs=epochs, seed=seed, log=log)
+    if net_path and os.path.exists(net_path):
+        net.load(net_path)
+        hist = [float("nan")]
+    else:
+        hist = train(net, Xtr, Ytr, epochs=epochs, seed=seed, log=log)
+        if net_path:
+            net.save(net_path)
     torch.cuda.synchronize()
     out["train_seconds"] = time.perf_counter() - tt
     with torch.no_grad():
@@ -442,28 +465,34 @@ def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 20
 
     def search(score_fn, batch_score, steps, rescore=None):
         if t0 is None:
-            ta, tb = calibrate_temperature(batch_score, seed + 7)
+            ta, tb = calibrate_temperature(batch_score, seed + 7, chi0=chi0, ratio=t_ratio)
         else:
             ta, tb = t0, t1
-        best, val, acc = anneal(score_fn, steps, seed=seed + 1, t0=ta, t1=tb, rescore=rescore)
-        return best, val, acc, ta, tb
+        st: dict = {}
+        best, val, acc = anneal(score_fn, steps, seed=seed + 1, t0=ta, t1=tb, rescore=rescore, stats=st)
+        return best, val, acc, ta, tb, st
 
-    (b1, v1, a1, ta1, tb1), s1, n1 = timed(lambda: search(traced, lambda ps: [traced(p) for p in ps], sa_steps))
-    out["plain"] = {"best_gain_db": v1, "seconds": s1, "traces": n1, "accepted": a1, "t0_db": ta1, "t1_db": tb1,
-                    "best": b1.__dict__}
-    log(f"plain search: {v1:.3f} dB in {s1:.1f} s ({n1} traces, {a1} accepted, t0 {ta1:.3f} dB)")
-    (b2, v2, a2, ta2, tb2), s2, n2 = timed(lambda: search(lambda p: float(predict_score(net, occ, [p])[0]),
-                                                          predicted_batch, sa_steps, rescore=traced))
-    out["surrogate"] = {"best_gain_db": v2, "seconds": s2, "traces": n2, "accepted": a2, "t0_db": ta2,
-                        "t1_db": tb2, "best": b2.__dict__}
-    log(f"surrogate search: {v2:.3f} dB in {s2:.1f} s ({n2} traces, {a2} accepted, t0 {ta2:.3f} dB)")
-    budget = max(1, n2 - 1)
-    (b3, v3, a3), s3, n3 = timed(lambda: anneal(traced, budget, seed=seed + 1, t0=ta1, t1=tb1))
-    out["plain_same_budget"] = {"best_gain_db": v3, "seconds": s3, "traces": n3, "steps": budget}
-    log(f"plain search with the same trace budget: {v3:.3f} dB in {s3:.1f} s")
-    out["gain_gap_db"] = v1 - v2
-    out["time_ratio"] = s2 / s1
-    out["surrogate_gain_over_same_budget_db"] = v2 - v3
+    out["chi0"], out["t_ratio"] = chi0, t_ratio
+    if "plain" in searches:
+        (b1, v1, a1, ta1, tb1, st1), s1, n1 = timed(lambda: search(traced, lambda ps: [traced(p) for p in ps],
+                                                                   sa_steps))
+        out["plain"] = {"best_gain_db": v1, "seconds": s1, "traces": n1, "accepted": a1, "t0_db": ta1,
+                        "t1_db": tb1, "accepted_by_quarter": st1, "best": b1.__dict__}
+        log(f"plain search: {v1:.3f} dB in {s1:.1f} s ({n1} traces, {a1} accepted, t0 {ta1:.3f} dB, {st1})")
+    if "surrogate" in searches:
+        (b2, v2, a2, ta2, tb2, st2), s2, n2 = timed(lambda: search(lambda p: float(predict_score(net, occ, [p])[0]),
+                                                                   predicted_batch, sa_steps, rescore=traced))
+        out["surrogate"] = {"best_gain_db": v2, "seconds": s2, "traces": n2, "accepted": a2, "t0_db": ta2,
+                            "t1_db": tb2, "accepted_by_quarter": st2, "best": b2.__dict__}
+        log(f"surrogate search: {v2:.3f} dB in {s2:.1f} s ({n2} traces, {a2} accepted, t0 {ta2:.3f} dB, {st2})")
+    if "plain" in searches and "surrogate" in searches:
+        budget = max(1, n2 - 1)
+        (b3, v3, a3), s3, n3 = timed(lambda: anneal(traced, budget, seed=seed + 1, t0=ta1, t1=tb1))
+        out["plain_same_budget"] = {"best_gain_db": v3, "seconds": s3, "traces": n3, "steps": budget}
+        log(f"plain search with the same trace budget: {v3:.3f} dB in {s3:.1f} s")
+        out["gain_gap_db"] = v1 - v2
+        out["time_ratio"] = s2 / s1
+        out["surrogate_gain_over_same_budget_db"] = v2 - v3
     return out
 
 
@@ -482,6 +511,10 @@ if __name__ == "__main__":
     ap.add_argument("--make-dataset", default=None, help="only trace a dataset and save it (npz)")
     ap.add_argument("--first-scene", type=int, default=0)
     ap.add_argument("--datasets", default=None, help="comma-separated npz files to train on")
+    ap.add_argument("--net", default=None, help="network weights (.npy): loaded if present, else saved after training")
+    ap.add_argument("--chi0", type=float, default=0.5, help="start acceptance of a median downhill move")
+    ap.add_argument("--t-ratio", type=float, default=100.0, help="start / end temperature")
+    ap.add_argument("--searches", default="plain,surrogate")
     a = ap.parse_args()
     logf = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if a.make_dataset:
@@ -492,7 +525,8 @@ if __name__ == "__main__":
         print(json.dumps({"samples": len(P), "seconds": time.perf_counter() - t}), flush=True)
         sys.exit(0)
     res = run_experiment(a.scenes, a.per_scene, a.sa_steps, epochs=a.epochs, fib=a.fib, log=logf,
-                         datasets=a.datasets.split(",") if a.datasets else None)
+                         datasets=a.datasets.split(",") if a.datasets else None, chi0=a.chi0, t_ratio=a.t_ratio,
+                         net_path=a.net, searches=tuple(a.searches.split(",")))
     txt = json.dumps(res)
     print(txt, flush=True)
     if a.out:

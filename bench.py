@@ -189,7 +189,7 @@ def main():
                     help="short run for rocprof counter passes: 1 traced step, 5 contraction GEMMs and a "
                          "1 GiB device copy that calibrates FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-fib", type=int, default=1 << 16)
+    ap.add_argument("--cpu-sample-fib", type=int, default=1 << 18)
     ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
     args = ap.parse_args()
     if args.gpus != 1:

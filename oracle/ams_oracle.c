@@ -1076,23 +1076,58 @@ int or_trace_channel(void *scene, const float *src, int n0_src, int n1_src, cons
  * stratified direction of rand_dir are computed in f32 as the reference does,
  * then traced in FP64; the 32-slot branch tree is walked in slot order with
  * per-tile min_hc gating and the reserve cap. */
-static unsigned or_tea4(unsigned v0, unsigned v1) {
+/* TEA (Wheeler & Needham 1994) with `rounds` rounds and key k; random.h:31-45
+ * seeds with 4 rounds and the key below. */
+void or_tea(unsigned v0, unsigned v1, const unsigned *k, int rounds, unsigned *out) {
   unsigned s0 = 0;
-  for (int n = 0; n < 4; n++) {
+  for (int n = 0; n < rounds; n++) {
     s0 += 0x9e3779b9u;
-    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
-    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    v0 += ((v1 << 4) + k[0]) ^ (v1 + s0) ^ ((v1 >> 5) + k[1]);
+    v1 += ((v0 << 4) + k[2]) ^ (v0 + s0) ^ ((v0 >> 5) + k[3]);
   }
-  return v0;
+  out[0] = v0;
+  out[1] = v1;
 }
+static const unsigned or_tea_key[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+static unsigned or_tea4(unsigned v0, unsigned v1) {
+  unsigned o[2];
+  or_tea(v0, v1, or_tea_key, 4, o);
+  return o[0];
+}
+/* Numerical Recipes LCG and its [0, 1) float, random.h:47-67 */
+unsigned or_lcg(unsigned st) { return 1664525u * st + 1013904223u; }
 static float or_rnd(unsigned *st) {
-  *st = 1664525u * *st + 1013904223u;
+  *st = or_lcg(*st);
   return (float)(*st & 0x00FFFFFFu) / (float)0x01000000u;
+}
+/* the sample-th ray of launch index t = ix * launch_y + iy: Tx element and
+ * rand_dir's stratified direction (optixPathTracer.cu:56-68,143,151) */
+static int or_ref_sample(unsigned *seed, int ix, int iy, int launch_x, int launch_y, int n_src, float *d) {
+  int e = (int)(or_rnd(seed) * (float)n_src);
+  if (e >= n_src) e = n_src - 1;
+  float r1 = (or_rnd(seed) + (float)ix) / (float)launch_x;
+  float r2 = (or_rnd(seed) + (float)iy) / (float)launch_y;
+  float th = 2.0f * (float)M_PI * r1, rr = sqrtf(r2);
+  float u = rr * cosf(th), v = rr * sinf(th), q = sqrtf(fmaxf(0.0f, 1.0f - rr * rr));
+  d[0] = 2.0f * u * q;
+  d[1] = 2.0f * v * q;
+  d[2] = 1.0f - 2.0f * rr * rr;
+  return e;
+}
+/* every ray of a launch: elem[t * samples + s], dir[3 * (t * samples + s)] */
+void or_ref_rays(int launch_x, int launch_y, int samples, int n_src, int *elem, float *dir) {
+  for (long t = 0; t < (long)launch_x * launch_y; t++) {
+    unsigned seed = or_tea4((unsigned)t, 0u);
+    for (int smp = 0; smp < samples; smp++) {
+      long j = t * samples + smp;
+      elem[j] = or_ref_sample(&seed, (int)(t / launch_y), (int)(t % launch_y), launch_x, launch_y, n_src, dir + 3 * j);
+    }
+  }
 }
 
 long or_discover_ref(void *scene, const float *src, int n_src, const float *dst, int n0_dst, int n1_dst,
                      double dst_r, int max_reflection, int gridsize, int launch_x, int launch_y, int samples,
-                     int reserve, uint64_t *out, long cap, long *stats) {
+                     int reserve, int gating, uint64_t *out, long cap, long *stats) {
   or_scene *s = (or_scene *)scene;
   or_spheres *rx = spheres_create(n0_dst * n1_dst, dst, dst_r);
   int n1g = (n1_dst - 1) / gridsize + 1, n0g = (n0_dst - 1) / gridsize + 1, n_grid = n0g * n1g;
@@ -1120,15 +1155,11 @@ long or_discover_ref(void *scene, const float *src, int n_src, const float *dst,
       memset(min_hc, 0, sizeof(int) * n_grid);
       int res_cnt = 0;
       for (int smp = 0; smp < samples; smp++) {
-        int e = (int)(or_rnd(&seed) * (float)n_src);
-        if (e >= n_src) e = n_src - 1;
-        float r1 = (or_rnd(&seed) + (float)ix) / (float)launch_x;
-        float r2 = (or_rnd(&seed) + (float)iy) / (float)launch_y;
-        float th = 2.0f * (float)M_PI * r1, rr = sqrtf(r2);
-        float u = rr * cosf(th), v = rr * sinf(th), q = sqrtf(fmaxf(0.0f, 1.0f - rr * rr));
+        float dd[3];
+        int e = or_ref_sample(&seed, ix, iy, launch_x, launch_y, n_src, dd);
         for (int k = 0; k < (1 << OR_MAX_HIT); k++) slot[k].ttl = 0;
         slot[0].o = mk(src[3 * e], src[3 * e + 1], src[3 * e + 2]);
-        slot[0].d = nrm(mk(2.0f * u * q, 2.0f * v * q, 1.0f - 2.0f * rr * rr));
+        slot[0].d = nrm(mk(dd[0], dd[1], dd[2]));
         slot[0].ttl = K + 1;
         slot[0].b = 0;
         slot[0].nref = 0;
@@ -1141,10 +1172,10 @@ long or_discover_ref(void *scene, const float *src, int n_src, const float *dst,
             sph_q sq = closest_sphere(rx, cur.o, cur.d, OR_TMIN, tq.t);
             nq++;
             if (sq.idx >= 0 && sq.t < tq.t) {
-              if (res_cnt < reserve) {
+              if (!gating || res_cnt < reserve) {
                 int grid = (int)tile_of(sq.idx, n1_dst, gridsize);
                 int h = K + 1 - cur.ttl;
-                if (min_hc[grid] <= h) {
+                if (!gating || min_hc[grid] <= h) {
                   min_hc[grid] = h + 1;
                   uint64_t hi, lo;
                   key_pack(h, cur.b, cur.pl, (unsigned)grid, &hi, &lo);

@@ -59,7 +59,13 @@ def lib():
                                        C.c_int, u64p, C.c_long, dp, lp, u64p, C.c_long, lp]
         L.or_trace_channel.restype = C.c_int
         L.or_discover_ref.argtypes = [vp, fp, C.c_int, fp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
-                                      C.c_int, C.c_int, C.c_int, u64p, C.c_long, lp]
+                                      C.c_int, C.c_int, C.c_int, C.c_int, u64p, C.c_long, lp]
+        L.or_tea.argtypes = [C.c_uint, C.c_uint, C.POINTER(C.c_uint), C.c_int, C.POINTER(C.c_uint)]
+        L.or_tea.restype = None
+        L.or_lcg.argtypes = [C.c_uint]
+        L.or_lcg.restype = C.c_uint
+        L.or_ref_rays.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), fp]
+        L.or_ref_rays.restype = None
         L.or_discover_ref.restype = C.c_long
         _lib = L
     return _lib
@@ -156,9 +162,32 @@ def discover(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, fib_n, 
     return out[:m].copy(), dict(rays=int(st[0]), queries=int(st[1]), unique=int(st[2]), ties=int(st[3]))
 
 
+def tea(v0: int, v1: int, key, rounds: int):
+    k = (C.c_uint * 4)(*key)
+    out = (C.c_uint * 2)()
+    lib().or_tea(v0, v1, k, rounds, out)
+    return int(out[0]), int(out[1])
+
+
+def lcg(state: int) -> int:
+    return int(lib().or_lcg(state))
+
+
+def ref_rays(launch_x: int, launch_y: int, samples: int, n_src: int):
+    """(element [n], direction f32 [n, 3]) of every ray of a reference launch,
+    ordered by launch index t = ix * launch_y + iy, then sample."""
+    n = launch_x * launch_y * samples
+    elem = np.zeros(n, np.int32)
+    d = np.zeros((n, 3), np.float32)
+    lib().or_ref_rays(launch_x, launch_y, samples, n_src, elem.ctypes.data_as(C.POINTER(C.c_int)),
+                      d.ctypes.data_as(C.POINTER(C.c_float)))
+    return elem, d
+
+
 def discover_ref(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, launch_x, launch_y, samples,
-                 reserve, cap=1 << 24):
-    """Reference-compatible discovery (TEA/LCG rays, min_hc gating, reserve)."""
+                 reserve, cap=1 << 24, gating: bool = True):
+    """Reference-compatible discovery (TEA/LCG rays, min_hc gating, reserve);
+    gating False records every Rx hit of the same rays (for subset checks)."""
     src_a, sp = _f(np.asarray(src).reshape(-1, 3))
     dst = np.asarray(dst)
     n0, n1 = dst.shape[0], dst.shape[1]
@@ -166,7 +195,8 @@ def discover_ref(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, lau
     out = np.zeros((cap, 2), np.uint64)
     st = np.zeros(4, np.int64)
     m = lib().or_discover_ref(sc.h, sp, src_a.shape[0], dp, n0, n1, dst_r, max_reflection, gridsize, launch_x,
-                              launch_y, samples, reserve, out.ctypes.data_as(C.POINTER(C.c_uint64)), cap,
+                              launch_y, samples, reserve, 1 if gating else 0,
+                              out.ctypes.data_as(C.POINTER(C.c_uint64)), cap,
                               st.ctypes.data_as(C.POINTER(C.c_long)))
     if m < 0:
         raise RuntimeError("oracle discovery cap exceeded")

@@ -15,7 +15,11 @@ image sign, transposed index, wrong bin rounding) fails it:
   * Fraunhofer split: two small plates give
     G_A G_B lambda/(4 pi d1) * lambda/(4 pi (d2 + d3));
   * BVH closest hit = brute force closest hit on random scenes;
-  * a full oracle trace over a floor reproduces LOS + floor image at every Rx.
+  * a full oracle trace over a floor reproduces LOS + floor image at every Rx;
+  * reference-mode rays: TEA against its published vector, the Numerical
+    Recipes LCG sequence, rand_dir's strata (launch index x -> azimuth cell,
+    y -> z cell), and the min-hit / reserve gating as a subset of the
+    ungated record set with its per-thread bound.
 """
 import math
 
@@ -238,3 +242,62 @@ def test_discovery_is_deterministic_and_sorted():
         assert 0 <= h <= w.max_reflection
         assert b == (1 << h) - 1  # no penetrable material: every interaction reflects
         assert all(p != 0xFFFFF for p in pl[:h]) and all(p == 0xFFFFF for p in pl[h:])
+
+
+def test_tea_published_vector():
+    """TEA, 32 rounds, zero key and plaintext -> 41ea3a0a 94baa940 (the
+    published test vector); the reference seeds with 4 rounds of the same
+    round function (random.h:31-45)."""
+    assert O.tea(0, 0, [0, 0, 0, 0], 32) == (0x41EA3A0A, 0x94BAA940)
+    a = O.tea(0, 0, [0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E], 4)
+    b = O.tea(1, 0, [0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E], 4)
+    assert a != b
+
+
+def test_lcg_numerical_recipes_sequence():
+    """x' = 1664525 x + 1013904223 mod 2^32 from 0 (random.h:47-53)."""
+    x, seq = 0, []
+    for _ in range(5):
+        x = O.lcg(x)
+        seq.append(x)
+    assert seq == [1013904223, 1196435762, 3519870697, 2868466484, 1649599747]
+
+
+def test_reference_rays_are_stratified():
+    """rand_dir (optixPathTracer.cu:56-68): launch index x picks the azimuth
+    cell [2 pi x / lx, 2 pi (x + 1) / lx), y the z cell
+    (1 - 2 (y + 1) / ly, 1 - 2 y / ly]; unit directions; elements in range.
+    lx != ly, so swapping the two indices fails."""
+    lx, ly, spl, n_src = 16, 12, 3, 5
+    elem, d = O.ref_rays(lx, ly, spl, n_src)
+    assert elem.min() >= 0 and elem.max() < n_src and len(np.unique(elem)) == n_src
+    assert np.allclose(np.linalg.norm(d.astype(np.float64), axis=1), 1.0, atol=2e-6)
+    t = np.repeat(np.arange(lx * ly), spl)
+    ix, iy = t // ly, t % ly
+    z = d[:, 2].astype(np.float64)
+    assert np.all(z <= 1 - 2 * iy / ly + 1e-6) and np.all(z >= 1 - 2 * (iy + 1) / ly - 1e-6)
+    phi = np.mod(np.arctan2(d[:, 1], d[:, 0]).astype(np.float64), 2 * math.pi)
+    lo, hi = 2 * math.pi * ix / lx, 2 * math.pi * (ix + 1) / lx
+    ok = ((phi >= lo - 1e-5) & (phi <= hi + 1e-5)) | (np.hypot(d[:, 0], d[:, 1]) < 1e-3)
+    ok |= (ix == lx - 1) & (phi < 1e-5)
+    assert ok.all()
+
+
+def test_reference_discovery_gating():
+    """min_hc gating and the reserve cap only remove records: the gated
+    unique set is a subset of the ungated one from the same rays; a zero
+    reserve records nothing; raw records stay within launch x reserve."""
+    from automs_amd import scenes
+    w = scenes.room_boxes(fib_n=1 << 10)
+    sc = O.OracleScene.from_scene(w.scene)
+    args = (sc, w.tx.pos, w.rx.pos, w.rx.radius, w.max_reflection, w.gridsize, 30, 20, 2)
+    g, gs = O.discover_ref(*args, 100)
+    u, us = O.discover_ref(*args, 100, gating=False)
+    z, zs = O.discover_ref(*args, 0)
+    r1, r1s = O.discover_ref(*args, 1)
+    sg = set(map(tuple, g.tolist()))
+    su = set(map(tuple, u.tolist()))
+    assert len(sg) > 50 and sg <= su and len(su) > len(sg)
+    assert zs["raw"] == 0 and len(z) == 0
+    assert r1s["raw"] <= 30 * 20 and gs["raw"] <= 30 * 20 * 100
+    assert us["raw"] >= gs["raw"] and us["rays"] == gs["rays"] == 30 * 20 * 2
