@@ -194,8 +194,9 @@ def build_channel(src: torch.Tensor, dst: torch.Tensor, src_dir, dst_dir, src_bp
     """go.py:623-767: trace src -> dst and return complex64 [n_src, n_dst].
 
     Either scene_folder (STL files + mat.txt) or an assembled `scene`.
-    rays defaults to the reference's 100 x 100 x samples_per_launch launch;
-    per_element=True gives every src element the full Fibonacci set."""
+    rays None: the reference's own launch (go.py:739-765), 100 x 100 threads
+    x samples_per_launch TEA/LCG rays with reserved_size records per thread;
+    rays given: that many Fibonacci rays (per src element if per_element)."""
     if scene is None:
         if scene_folder is None:
             raise ValueError("build_channel needs scene_folder or scene")
@@ -207,12 +208,15 @@ def build_channel(src: torch.Tensor, dst: torch.Tensor, src_dir, dst_dir, src_bp
     s, d = _np(src), _np(dst)
     tx = ArraySpec(s, _np(src_dir).reshape(3), int(src_bptype), float(src_radius))
     rx = ArraySpec(d, _np(dst_dir).reshape(3), int(dst_bptype), float(dst_radius))
-    n_rays = rays if rays is not None else 100 * 100 * int(samples_per_launch)
     result = torch.zeros(tx.n, rx.n, 2, dtype=torch.float32, device=device)
     ts = TraceScene(scene)
     try:
-        trace(ts, tx, rx, wave_len, int(max_reflection), int(n_rays), per_element=per_element,
-              gridsize=int(dst_gridsize), result=result)
+        if rays is None:
+            trace(ts, tx, rx, wave_len, int(max_reflection), 0, per_element=False, gridsize=int(dst_gridsize),
+                  result=result, reference_launch=(100, 100, int(samples_per_launch), int(reserved_size)))
+        else:
+            trace(ts, tx, rx, wave_len, int(max_reflection), int(rays), per_element=per_element,
+                  gridsize=int(dst_gridsize), result=result)
     finally:
         ts.close()
     return torch.complex(result[..., 0], result[..., 1])

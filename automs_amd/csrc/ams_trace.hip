@@ -154,6 +154,28 @@ __device__ __forceinline__ int closest_tri(const AmsNode *__restrict__ nodes, in
   return hit;
 }
 
+// One reception sphere (centre, original index in .w): accepted when the
+// miss distance is below the acceptance radius, the closest approach b lies
+// in [tmin, tbest] and the origin is outside the sphere
+// (optixPathTracer.cu:436-444).  Ties on b go to the smaller index, so the
+// outcome does not depend on the order candidates are visited in.
+__device__ __forceinline__ void sphere_candidate(float4 s, const Ray &r, float tmin, float &tbest, float acc2,
+                                                 float r2, int &hit, int &horig) {
+  float3 O = f3(s.x, s.y, s.z) - r.o;
+  float b = dot3(O, r.d);
+  float3 P = O - r.d * b;
+  float perp2 = dot3(P, P);
+  float c = dot3(O, O);
+  int orig = __float_as_int(s.w);
+  if (perp2 < acc2 && b >= tmin && b <= tbest && c > r2) {
+    if (b < tbest || orig < horig) {
+      tbest = b;
+      hit = orig;
+      horig = orig;
+    }
+  }
+}
+
 // Closest reception sphere whose closest approach b lies in [tmin, tmax],
 // with miss distance below the acceptance radius and the origin outside the
 // sphere (optixPathTracer.cu:436-444).  Reported at t = b.
@@ -174,24 +196,76 @@ __device__ __forceinline__ int closest_sphere(const AmsNode *__restrict__ nodes,
       }
       int first = info >> 3, cnt = info & 7;
       for (int k = 0; k < cnt; k++) {
-        float4 s = sph[first + k];
         ct.sph++;
-        float3 O = f3(s.x, s.y, s.z) - r.o;
-        float b = dot3(O, r.d);
-        float3 P = O - r.d * b;
-        float perp2 = dot3(P, P);
-        float c = dot3(O, O);
-        int orig = __float_as_int(s.w);
-        if (perp2 < acc2 && b >= tmin && b <= tbest && c > r2) {
-          if (b < tbest || orig < horig) {
-            tbest = b;
-            hit = orig;
-            horig = orig;
-          }
-        }
+        sphere_candidate(sph[first + k], r, tmin, tbest, acc2, r2, hit, horig);
       }
     }
     i = miss;
+  }
+  return hit;
+}
+
+// Rx arrays laid out as a regular lattice c0 + i u + j v (within delta)
+// skip the sphere hierarchy.  A sphere that passes sphere_candidate has its
+// closest approach q within acc of its centre, hence within acc + delta of
+// its lattice point: q lies in the slab |n.(q - c0)| <= W around the lattice
+// plane and its lattice coordinates are within (acc + delta) / h of (i, j),
+// h being the spacing of the lattice lines.  The cells met that way along
+// the clipped ray segment are a superset of the passing spheres, and the
+// same per-sphere test with its order-free tie rule gives the same hit.
+struct RxLattice {
+  int on, n0, n1;
+  float3 c0, n, gi, gj;  // i(x) = gi . (x - c0), j(x) = gj . (x - c0)
+  float W, Mi, Mj;       // slab half-width (m), cell margins
+};
+
+__device__ __forceinline__ void lattice_clip(float x0, float dx, float lo, float hi, float &ta, float &tb) {
+  if (fabsf(dx) > 1e-30f) {
+    float t1 = (lo - x0) / dx, t2 = (hi - x0) / dx;
+    ta = fmaxf(ta, fminf(t1, t2));
+    tb = fminf(tb, fmaxf(t1, t2));
+  } else if (x0 < lo || x0 > hi) {
+    tb = -INFINITY;
+  }
+}
+__device__ __forceinline__ int lattice_lo(float v, int n) {
+  return (int)ceilf(fminf(fmaxf(v, -1.0f), (float)n));
+}
+__device__ __forceinline__ int lattice_hi(float v, int n) {
+  return (int)floorf(fminf(fmaxf(v, -1.0f), (float)n));
+}
+
+__device__ __forceinline__ int closest_sphere_lattice(const RxLattice &L, const float4 *__restrict__ sph_o,
+                                                      const Ray &r, float tmin, float &tbest, float acc2, float r2,
+                                                      Counters &ct) {
+  int hit = -1, horig = 0x7fffffff;
+  const float3 w0 = r.o - L.c0;
+  const float s0 = dot3(L.n, w0), sd = dot3(L.n, r.d);
+  float ta = tmin, tb = fminf(tbest, 1e30f);
+  lattice_clip(s0, sd, -L.W, L.W, ta, tb);
+  const float i0 = dot3(L.gi, w0), di = dot3(L.gi, r.d);
+  const float j0 = dot3(L.gj, w0), dj = dot3(L.gj, r.d);
+  lattice_clip(i0, di, -L.Mi, (float)(L.n0 - 1) + L.Mi, ta, tb);
+  lattice_clip(j0, dj, -L.Mj, (float)(L.n1 - 1) + L.Mj, ta, tb);
+  if (!(ta <= tb)) return -1;
+  // walk the axis along which the segment is longer, one lattice line at a time
+  const bool imaj = fabsf(di) >= fabsf(dj);
+  const float a0 = imaj ? i0 : j0, da = imaj ? di : dj, b0 = imaj ? j0 : i0, db = imaj ? dj : di;
+  const float Ma = imaj ? L.Mi : L.Mj, Mb = imaj ? L.Mj : L.Mi;
+  const int na = imaj ? L.n0 : L.n1, nb = imaj ? L.n1 : L.n0;
+  const float aA = a0 + ta * da, aB = a0 + tb * da;
+  const int alo = max(0, lattice_lo(fminf(aA, aB) - Ma, na)), ahi = min(na - 1, lattice_hi(fmaxf(aA, aB) + Ma, na));
+  for (int ka = alo; ka <= ahi; ka++) {
+    float tl = ta, th = tb;
+    lattice_clip(a0, da, (float)ka - Ma, (float)ka + Ma, tl, th);
+    if (!(tl <= th)) continue;
+    const float bA = b0 + tl * db, bB = b0 + th * db;
+    const int blo = max(0, lattice_lo(fminf(bA, bB) - Mb, nb)), bhi = min(nb - 1, lattice_hi(fmaxf(bA, bB) + Mb, nb));
+    for (int kb = blo; kb <= bhi; kb++) {
+      const int idx = imaj ? ka * L.n1 + kb : kb * L.n1 + ka;
+      ct.sph++;
+      sphere_candidate(sph_o[idx], r, tmin, tbest, acc2, r2, hit, horig);
+    }
   }
   return hit;
 }
@@ -340,15 +414,36 @@ __global__ void k_fib_keys(long long n, unsigned *__restrict__ keys, unsigned *_
   idx[k] = (unsigned)k;
 }
 
+// Scenes whose threaded BVH and leaf-ordered triangles fit in kSceneLds
+// bytes are staged in LDS by every workgroup (the kernels' LDS template
+// instances): the traversal is a chain of dependent node loads, and from LDS
+// each costs a fraction of an L2 hit.  Larger scenes are read from L2/HBM.
+constexpr int kSceneLds = 32 * 1024;
+static_assert(sizeof(AmsNode) == 2 * sizeof(float4) && sizeof(AmsTri) == 3 * sizeof(float4),
+              "stage_scene copies nodes and triangles as float4");
+
+__device__ __forceinline__ void stage_scene(const AmsNode *__restrict__ nodes, int n_nodes,
+                                            const AmsTri *__restrict__ tris, int n_tris, float4 *smem) {
+  const float4 *sn = reinterpret_cast<const float4 *>(nodes);
+  const float4 *st = reinterpret_cast<const float4 *>(tris);
+  const int n4 = 2 * n_nodes, m4 = 3 * n_tris;
+  for (int k = threadIdx.x; k < n4; k += blockDim.x) smem[k] = sn[k];
+  for (int k = threadIdx.x; k < m4; k += blockDim.x) smem[n4 + k] = st[k];
+  __syncthreads();
+}
+
 struct DiscoverArgs {
   const AmsNode *tnodes;
   int n_tnodes;
   const AmsTri *tris;
+  int n_tris;
   const AmsTriAux *aux;
   const float *mat_prop;
   const AmsNode *snodes;
   int n_snodes;
   const float4 *sph;
+  const float4 *sph_o;  // Rx centres and indices in array order (lattice path)
+  RxLattice lat;
   const float *src;
   int n_src;
   int n1_dst, gs;
@@ -358,11 +453,17 @@ struct DiscoverArgs {
   int per_element;
   long long ray_begin, ray_end;
   const unsigned *perm;  // slot -> direction (per-element) or ray (distributed) index
+  int flags;             // ams_trace_opts.flags
   uint64_t *out_hi, *out_lo;
   unsigned long long *out_count;
   unsigned long long cap;
   unsigned long long *counters;  // nodes, tris, spheres, queries, records(raw)
 };
+
+__device__ __forceinline__ int rx_query(const DiscoverArgs &a, const Ray &ray, float &ts, Counters &ct) {
+  return a.lat.on ? closest_sphere_lattice(a.lat, a.sph_o, ray, AMS_TMIN, ts, a.acc2, a.r2, ct)
+                  : closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
+}
 
 struct Branch {
   float3 o, d;
@@ -370,7 +471,16 @@ struct Branch {
   uint64_t khi, klo;
 };
 
+template <bool LDS>
 __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
+  extern __shared__ float4 ams_smem[];
+  const AmsNode *tnodes = a.tnodes;
+  const AmsTri *tris = a.tris;
+  if (LDS) {
+    stage_scene(a.tnodes, a.n_tnodes, a.tris, a.n_tris, ams_smem);
+    tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
+    tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
+  }
   long long r = a.ray_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   Counters ct;
   bool active = r < a.ray_end;
@@ -401,16 +511,36 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
     uint64_t last_hi = ~0ull, last_lo = ~0ull;  // previous record of this ray
     bool more = true;
     while (more) {
+      // After an Rx hit the ray goes on in the same direction, so its next
+      // scene hit is the one already found (tri_rem further on); only the
+      // sphere query is repeated (flags bit 3 repeats both).
+      bool tri_cached = false;
+      float tri_rem = 0.0f;
+      int tri_slot = -1, tri_orig = 0;
       while (cur.ttl > 0) {
         Ray ray;
         ray_setup(ray, cur.o, cur.d);
-        float tt = INFINITY;
-        int torig;
-        int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+        float tt;
+        int torig, slot;
+        if (tri_cached) {
+          tt = tri_rem;
+          slot = tri_slot;
+          torig = tri_orig;
+        } else {
+          tt = INFINITY;
+          slot = closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+        }
         float ts = tt;
-        int rx = closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
+        int rx = rx_query(a, ray, ts, ct);
         ct.queries++;
+        tri_cached = false;
         if (rx >= 0 && ts < tt) {
+          if (!(a.flags & 8)) {
+            tri_cached = true;
+            tri_rem = tt - ts;
+            tri_slot = slot;
+            tri_orig = torig;
+          }
           // Rx hit: record (key, tile) and continue from the closest approach
           int h = a.K + 1 - cur.ttl;
           uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
@@ -431,7 +561,7 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
           continue;
         }
         if (slot < 0) break;  // miss, optixPathTracer.cu:276-279
-        const AmsTri tr = a.tris[slot];
+        const AmsTri tr = tris[slot];
         int pl = __float_as_int(tr.b.w);
         cur.o = cur.o + cur.d * tt;
         cur.last_pl = pl;
@@ -478,6 +608,9 @@ k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
   Branch cur;
   bool have = false;
   uint64_t last_hi = ~0ull, last_lo = ~0ull;  // the lane's previous record (consecutive duplicates)
+  bool tri_cached = false;                     // scene hit kept across an Rx hit (see k_discover)
+  float tri_rem = 0.0f;
+  int tri_slot = -1, tri_orig = 0;
   const int lane = __lane_id();
   while (true) {
     unsigned long long need = __ballot(!have);
@@ -510,6 +643,7 @@ k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
           cur.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
           last_hi = ~0ull;
           last_lo = ~0ull;
+          tri_cached = false;
           have = true;
         }
       }
@@ -518,14 +652,28 @@ k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
     if (have) {
       Ray ray;
       ray_setup(ray, cur.o, cur.d);
-      float tt = INFINITY;
-      int torig;
-      int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+      float tt;
+      int torig, slot;
+      if (tri_cached) {
+        tt = tri_rem;
+        slot = tri_slot;
+        torig = tri_orig;
+      } else {
+        tt = INFINITY;
+        slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+      }
       float ts = tt;
-      int rx = closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
+      int rx = rx_query(a, ray, ts, ct);
       ct.queries++;
       bool ended = false;
+      tri_cached = false;
       if (rx >= 0 && ts < tt) {
+        if (!(a.flags & 8)) {
+          tri_cached = true;
+          tri_rem = tt - ts;
+          tri_slot = slot;
+          tri_orig = torig;
+        }
         int h = a.K + 1 - cur.ttl;
         uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
         uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
@@ -566,6 +714,7 @@ k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
         }
       }
       if (ended || cur.ttl <= 0) {
+        tri_cached = false;
         if (sp > 0) cur = st[--sp];
         else have = false;
       }
@@ -604,8 +753,17 @@ struct RefArgs {
 // order (a reflection child lands in a higher slot than its parent), so the
 // order of Rx hits -- and with it the per-tile min_hc gating and the
 // reserve cap -- follows the reference.
+template <bool LDS>
 __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
   const DiscoverArgs &a = ra.d;
+  extern __shared__ float4 ams_smem[];
+  const AmsNode *tnodes = a.tnodes;
+  const AmsTri *tris = a.tris;
+  if (LDS) {
+    stage_scene(a.tnodes, a.n_tnodes, a.tris, a.n_tris, ams_smem);
+    tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
+    tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   Counters ct;
   unsigned nrec = 0;
@@ -635,16 +793,33 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
       s0.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
       for (int i = 0; i < (1 << AMS_MAX_HIT); i++) {
         Branch cur = slot[i];
+        bool tri_cached = false;  // scene hit kept across an Rx hit (see k_discover)
+        float tri_rem = 0.0f;
+        int tri_slot = -1, tri_orig = 0;
         while (cur.ttl > 0) {
           Ray ray;
           ray_setup(ray, cur.o, cur.d);
-          float tt = INFINITY;
-          int torig;
-          int sl = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+          float tt;
+          int torig, sl;
+          if (tri_cached) {
+            tt = tri_rem;
+            sl = tri_slot;
+            torig = tri_orig;
+          } else {
+            tt = INFINITY;
+            sl = closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+          }
           float ts = tt;
-          int rx = closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
+          int rx = rx_query(a, ray, ts, ct);
           ct.queries++;
+          tri_cached = false;
           if (rx >= 0 && ts < tt) {
+            if (!(a.flags & 8)) {
+              tri_cached = true;
+              tri_rem = tt - ts;
+              tri_slot = sl;
+              tri_orig = torig;
+            }
             if (res_cnt < ra.reserve) {  // optixPathTracer.cu:173-185
               int grid = (int)ams_tile_of(rx, a.n1_dst, a.gs);
               int h = a.K + 1 - cur.ttl;
@@ -666,7 +841,7 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
             continue;
           }
           if (sl < 0) break;
-          const AmsTri tr = a.tris[sl];
+          const AmsTri tr = tris[sl];
           int pl = __float_as_int(tr.b.w);
           cur.o = cur.o + cur.d * tt;
           cur.last_pl = pl;
@@ -705,6 +880,7 @@ struct EvalArgs {
   const AmsNode *tnodes;
   int n_tnodes;
   const AmsTri *tris;
+  int n_tris;
   const AmsTriAux *aux;
   const float *mat_prop;
   const float *refl_phase;
@@ -729,7 +905,16 @@ struct EvalArgs {
   unsigned long long *counters;  // nodes, tris, spheres, queries, chains, successes
 };
 
+template <bool LDS>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) k_evaluate(EvalArgs a) {
+  extern __shared__ float4 ams_smem[];
+  const AmsNode *tnodes = a.tnodes;
+  const AmsTri *tris = a.tris;
+  if (LDS) {
+    stage_scene(a.tnodes, a.n_tnodes, a.tris, a.n_tris, ams_smem);
+    tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
+    tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
+  }
   int unit = blockIdx.y;
   int item = blockIdx.x * blockDim.x + threadIdx.x;
   int g = a.unit_tile[unit];
@@ -801,7 +986,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
         ray_setup(ray, o, d);
         float tt = tlim;
         int torig;
-        int slot = closest_tri(a.tnodes, a.n_tnodes, a.tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
+        int slot = closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
         ct.queries++;
         if (slot < 0) tt = INFINITY;
         if (sh && bs < tt && bs <= tlim) {
@@ -821,7 +1006,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
           break;
         }
         if (slot < 0) break;  // miss
-        const AmsTri tr = a.tris[slot];
+        const AmsTri tr = tris[slot];
         int pl = __float_as_int(tr.b.w);
         o = o + d * tt;
         D += tt;
@@ -1023,6 +1208,62 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Fit c0 + i u + j v to an Rx array (u, v from its corner elements, FP64)
+// and return the lattice query's parameters when every element lies within
+// a quarter of the line spacing of its lattice point.
+static RxLattice rx_lattice(const ams_array *rx, float acc, bool allow) {
+  RxLattice L;
+  memset(&L, 0, sizeof L);
+  const int n0 = rx->n0, n1 = rx->n1;
+  if (!allow || n0 < 2 || n1 < 2) return L;
+  auto P = [&](int i, int j, int k) { return (double)rx->pos[3 * ((size_t)i * n1 + j) + k]; };
+  double c0[3], u[3], v[3];
+  for (int k = 0; k < 3; k++) {
+    c0[k] = P(0, 0, k);
+    u[k] = (P(n0 - 1, 0, k) - c0[k]) / (n0 - 1);
+    v[k] = (P(0, n1 - 1, k) - c0[k]) / (n1 - 1);
+  }
+  double cr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+  double area = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+  double lu = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), lv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (!(area > 1e-12) || !(lu > 0) || !(lv > 0)) return L;
+  double hi = area / lv, hj = area / lu;  // spacing of the lines of constant i, of constant j
+  double delta = 0.0, cmax = 0.0;
+  for (int i = 0; i < n0; i++)
+    for (int j = 0; j < n1; j++) {
+      double e2 = 0.0;
+      for (int k = 0; k < 3; k++) {
+        double e = P(i, j, k) - (c0[k] + i * u[k] + j * v[k]);
+        e2 += e * e;
+        cmax = std::max(cmax, std::fabs(P(i, j, k)));
+      }
+      delta = std::max(delta, std::sqrt(e2));
+    }
+  if (!(delta <= 0.25 * std::min(hi, hj))) return L;
+  double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2], vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  double uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+  double det = uu * vv - uv * uv;
+  double gi[3], gj[3];
+  for (int k = 0; k < 3; k++) {
+    gi[k] = (vv * u[k] - uv * v[k]) / det;
+    gj[k] = (uu * v[k] - uv * u[k]) / det;
+  }
+  // f32 evaluation of the slab and the lattice coordinates errs by far less
+  // than the 0.1 mm and 0.01-cell allowances added here
+  double reach = (double)acc + delta + 1e-4 + 1e-6 * cmax;
+  L.on = 1;
+  L.n0 = n0;
+  L.n1 = n1;
+  L.c0 = make_float3((float)c0[0], (float)c0[1], (float)c0[2]);
+  L.n = make_float3((float)(cr[0] / area), (float)(cr[1] / area), (float)(cr[2] / area));
+  L.gi = make_float3((float)gi[0], (float)gi[1], (float)gi[2]);
+  L.gj = make_float3((float)gj[0], (float)gj[1], (float)gj[2]);
+  L.W = (float)reach;
+  L.Mi = (float)(reach / hi + 0.01);
+  L.Mj = (float)(reach / hj + 0.01);
+  return L;
+}
+
 int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const ams_trace_opts *o, float *H_out,
                   ams_trace_stats *stats) {
   auto t_start = std::chrono::steady_clock::now();
@@ -1062,6 +1303,17 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   HIPCHK(hipMemcpyAsync(d_snodes, SB.nodes.data(), sizeof(AmsNode) * SB.nodes.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_sph, sph.data(), sizeof(float4) * sph.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
+  // regular Rx lattice: sphere candidates found without the hierarchy (flags bit 5 keeps it)
+  RxLattice lat = rx_lattice(rx, dst_acc, !(o->flags & 32));
+  float4 *d_sph_o = nullptr;
+  if (lat.on) {
+    std::vector<float4> sph_o(n_dst);
+    for (int j = 0; j < n_dst; j++)
+      sph_o[j] = make_float4(rx->pos[3 * j], rx->pos[3 * j + 1], rx->pos[3 * j + 2], bits_f(j));
+    d_sph_o = (float4 *)sc.get("sph_o", sizeof(float4) * n_dst);
+    if (!d_sph_o) return AMS_ERR_ALLOC;
+    HIPCHK(hipMemcpyAsync(d_sph_o, sph_o.data(), sizeof(float4) * n_dst, hipMemcpyHostToDevice, st));
+  }
 
   // ---------------- forward discovery in chunks of rays -----------------
   const long long n_rays = o->discovery_mode == 1
@@ -1081,11 +1333,18 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   da.tnodes = s->d_tnodes;
   da.n_tnodes = s->n_tnodes;
   da.tris = s->d_tris;
+  da.n_tris = s->n_tri;
+  // LDS-staged scene for small scenes (flags bit 4 keeps it in global memory)
+  const size_t scene_lds = (size_t)s->n_tnodes * sizeof(AmsNode) + (size_t)s->n_tri * sizeof(AmsTri);
+  const bool use_lds = scene_lds <= (size_t)kSceneLds && !(o->flags & 16);
+  const unsigned lds_bytes = use_lds ? (unsigned)scene_lds : 0u;
   da.aux = s->d_aux;
   da.mat_prop = s->d_mat_prop;
   da.snodes = d_snodes;
   da.n_snodes = (int)SB.nodes.size();
   da.sph = d_sph;
+  da.sph_o = d_sph_o;
+  da.lat = lat;
   da.src = d_src;
   da.n_src = n_src;
   da.n1_dst = rx->n1;
@@ -1095,6 +1354,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   da.K = K;
   da.fib_n = o->fib_n;
   da.per_element = o->per_element;
+  da.flags = o->flags;
   da.out_hi = c_hi;
   da.out_lo = c_lo;
   da.out_count = d_cnt;
@@ -1179,7 +1439,8 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ra.n_grid = (int)n_grid;
     ra.min_hc = d_minhc;
     HIPCHK(hipEventRecord(e0, st));
-    k_discover_ref<<<(unsigned)((nthr + 255) / 256), 256, 0, st>>>(ra);
+    if (use_lds) k_discover_ref<true><<<(unsigned)((nthr + 255) / 256), 256, lds_bytes, st>>>(ra);
+    else k_discover_ref<false><<<(unsigned)((nthr + 255) / 256), 256, 0, st>>>(ra);
     HIPCHK(hipGetLastError());
     S.n_discover_launches++;
     HIPCHK(hipEventRecord(e1, st));
@@ -1227,7 +1488,8 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
       unsigned pblocks = std::min<unsigned>(blocks, (unsigned)(n_cu * 8));
       k_discover_persistent<<<pblocks, 256, 0, st>>>(da, d_next);
     } else {
-      k_discover<<<blocks, 256, 0, st>>>(da);
+      if (use_lds) k_discover<true><<<blocks, 256, lds_bytes, st>>>(da);
+      else k_discover<false><<<blocks, 256, 0, st>>>(da);
     }
     HIPCHK(hipGetLastError());
     S.n_discover_launches++;
@@ -1368,6 +1630,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ea.tnodes = s->d_tnodes;
     ea.n_tnodes = s->n_tnodes;
     ea.tris = s->d_tris;
+    ea.n_tris = s->n_tri;
     ea.aux = s->d_aux;
     ea.mat_prop = s->d_mat_prop;
     ea.refl_phase = s->d_refl_phase;
@@ -1437,7 +1700,8 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         eb.unit_tile += u0;
         eb.unit_begin += u0;
         eb.unit_end += u0;
-        k_evaluate<<<dim3(bpt, un), block, 0, st>>>(eb);
+        if (use_lds) k_evaluate<true><<<dim3(bpt, un), block, lds_bytes, st>>>(eb);
+        else k_evaluate<false><<<dim3(bpt, un), block, 0, st>>>(eb);
         HIPCHK(hipGetLastError());
         S.n_evaluate_launches++;
       }

@@ -400,8 +400,8 @@ def calibrate_temperature(batch_score, seed: int, pairs: int = 32, chi0: float =
 
 def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 2000, search_scene: int = 1000,
                    epochs: int = 20, fib: int = 1 << 18, seed: int = 0, log=print, datasets=None,
-                   t0: Optional[float] = None, t1: Optional[float] = None, chi0: float = 0.5,
-                   t_ratio: float = 100.0, net_path: Optional[str] = None, searches=("plain", "surrogate")) -> dict:
+                   t0: Optional[float] = None, t1: Optional[float] = None, chi0: float = 0.2,
+                   t_ratio: float = 1000.0, net_path: Optional[str] = None, searches=("plain", "surrogate")) -> dict:
     """Config-4 experiment: traced dataset -> trained surrogate -> placement
     search three ways on an unseen scene: plain annealing (every candidate
     traced), surrogate-guided annealing (accepted candidates traced), and
@@ -512,8 +512,8 @@ if __name__ == "__main__":
     ap.add_argument("--first-scene", type=int, default=0)
     ap.add_argument("--datasets", default=None, help="comma-separated npz files to train on")
     ap.add_argument("--net", default=None, help="network weights (.npy): loaded if present, else saved after training")
-    ap.add_argument("--chi0", type=float, default=0.5, help="start acceptance of a median downhill move")
-    ap.add_argument("--t-ratio", type=float, default=100.0, help="start / end temperature")
+    ap.add_argument("--chi0", type=float, default=0.2, help="start acceptance of a median downhill move")
+    ap.add_argument("--t-ratio", type=float, default=1000.0, help="start / end temperature")
     ap.add_argument("--searches", default="plain,surrogate")
     a = ap.parse_args()
     logf = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731

@@ -45,7 +45,13 @@ typedef struct {
   int32_t flags;          /* bit 0: keep the reference's row wrap of the last tile column;
                              bit 1: evaluate the round-1 tasks only (no neighbour expansion);
                              bit 2: wavefront-persistent discovery (ray refill) instead of
-                                    one thread per ray */
+                                    one thread per ray;
+                             bit 3: after an Rx hit, query the scene again instead of keeping
+                                    the scene hit already found on the same line;
+                             bit 4: read the scene hierarchy from global memory even when it is
+                                    small enough to be staged in LDS (32 KB);
+                             bit 5: find Rx sphere hits through the sphere hierarchy also when
+                                    the Rx array is a regular lattice */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
   void *stream;           /* hipStream_t; NULL = default stream */
   uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */

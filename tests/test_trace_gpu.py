@@ -203,3 +203,34 @@ def test_reference_mode_rejects_bad_launch(trace_scene, small_room):
     with pytest.raises(RuntimeError):
         trace(trace_scene, w.tx, w.rx, w.wave_len, w.max_reflection, 0, False, w.gridsize,
               reference_launch=(0, 10, 1, 10))
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.06])
+def test_rx_lattice_query_matches_hierarchy(small_room, trace_scene, jitter):
+    """The lattice sphere query (regular Rx arrays) returns the hits of the
+    sphere hierarchy (flags bit 5): identical record sets and H, also for Rx
+    positions jittered off the lattice by up to 6% of the spacing."""
+    import torch
+
+    from automs_amd.scenes import ArraySpec
+    from automs_amd.tracer import trace
+    w = small_room
+    _flush(f"case: rx lattice vs hierarchy jitter={jitter}")
+    pos = np.asarray(w.rx.pos, np.float32).copy()
+    if jitter:
+        sp = float(np.linalg.norm(pos[1, 0] - pos[0, 0]))
+        pos += (np.random.default_rng(3).uniform(-1, 1, pos.shape) * jitter * sp / np.sqrt(3)).astype(np.float32)
+    rx = ArraySpec(pos, w.rx.dir, w.rx.bptype, w.rx.radius)
+    out = {}
+    for flags in (0, 32):
+        res, st, keys = trace(trace_scene, w.tx, rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
+                              w.gridsize, flags=flags, keys_cap=1 << 20)
+        torch.cuda.synchronize()
+        out[flags] = (res.double().cpu().numpy(), _keyset(keys), st)
+    (Ha, ka, sa), (Hb, kb, sb) = out[0], out[32]
+    rel = np.linalg.norm(Ha - Hb) / np.linalg.norm(Hb)
+    _flush(f"  keys lattice={len(ka)} hierarchy={len(kb)} symdiff={len(ka ^ kb)} H rel={rel:.2e} "
+           f"sphere tests {sa['fwd_spheres']} vs {sb['fwd_spheres']}, nodes {sa['fwd_nodes']} vs {sb['fwd_nodes']}")
+    assert ka == kb
+    assert rel < 1e-6
+    assert sa["fwd_nodes"] < sb["fwd_nodes"]
