@@ -346,6 +346,41 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long *co
   return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Per-workgroup filter of discovery records in LDS: neighbouring rays keep
+// finding the same (path, tile) keys, and only the first copy a workgroup
+// finds needs to reach the global record list (its sort and dedup produce
+// the unique set).  A key is dropped only when the same key is already in
+// the table and was written out; a slot another lane is filling, or a
+// crowded neighbourhood, lets the record through, so every unique key is
+// written at least once whatever the interleaving.
+constexpr int kDedup = 512;  // entries (10 KB of LDS)
+struct DedupTable {
+  unsigned long long hi[kDedup], lo[kDedup];
+  unsigned state[kDedup];  // 0 empty, 1 being filled, 2 holds a key already written out
+};
+__device__ __forceinline__ void dedup_init(DedupTable &t) {
+  for (int k = threadIdx.x; k < kDedup; k += blockDim.x) t.state[k] = 0u;
+  __syncthreads();
+}
+__device__ __forceinline__ bool dedup_first(DedupTable &t, uint64_t hi, uint64_t lo) {
+  uint64_t m = (hi ^ (lo * 0x9E3779B97F4A7C15ull)) * 0xBF58476D1CE4E5B9ull;
+  unsigned h = (unsigned)(m >> 40);
+  for (int p = 0; p < 8; p++) {
+    unsigned slot = (h + p) & (kDedup - 1);
+    unsigned st = atomicCAS(&t.state[slot], 0u, 1u);
+    if (st == 0u) {
+      t.hi[slot] = hi;
+      t.lo[slot] = lo;
+      __threadfence_block();
+      atomicExch(&t.state[slot], 2u);
+      return true;
+    }
+    if (st == 1u) return true;
+    if (t.hi[slot] == hi && t.lo[slot] == lo) return false;
+  }
+  return true;
+}
+
 __device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const Counters &c) {
   unsigned long long v[4] = {(unsigned long long)c.nodes, (unsigned long long)c.tris, (unsigned long long)c.sph,
                              (unsigned long long)c.queries};
@@ -481,6 +516,8 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
     tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
     tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
   }
+  __shared__ DedupTable dd;
+  dedup_init(dd);
   long long r = a.ray_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   Counters ct;
   bool active = r < a.ray_end;
@@ -546,9 +583,7 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
           uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
           uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
           uint64_t lo = (cur.klo & ~0xFFFFFull) | tile;
-          if (hi != last_hi || lo != last_lo) {
-            last_hi = hi;
-            last_lo = lo;
+          if ((hi != last_hi || lo != last_lo) && dedup_first(dd, hi, lo)) {
             unsigned long long idx = wave_append(a.out_count);
             nrec++;
             if (idx < a.cap) {
@@ -556,6 +591,8 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
               a.out_lo[idx] = lo;
             }
           }
+          last_hi = hi;
+          last_lo = lo;
           cur.o = cur.o + cur.d * ts;
           cur.last_pl = -1;
           continue;
@@ -764,6 +801,8 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
     tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
     tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
   }
+  __shared__ DedupTable dd;
+  dedup_init(dd);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   Counters ct;
   unsigned nrec = 0;
@@ -827,11 +866,13 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
                 min_hc[grid] = h + 1;
                 uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
                 uint64_t lo = (cur.klo & ~0xFFFFFull) | (uint64_t)grid;
-                unsigned long long idx = wave_append(a.out_count);
-                nrec++;
-                if (idx < a.cap) {
-                  a.out_hi[idx] = hi;
-                  a.out_lo[idx] = lo;
+                if (dedup_first(dd, hi, lo)) {
+                  unsigned long long idx = wave_append(a.out_count);
+                  nrec++;
+                  if (idx < a.cap) {
+                    a.out_hi[idx] = hi;
+                    a.out_lo[idx] = lo;
+                  }
                 }
                 res_cnt++;
               }
