@@ -235,19 +235,29 @@ __device__ __forceinline__ int lattice_hi(float v, int n) {
   return (int)floorf(fminf(fmaxf(v, -1.0f), (float)n));
 }
 
+// The part [ta, tb] of the ray o + t d that lies in the lattice slab within
+// the lattice's extent (margins included), and its lattice coordinates
+// i(t) = i0 + t di, j(t) = j0 + t dj; false when that part is empty.
+__device__ __forceinline__ bool lattice_segment(const RxLattice &L, float3 o, float3 d, float &ta, float &tb,
+                                                float &i0, float &di, float &j0, float &dj) {
+  const float3 w0 = o - L.c0;
+  const float s0 = dot3(L.n, w0), sd = dot3(L.n, d);
+  lattice_clip(s0, sd, -L.W, L.W, ta, tb);
+  i0 = dot3(L.gi, w0);
+  di = dot3(L.gi, d);
+  j0 = dot3(L.gj, w0);
+  dj = dot3(L.gj, d);
+  lattice_clip(i0, di, -L.Mi, (float)(L.n0 - 1) + L.Mi, ta, tb);
+  lattice_clip(j0, dj, -L.Mj, (float)(L.n1 - 1) + L.Mj, ta, tb);
+  return ta <= tb;
+}
+
 __device__ __forceinline__ int closest_sphere_lattice(const RxLattice &L, const float4 *__restrict__ sph_o,
                                                       const Ray &r, float tmin, float &tbest, float acc2, float r2,
                                                       Counters &ct) {
   int hit = -1, horig = 0x7fffffff;
-  const float3 w0 = r.o - L.c0;
-  const float s0 = dot3(L.n, w0), sd = dot3(L.n, r.d);
-  float ta = tmin, tb = fminf(tbest, 1e30f);
-  lattice_clip(s0, sd, -L.W, L.W, ta, tb);
-  const float i0 = dot3(L.gi, w0), di = dot3(L.gi, r.d);
-  const float j0 = dot3(L.gj, w0), dj = dot3(L.gj, r.d);
-  lattice_clip(i0, di, -L.Mi, (float)(L.n0 - 1) + L.Mi, ta, tb);
-  lattice_clip(j0, dj, -L.Mj, (float)(L.n1 - 1) + L.Mj, ta, tb);
-  if (!(ta <= tb)) return -1;
+  float ta = tmin, tb = fminf(tbest, 1e30f), i0, di, j0, dj;
+  if (!lattice_segment(L, r.o, r.d, ta, tb, i0, di, j0, dj)) return -1;
   // walk the axis along which the segment is longer, one lattice line at a time
   const bool imaj = fabsf(di) >= fabsf(dj);
   const float a0 = imaj ? i0 : j0, da = imaj ? di : dj, b0 = imaj ? j0 : i0, db = imaj ? dj : di;
@@ -255,10 +265,15 @@ __device__ __forceinline__ int closest_sphere_lattice(const RxLattice &L, const 
   const int na = imaj ? L.n0 : L.n1, nb = imaj ? L.n1 : L.n0;
   const float aA = a0 + ta * da, aB = a0 + tb * da;
   const int alo = max(0, lattice_lo(fminf(aA, aB) - Ma, na)), ahi = min(na - 1, lattice_hi(fmaxf(aA, aB) + Ma, na));
-  for (int ka = alo; ka <= ahi; ka++) {
+  // lines in the order the ray meets them: once a line's part of the ray
+  // starts beyond the best hit, so do all later ones
+  const int step = da >= 0.0f ? 1 : -1;
+  const int kend = da >= 0.0f ? ahi + 1 : alo - 1;
+  for (int ka = da >= 0.0f ? alo : ahi; ka != kend; ka += step) {
     float tl = ta, th = tb;
     lattice_clip(a0, da, (float)ka - Ma, (float)ka + Ma, tl, th);
     if (!(tl <= th)) continue;
+    if (tl > tbest) break;
     const float bA = b0 + tl * db, bB = b0 + th * db;
     const int blo = max(0, lattice_lo(fminf(bA, bB) - Mb, nb)), bhi = min(nb - 1, lattice_hi(fmaxf(bA, bB) + Mb, nb));
     for (int kb = blo; kb <= bhi; kb++) {
@@ -500,6 +515,22 @@ __device__ __forceinline__ int rx_query(const DiscoverArgs &a, const Ray &ray, f
                   : closest_sphere(a.snodes, a.n_snodes, a.sph, ray, AMS_TMIN, ts, a.acc2, a.r2, ct);
 }
 
+// Can the ray o + t d (t >= tmin) still hit a reception sphere?  A branch
+// with no reflection left (ttl == 1) records nothing else, so when the
+// answer is no its remaining pass-through queries are skipped.  Exact: a hit
+// needs the closest approach inside the lattice slab and extent, or inside
+// the sphere hierarchy's root box.
+__device__ __forceinline__ bool rx_reachable(const DiscoverArgs &a, float3 o, float3 d) {
+  if (a.lat.on) {
+    float ta = AMS_TMIN, tb = 1e30f, i0, di, j0, dj;
+    return lattice_segment(a.lat, o, d, ta, tb, i0, di, j0, dj);
+  }
+  Ray ray;
+  ray_setup(ray, o, d);
+  const float4 *np = reinterpret_cast<const float4 *>(a.snodes);
+  return box_test(ray, np[0], np[1], AMS_TMIN, INFINITY);
+}
+
 struct Branch {
   float3 o, d;
   int ttl, b, nref, last_pl;
@@ -619,6 +650,7 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
           cur.nref++;
         }
         // ttl == 1: scene is pass-through (optixPathTracer.cu:294)
+        if (cur.ttl == 1 && !(a.flags & 128) && !rx_reachable(a, cur.o, cur.d)) break;
       }
       if (sp > 0) cur = st[--sp];
       else more = false;
@@ -749,6 +781,7 @@ k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
           if (cur.nref < AMS_MAX_HIT) key_set_plane(cur.khi, cur.klo, cur.nref, (uint32_t)pl);
           cur.nref++;
         }
+        if (cur.ttl == 1 && !(a.flags & 128) && !rx_reachable(a, cur.o, cur.d)) ended = true;
       }
       if (ended || cur.ttl <= 0) {
         tri_cached = false;
@@ -904,6 +937,7 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
             slot[cs] = ch;
             if (!pen) break;
           }
+          if (cur.ttl == 1 && !(a.flags & 128) && !rx_reachable(a, cur.o, cur.d)) break;
         }
       }
     }
@@ -1360,8 +1394,11 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   const long long n_rays = o->discovery_mode == 1
                                ? (long long)o->launch_x * o->launch_y * o->samples_per_launch
                                : (o->per_element ? (long long)n_src * o->fib_n : o->fib_n);
-  const long long chunk = std::min<long long>(n_rays, 1LL << 22);
-  const unsigned long long cap = (unsigned long long)chunk * 6 + 1024;
+  // one launch covers up to 2^24 rays; the per-workgroup record filter keeps
+  // the raw records near 0.3 per ray, and a launch whose records overflow
+  // the 4-per-ray buffer (1 GB at 2^24 rays) is split and redone
+  const long long chunk = std::min<long long>(n_rays, 1LL << 24);
+  const unsigned long long cap = (unsigned long long)chunk * 4 + 1024;
   uint64_t *c_hi = (uint64_t *)sc.get("c_hi", cap * 8), *c_lo = (uint64_t *)sc.get("c_lo", cap * 8);
   unsigned long long *d_cnt = (unsigned long long *)sc.get("c_cnt", 8);
   unsigned long long *d_next = (unsigned long long *)sc.get("c_next", 8);

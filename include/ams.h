@@ -51,7 +51,9 @@ typedef struct {
                              bit 4: read the scene hierarchy from global memory even when it is
                                     small enough to be staged in LDS (32 KB);
                              bit 5: find Rx sphere hits through the sphere hierarchy also when
-                                    the Rx array is a regular lattice */
+                                    the Rx array is a regular lattice;
+                             bit 7: keep tracing a branch with no reflection left even when it
+                                    can no longer reach the Rx array */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
   void *stream;           /* hipStream_t; NULL = default stream */
   uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */
