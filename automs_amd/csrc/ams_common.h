@@ -42,6 +42,11 @@ struct __align__(16) AmsPlane {
   double d;
 };
 
+// Axis-aligned bounds of all triangles of a plane id (FP64 rounded outward).
+struct __align__(16) AmsPlaneBox {
+  float4 lo, hi;
+};
+
 // Path key, compared as (hi, lo):
 //   hi = h:3 | b:5 | p0:20 | p1:20 | p2[19:4]:16
 //   lo = p2[3:0]:4 | p3:20 | p4:20 | tile:20

@@ -69,6 +69,7 @@ struct ams_scene {
   AmsTri *d_tris = nullptr;
   AmsTriAux *d_aux = nullptr;
   AmsPlane *d_planes = nullptr;
+  AmsPlaneBox *d_pbox = nullptr;
   float *d_mat_prop = nullptr;
   float *d_refl_phase = nullptr;
   ams::Scratch scratch;

@@ -960,6 +960,7 @@ struct EvalArgs {
   const float *mat_prop;
   const float *refl_phase;
   const AmsPlane *planes;
+  const AmsPlaneBox *pbox;
   const float *src;
   int n_src;
   const float *dst;
@@ -1034,6 +1035,38 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
       float D = 0.0f, Dres = 0.0f, far = 1e6f, psi = 0.0f;
       int ttl = h + 1, ri = 0, last_pl = -1;
       nchain++;
+      // Unfold the path analytically before any query: from the Rx towards
+      // the image, every reflection point must lie ahead and inside the
+      // bounds of its plane's triangles, and the Tx ahead of the last one.
+      // A chain failing that (with 0.1 mm / 1 mm of slack over the FP32
+      // walk) fails the traced test as well, so it is skipped unqueried.
+      if (!(a.flags & 256)) {
+        float3 po = o, pd = d;
+        bool geo = true;
+        for (int q = 0; q < nref && geo; q++) {
+          const uint32_t pid = key_get_plane(khi, klo, nref - 1 - q);
+          const AmsPlane P = a.planes[pid];
+          float3 pn = f3((float)P.n[0], (float)P.n[1], (float)P.n[2]);
+          float nd = dot3(pn, pd);
+          float tp = nd != 0.0f ? ((float)P.d - dot3(pn, po)) / nd : -1.0f;
+          if (!(tp > -1e-4f)) {
+            geo = false;
+            break;
+          }
+          float3 X = po + pd * tp;
+          const AmsPlaneBox bx = a.pbox[pid];
+          float mg = 1e-3f + 1e-6f * fmaxf(fmaxf(fabsf(X.x), fabsf(X.y)), fabsf(X.z));
+          if (X.x < bx.lo.x - mg || X.x > bx.hi.x + mg || X.y < bx.lo.y - mg || X.y > bx.hi.y + mg ||
+              X.z < bx.lo.z - mg || X.z > bx.hi.z + mg) {
+            geo = false;
+            break;
+          }
+          pd = norm3(pd - pn * (2.0f * dot3(pd, pn)));
+          po = X;
+        }
+        if (geo && !(dot3(s3 - po, pd) > -1e-4f)) geo = false;
+        if (!geo) ttl = 0;  // no leg is traced; the chain counts as failed
+      }
       while (ttl > 0) {
         // own Tx sphere (other Tx spheres are pass-through: optixPathTracer.cu:396)
         float3 O = s3 - o;
@@ -1210,6 +1243,33 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
     planes[i].n[2] = pl_n[3 * i + 2];
     planes[i].d = pl_d[i];
   }
+  // bounds of every plane id's triangles, for the evaluation's path pre-check
+  std::vector<double> pb(6 * (size_t)std::max(n_tri, 1));
+  for (int i = 0; i < n_tri; i++) {
+    double *b = &pb[6 * (size_t)s->plane[i]];
+    if (s->plane[i] == i) {
+      for (int k = 0; k < 3; k++) {
+        b[k] = INFINITY;
+        b[3 + k] = -INFINITY;
+      }
+    }
+  }
+  for (int i = 0; i < n_tri; i++) {
+    double *b = &pb[6 * (size_t)s->plane[i]];
+    for (int k = 0; k < 3; k++) {
+      b[k] = std::min(b[k], lo[3 * i + k]);
+      b[3 + k] = std::max(b[3 + k], hi[3 * i + k]);
+    }
+  }
+  std::vector<AmsPlaneBox> pbox(std::max(n_tri, 1));
+  for (int i = 0; i < n_tri; i++) {
+    const double *b = &pb[6 * (size_t)i];
+    if (s->plane[i] != i) continue;
+    pbox[i].lo = make_float4(std::nextafter((float)b[0], -INFINITY), std::nextafter((float)b[1], -INFINITY),
+                             std::nextafter((float)b[2], -INFINITY), 0.0f);
+    pbox[i].hi = make_float4(std::nextafter((float)b[3], INFINITY), std::nextafter((float)b[4], INFINITY),
+                             std::nextafter((float)b[5], INFINITY), 0.0f);
+  }
   AmsBvhBuild B;
   ams_bvh_build(n_tri, lo.data(), hi.data(), AMS_LEAF_MAX, &B);
   std::vector<AmsTri> tris(n_tri > 0 ? n_tri : 1);
@@ -1240,6 +1300,7 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
       dev_upload(&s->d_tris, tris.data(), sizeof(AmsTri) * tris.size()) ||
       dev_upload(&s->d_aux, aux.data(), sizeof(AmsTriAux) * aux.size()) ||
       dev_upload(&s->d_planes, planes.data(), sizeof(AmsPlane) * planes.size()) ||
+      dev_upload(&s->d_pbox, pbox.data(), sizeof(AmsPlaneBox) * pbox.size()) ||
       dev_upload(&s->d_mat_prop, mp.data(), sizeof(float) * mp.size()) ||
       dev_upload(&s->d_refl_phase, rp.data(), sizeof(float) * rp.size()))
     return AMS_ERR_HIP;
@@ -1713,6 +1774,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ea.mat_prop = s->d_mat_prop;
     ea.refl_phase = s->d_refl_phase;
     ea.planes = s->d_planes;
+    ea.pbox = s->d_pbox;
     ea.src = d_src;
     ea.n_src = n_src;
     ea.dst = d_dst;

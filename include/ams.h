@@ -52,6 +52,8 @@ typedef struct {
                                     small enough to be staged in LDS (32 KB);
                              bit 5: find Rx sphere hits through the sphere hierarchy also when
                                     the Rx array is a regular lattice;
+                             bit 8: no analytic pre-check of backward chains (every chain is
+                                    traced leg by leg);
                              bit 7: keep tracing a branch with no reflection left even when it
                                     can no longer reach the Rx array */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
