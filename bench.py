@@ -31,8 +31,8 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # (calibrated against a 1 GiB device copy); the table and the commands that
 # produced it are in profiles/hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("room_boxes_ms", "k_discover"): {"bytes": 20.3e6 + 108.9e6, "l2_hit": 0.888},
-    ("office_batched", "k_cgemm"): {"bytes": 1547.4e6 + 134.2e6, "l2_hit": 0.782, "mfma_util": 0.590},
+    ("room_boxes_ms", "k_discover"): {"bytes": 32.4e6 + 137.5e6, "l2_hit": 0.947},
+    ("office_batched", "k_cgemm"): {"bytes": 1547.4e6 + 134.2e6, "l2_hit": 0.782, "mfma_util": 0.592},
 }
 
 
