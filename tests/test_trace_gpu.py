@@ -234,3 +234,31 @@ def test_rx_lattice_query_matches_hierarchy(small_room, trace_scene, jitter):
     assert ka == kb
     assert rel < 1e-6
     assert sa["fwd_nodes"] < sb["fwd_nodes"]
+
+
+@pytest.mark.parametrize("flags", [128, 256, 8])
+def test_culling_leaves_results_unchanged(small_room, trace_scene, flags):
+    """Early termination of branches that can no longer reach the Rx array
+    (disabled by flags bit 7) and the analytic pre-check of backward chains
+    (bit 8) only skip queries whose outcome is known: identical record sets,
+    success counts and H.  Keeping the scene hit across Rx hits (bit 3
+    re-queries) may differ only for Rx within 10 um of a wall: none here."""
+    import torch
+
+    from automs_amd.tracer import trace
+    w = small_room
+    _flush(f"case: culling flags={flags}")
+    out = {}
+    for f in (0, flags):
+        res, st, keys = trace(trace_scene, w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
+                              w.gridsize, flags=f, keys_cap=1 << 20)
+        torch.cuda.synchronize()
+        out[f] = (res.double().cpu().numpy(), _keyset(keys), st)
+    (Ha, ka, sa), (Hb, kb, sb) = out[0], out[flags]
+    rel = np.linalg.norm(Ha - Hb) / np.linalg.norm(Hb)
+    _flush(f"  keys {len(ka)} vs {len(kb)} successes {sa['successes']} vs {sb['successes']} queries "
+           f"{sa['fwd_queries'] + sa['bwd_queries']} vs {sb['fwd_queries'] + sb['bwd_queries']} H rel {rel:.1e}")
+    assert ka == kb
+    assert sa["successes"] == sb["successes"]
+    assert rel < 1e-6
+    assert sa["fwd_queries"] + sa["bwd_queries"] <= sb["fwd_queries"] + sb["bwd_queries"]
