@@ -1168,6 +1168,51 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
   }
 }
 
+// Path keys of small scenes packed losslessly into 64 bits: every field
+// keeps its order (plane ids < 2^pb - 1, the "no plane" value 0xFFFFF maps
+// to 2^pb - 1, tiles < 2^tb), so a single keys-only radix sort over
+// 8 + 5 pb + tb bits orders them as the 128-bit comparison does.
+struct KeyBits {
+  int pb, tb;  // bits per plane id, bits of the tile index; pb = 0: no packing
+};
+__device__ __forceinline__ uint64_t key_compact(uint64_t hi, uint64_t lo, KeyBits kb) {
+  const uint32_t none = (1u << kb.pb) - 1u;
+  uint64_t c = (hi >> 56);  // h:3 | b:5
+  for (int k = 0; k < AMS_MAX_HIT; k++) {
+    uint32_t p = key_get_plane(hi, lo, k);
+    c = (c << kb.pb) | (p == 0xFFFFFu ? none : p);
+  }
+  return (c << kb.tb) | (lo & 0xFFFFFull);
+}
+__global__ void k_key_compact(const uint64_t *__restrict__ hi, const uint64_t *__restrict__ lo, size_t n, KeyBits kb,
+                              uint64_t *__restrict__ out) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = key_compact(hi[k], lo[k], kb);
+}
+__global__ void k_key_expand(const uint64_t *__restrict__ c, size_t n, KeyBits kb, uint64_t *__restrict__ hi,
+                             uint64_t *__restrict__ lo) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint64_t v = c[k];
+  const uint32_t tile = (uint32_t)(v & ((1ull << kb.tb) - 1ull));
+  v >>= kb.tb;
+  const uint32_t none = (1u << kb.pb) - 1u;
+  uint32_t pl[AMS_MAX_HIT];
+  for (int q = AMS_MAX_HIT - 1; q >= 0; q--) {
+    uint32_t p = (uint32_t)(v & none);
+    pl[q] = p == none ? 0xFFFFFu : p;
+    v >>= kb.pb;
+  }
+  uint64_t h, l;
+  ams_key_pack((int)(v >> 5), (int)(v & 31), pl, tile, &h, &l);
+  hi[k] = h;
+  lo[k] = l;
+}
+__global__ void k_unique_flags64(const uint64_t *__restrict__ c, size_t n, unsigned char *__restrict__ flag) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) flag[k] = (k == 0) || c[k] != c[k - 1];
+}
+
 __global__ void k_add_h(const double *__restrict__ H64, float *__restrict__ out, size_t n) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] += (float)H64[k];
@@ -1308,9 +1353,37 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
 }
 
 // (hi, lo) sort + unique in place on device; returns the unique count.
-static int sort_unique(Scratch &sc, uint64_t *hi, uint64_t *lo, size_t n, hipStream_t st, size_t *n_out) {
+static int sort_unique(Scratch &sc, uint64_t *hi, uint64_t *lo, size_t n, hipStream_t st, size_t *n_out,
+                       KeyBits kb = KeyBits{0, 0}) {
   if (n == 0) {
     *n_out = 0;
+    return AMS_OK;
+  }
+  if (kb.pb > 0) {
+    // packed keys: one keys-only sort over the used bits, unique, unpack
+    const int bits = 8 + AMS_MAX_HIT * kb.pb + kb.tb;
+    uint64_t *c1 = (uint64_t *)sc.get("su_c1", n * 8), *c2 = (uint64_t *)sc.get("su_c2", n * 8);
+    unsigned char *flag = (unsigned char *)sc.get("su_flag", n);
+    size_t *cnt = (size_t *)sc.get("su_cnt", 16);
+    if (!c1 || !c2 || !flag || !cnt) return AMS_ERR_ALLOC;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    k_key_compact<<<g, 256, 0, st>>>(hi, lo, n, kb, c1);
+    HIPCHK(hipGetLastError());
+    size_t tb = 0, tb3 = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, c1, c2, (int)n, 0, bits, st));
+    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tb3, c2, flag, c1, cnt, (int)n, st));
+    void *tmp = sc.get("su_tmp", std::max(tb, tb3));
+    if (!tmp) return AMS_ERR_ALLOC;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp, tb, c1, c2, (int)n, 0, bits, st));
+    k_unique_flags64<<<g, 256, 0, st>>>(c2, n, flag);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipcub::DeviceSelect::Flagged(tmp, tb3, c2, flag, c1, cnt, (int)n, st));
+    size_t m = 0;
+    HIPCHK(hipMemcpyAsync(&m, cnt, sizeof(m), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (m) k_key_expand<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(c1, m, kb, hi, lo);
+    HIPCHK(hipGetLastError());
+    *n_out = m;
     return AMS_OK;
   }
   uint64_t *hi2 = (uint64_t *)sc.get("su_hi2", n * 8), *lo2 = (uint64_t *)sc.get("su_lo2", n * 8);
@@ -1449,6 +1522,15 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     d_sph_o = (float4 *)sc.get("sph_o", sizeof(float4) * n_dst);
     if (!d_sph_o) return AMS_ERR_ALLOC;
     HIPCHK(hipMemcpyAsync(d_sph_o, sph_o.data(), sizeof(float4) * n_dst, hipMemcpyHostToDevice, st));
+  }
+
+  // key packing for the record sorts: plane ids and tiles of small scenes fit in 64 bits
+  KeyBits kbits{0, 0};
+  {
+    int pb = 1, tb = 1;
+    while (((1ll << pb) - 1) < (long long)std::max(1, s->n_tri) + 1) pb++;
+    while ((1ll << tb) < n_grid) tb++;
+    if (8 + AMS_MAX_HIT * pb + tb <= 64 && !(o->flags & 512)) kbits = KeyBits{pb, tb};
   }
 
   // ---------------- forward discovery in chunks of rays -----------------
@@ -1594,7 +1676,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
       return AMS_ERR_OVERFLOW;
     }
     size_t nu = 0;
-    int rc = sort_unique(sc, r_hi, r_lo, (size_t)hcnt, st, &nu);
+    int rc = sort_unique(sc, r_hi, r_lo, (size_t)hcnt, st, &nu, kbits);
     if (rc) return rc;
     S.records_raw += (long long)hcnt;
     if (nu > acc_cap) {
@@ -1649,13 +1731,13 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
       continue;  // redo [rb, rb + cur_chunk)
     }
     size_t nu = 0;
-    int rc = sort_unique(sc, c_hi, c_lo, (size_t)hcnt, st, &nu);
+    int rc = sort_unique(sc, c_hi, c_lo, (size_t)hcnt, st, &nu, kbits);
     if (rc) return rc;
     S.records_raw += (long long)hcnt;
     if (acc_n + nu > acc_cap) {
       // compact the running set, then grow its buffers if still needed
       size_t m = 0;
-      rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m);
+      rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m, kbits);
       if (rc) return rc;
       acc_n = m;
       if (acc_n + nu > acc_cap) {
@@ -1702,7 +1784,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     acc_lo.swap(l2);
   } else {
     size_t m = 0;
-    int rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m);
+    int rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m, kbits);
     if (rc) return rc;
     acc_hi.resize(m);
     acc_lo.resize(m);

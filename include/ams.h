@@ -54,6 +54,7 @@ typedef struct {
                                     the Rx array is a regular lattice;
                              bit 8: no analytic pre-check of backward chains (every chain is
                                     traced leg by leg);
+                             bit 9: sort record keys at 128 bits even when they pack into 64;
                              bit 7: keep tracing a branch with no reflection left even when it
                                     can no longer reach the Rx array */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
