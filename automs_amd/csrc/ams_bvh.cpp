@@ -243,3 +243,102 @@ void ams_plane_ids(int n_tri, const float *tri, int *plane_out, double *pl_n_out
     }
   }
 }
+
+namespace {
+double node_area(const AmsNode &n) {
+  double dx = (double)n.hi[0] - n.lo[0], dy = (double)n.hi[1] - n.lo[1], dz = (double)n.hi[2] - n.lo[2];
+  if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+  return dx * dy + dy * dz + dz * dx;
+}
+}  // namespace
+
+int ams_bvh_collapse4(const AmsBvhBuild &b, std::vector<AmsNode4> *out) {
+  const std::vector<AmsNode> &N = b.nodes;
+  out->clear();
+  const int nn = (int)N.size();
+  auto leaf_ref = [&](int i) { return -1 - N[i].info; };
+  auto kids = [&](int i, int *l, int *r) {
+    *l = i + 1;
+    *r = N[i + 1].miss;
+  };
+  int depth_max = 0;
+  // explicit work list: (binary inner node, four-wide slot, depth)
+  struct Job {
+    int bin, slot, depth;
+  };
+  std::vector<Job> work;
+  out->emplace_back();
+  if (nn <= 1 || N[0].info != 0) {
+    // a single leaf, or no triangles (no nodes, or the one placeholder node of
+    // an empty scene): one node with at most one child
+    AmsNode4 &r = out->back();
+    float *lo[3] = {&r.lox.x, &r.loy.x, &r.loz.x}, *hi[3] = {&r.hix.x, &r.hiy.x, &r.hiz.x};
+    for (int c = 0; c < 4; c++)
+      for (int k = 0; k < 3; k++) {
+        lo[k][c] = 0.0f;
+        hi[k][c] = 0.0f;
+      }
+    r.child = make_int4(AMS_NODE4_EMPTY, AMS_NODE4_EMPTY, AMS_NODE4_EMPTY, AMS_NODE4_EMPTY);
+    r.pad = make_int4(0, 0, 0, 0);
+    if (nn > 0 && N[0].info != 0) {
+      for (int k = 0; k < 3; k++) {
+        lo[k][0] = N[0].lo[k];
+        hi[k][0] = N[0].hi[k];
+      }
+      r.child.x = leaf_ref(0);
+    }
+    return 1;
+  }
+  work.push_back({0, 0, 1});
+  while (!work.empty()) {
+    Job j = work.back();
+    work.pop_back();
+    depth_max = std::max(depth_max, j.depth);
+    int c[4], m = 0;
+    kids(j.bin, &c[0], &c[1]);
+    m = 2;
+    while (m < 4) {  // open the largest inner child
+      int best = -1;
+      double ba = -1.0;
+      for (int k = 0; k < m; k++)
+        if (N[c[k]].info == 0 && node_area(N[c[k]]) > ba) {
+          ba = node_area(N[c[k]]);
+          best = k;
+        }
+      if (best < 0) break;
+      int l, r;
+      kids(c[best], &l, &r);
+      c[best] = l;
+      c[m++] = r;
+    }
+    AmsNode4 node;
+    float *lo[3] = {&node.lox.x, &node.loy.x, &node.loz.x}, *hi[3] = {&node.hix.x, &node.hiy.x, &node.hiz.x};
+    int ref[4] = {AMS_NODE4_EMPTY, AMS_NODE4_EMPTY, AMS_NODE4_EMPTY, AMS_NODE4_EMPTY};
+    for (int k = 0; k < 4; k++) {
+      if (k < m) {
+        for (int a = 0; a < 3; a++) {
+          lo[a][k] = N[c[k]].lo[a];
+          hi[a][k] = N[c[k]].hi[a];
+        }
+      } else {
+        for (int a = 0; a < 3; a++) {
+          lo[a][k] = 0.0f;
+          hi[a][k] = 0.0f;
+        }
+      }
+    }
+    for (int k = 0; k < m; k++) {
+      if (N[c[k]].info != 0) {
+        ref[k] = leaf_ref(c[k]);
+      } else {
+        ref[k] = (int)out->size();
+        out->emplace_back();
+        work.push_back({c[k], ref[k], j.depth + 1});
+      }
+    }
+    node.child = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    node.pad = make_int4(0, 0, 0, 0);
+    (*out)[j.slot] = node;
+  }
+  return depth_max;
+}

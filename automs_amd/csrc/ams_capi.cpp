@@ -100,6 +100,7 @@ int ams_scene_destroy(ams_scene *s) {
   ams::dfree(s->d_aux);
   ams::dfree(s->d_planes);
   ams::dfree(s->d_pbox);
+  ams::dfree(s->d_tnodes4);
   ams::dfree(s->d_mat_prop);
   ams::dfree(s->d_refl_phase);
   delete s;
@@ -184,6 +185,26 @@ int ams_host_bvh_build(const double *lo, const double *hi, int32_t n, int32_t le
     return AMS_ERR_ARG;
   }
   std::memcpy(nodes_out, B.nodes.data(), sizeof(AmsNode) * B.nodes.size());
+  for (int i = 0; i < n; i++) order_out[i] = B.order[i];
+  return AMS_OK;
+}
+
+int ams_host_bvh4_build(const double *lo, const double *hi, int32_t n, int32_t leaf_max, void *nodes4_out,
+                        int32_t nodes4_cap, int32_t *order_out, int32_t *n_nodes4_out) {
+  if ((n > 0 && (!lo || !hi)) || n < 0 || !nodes4_out || !order_out || !n_nodes4_out) {
+    ams::set_error("ams_host_bvh4_build: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  AmsBvhBuild B;
+  if (n > 0) ams_bvh_build(n, lo, hi, leaf_max, &B);
+  std::vector<AmsNode4> N4;
+  ams_bvh_collapse4(B, &N4);
+  *n_nodes4_out = (int32_t)N4.size();
+  if ((int32_t)N4.size() > nodes4_cap) {
+    ams::set_error("ams_host_bvh4_build: nodes4_cap too small");
+    return AMS_ERR_ARG;
+  }
+  std::memcpy(nodes4_out, N4.data(), sizeof(AmsNode4) * N4.size());
   for (int i = 0; i < n; i++) order_out[i] = B.order[i];
   return AMS_OK;
 }

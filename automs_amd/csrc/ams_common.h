@@ -10,6 +10,7 @@
 #define AMS_TMIN 1e-5f         // optixPathTracer.cu:98 (tmin)
 #define AMS_ACC_MIN 1e-4f      // floor of the Tx/Rx acceptance radius (m)
 #define AMS_LEAF_MAX 4
+#define AMS_NODE4_EMPTY ((int)0x80000000)
 
 // BVH node, 32 bytes, depth-first preorder.  A hit continues at i + 1 for an
 // inner node; a miss (or a finished leaf) continues at `miss`.  `info` is 0
@@ -19,6 +20,16 @@ struct __align__(16) AmsNode {
   int32_t miss;
   float hi[3];
   int32_t info;
+};
+
+// Four-wide node, 128 bytes, collapsed from the binary hierarchy: the boxes
+// of up to four children in structure-of-arrays form and their references
+// (>= 0: a four-wide node; < 0: a leaf, -1 - ((first << 3) | count) into the
+// leaf-ordered triangles; AMS_NODE4_EMPTY: an unused slot, never entered).
+struct __align__(16) AmsNode4 {
+  float4 lox, loy, loz, hix, hiy, hiz;
+  int4 child;
+  int4 pad;
 };
 
 // Triangle record, 48 bytes, stored in BVH leaf order.

@@ -66,6 +66,8 @@ struct ams_scene {
   std::vector<int> plane;
   AmsNode *d_tnodes = nullptr;
   int n_tnodes = 0;
+  AmsNode4 *d_tnodes4 = nullptr;  // four-wide form of the same hierarchy
+  int n_tnodes4 = 0, depth4 = 0;
   AmsTri *d_tris = nullptr;
   AmsTriAux *d_aux = nullptr;
   AmsPlane *d_planes = nullptr;

@@ -154,6 +154,120 @@ __device__ __forceinline__ int closest_tri(const AmsNode *__restrict__ nodes, in
   return hit;
 }
 
+// Entry distance of the ray into a box within [tmin, tmax] (INFINITY when
+// missed); the same conservative test as box_test.
+__device__ __forceinline__ float box_entry(const Ray &r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                           float tmin, float tmax) {
+  float tx0 = (lx - r.o.x) * r.inv.x, tx1 = (hx - r.o.x) * r.inv.x;
+  float ty0 = (ly - r.o.y) * r.inv.y, ty1 = (hy - r.o.y) * r.inv.y;
+  float tz0 = (lz - r.o.z) * r.inv.z, tz1 = (hz - r.o.z) * r.inv.z;
+  float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+  float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+  return t0 <= t1 * 1.0000004f ? t0 : INFINITY;
+}
+__device__ __forceinline__ void entry_swap(float &ea, int &ca, float &eb, int &cb) {
+  if (eb < ea) {
+    float e = ea;
+    ea = eb;
+    eb = e;
+    int c = ca;
+    ca = cb;
+    cb = c;
+  }
+}
+
+// Closest scene triangle through the four-wide hierarchy with a short
+// register stack: a step tests the four child boxes of a node and goes on
+// at the nearest one hit, the others wait on the stack.  A chain of node
+// loads is three to four times shorter than the binary walk's.  Same
+// triangle tests and tie rule as closest_tri, so the same hit; on a stack
+// overflow (never seen on the scenes here) `overflow` is set and the caller
+// repeats the query with closest_tri.
+constexpr int kStack4 = 12;
+__device__ __forceinline__ int closest_tri4(const AmsNode4 *__restrict__ nodes, const AmsTri *__restrict__ tris,
+                                            const Ray &r, float tmin, float &tbest, int skip_pl, int &orig_out,
+                                            Counters &ct, bool &overflow) {
+  int hit = -1, horig = 0x7fffffff;
+  int stk[kStack4];
+  int sp = 0;
+  int cur = 0;
+  overflow = false;
+  while (true) {
+    int next = AMS_NODE4_EMPTY;
+    if (cur >= 0) {
+      const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
+      const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+      const int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+      ct.nodes += 4;  // counted in 32-byte boxes, as the binary walk counts its nodes
+      float e0 = ch.x != AMS_NODE4_EMPTY ? box_entry(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, tbest) : INFINITY;
+      float e1 = ch.y != AMS_NODE4_EMPTY ? box_entry(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, tbest) : INFINITY;
+      float e2 = ch.z != AMS_NODE4_EMPTY ? box_entry(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, tbest) : INFINITY;
+      float e3 = ch.w != AMS_NODE4_EMPTY ? box_entry(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tmin, tbest) : INFINITY;
+      int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+      entry_swap(e0, c0, e1, c1);
+      entry_swap(e2, c2, e3, c3);
+      entry_swap(e0, c0, e2, c2);
+      entry_swap(e1, c1, e3, c3);
+      entry_swap(e1, c1, e2, c2);
+      if (e3 < INFINITY) {
+        if (sp < kStack4) stk[sp++] = c3;
+        else overflow = true;
+      }
+      if (e2 < INFINITY) {
+        if (sp < kStack4) stk[sp++] = c2;
+        else overflow = true;
+      }
+      if (e1 < INFINITY) {
+        if (sp < kStack4) stk[sp++] = c1;
+        else overflow = true;
+      }
+      if (e0 < INFINITY) next = c0;
+    } else {
+      const int code = -1 - cur;
+      const int first = code >> 3, cnt = code & 7;
+      for (int k = 0; k < cnt; k++) {
+        const AmsTri tr = tris[first + k];
+        ct.tris++;
+        if (__float_as_int(tr.b.w) == skip_pl) continue;
+        float t;
+        if (tri_test(r, tr.a, tr.b, tr.c, tmin, tbest, t)) {
+          int orig = __float_as_int(tr.a.w);
+          if (t < tbest || orig < horig) {
+            tbest = t;
+            hit = first + k;
+            horig = orig;
+          }
+        }
+      }
+    }
+    if (overflow) break;
+    if (next != AMS_NODE4_EMPTY) {
+      cur = next;
+    } else {
+      if (sp == 0) break;
+      cur = stk[--sp];
+    }
+  }
+  orig_out = horig;
+  return hit;
+}
+
+// The scene query of the kernels: four-wide walk when its hierarchy is given
+// (nodes4 != nullptr), the binary threaded walk otherwise or after a stack
+// overflow.
+__device__ __forceinline__ int scene_closest(const AmsNode4 *nodes4, const AmsNode *nodes, int n_nodes,
+                                             const AmsTri *tris, const Ray &r, float tmin, float &tbest, int skip_pl,
+                                             int &orig_out, Counters &ct) {
+  if (nodes4) {
+    const float t0 = tbest;
+    bool overflow;
+    int h = closest_tri4(nodes4, tris, r, tmin, tbest, skip_pl, orig_out, ct, overflow);
+    if (!overflow) return h;
+    tbest = t0;
+  }
+  return closest_tri(nodes, n_nodes, tris, r, tmin, tbest, skip_pl, orig_out, ct);
+}
+
 // One reception sphere (centre, original index in .w): accepted when the
 // miss distance is below the acceptance radius, the closest approach b lies
 // in [tmin, tbest] and the origin is outside the sphere
@@ -469,14 +583,14 @@ __global__ void k_fib_keys(long long n, unsigned *__restrict__ keys, unsigned *_
 // instances): the traversal is a chain of dependent node loads, and from LDS
 // each costs a fraction of an L2 hit.  Larger scenes are read from L2/HBM.
 constexpr int kSceneLds = 32 * 1024;
-static_assert(sizeof(AmsNode) == 2 * sizeof(float4) && sizeof(AmsTri) == 3 * sizeof(float4),
+static_assert(sizeof(AmsNode) == 2 * sizeof(float4) && sizeof(AmsNode4) == 8 * sizeof(float4) &&
+                  sizeof(AmsTri) == 3 * sizeof(float4),
               "stage_scene copies nodes and triangles as float4");
 
-__device__ __forceinline__ void stage_scene(const AmsNode *__restrict__ nodes, int n_nodes,
-                                            const AmsTri *__restrict__ tris, int n_tris, float4 *smem) {
-  const float4 *sn = reinterpret_cast<const float4 *>(nodes);
+__device__ __forceinline__ void stage_scene(const float4 *__restrict__ sn, int n4, const AmsTri *__restrict__ tris,
+                                            int n_tris, float4 *smem) {
   const float4 *st = reinterpret_cast<const float4 *>(tris);
-  const int n4 = 2 * n_nodes, m4 = 3 * n_tris;
+  const int m4 = 3 * n_tris;
   for (int k = threadIdx.x; k < n4; k += blockDim.x) smem[k] = sn[k];
   for (int k = threadIdx.x; k < m4; k += blockDim.x) smem[n4 + k] = st[k];
   __syncthreads();
@@ -485,6 +599,8 @@ __device__ __forceinline__ void stage_scene(const AmsNode *__restrict__ nodes, i
 struct DiscoverArgs {
   const AmsNode *tnodes;
   int n_tnodes;
+  const AmsNode4 *tnodes4;  // four-wide hierarchy, or nullptr for the binary walk
+  int n_tnodes4;
   const AmsTri *tris;
   int n_tris;
   const AmsTriAux *aux;
@@ -541,11 +657,16 @@ template <bool LDS>
 __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
   extern __shared__ float4 ams_smem[];
   const AmsNode *tnodes = a.tnodes;
+  const AmsNode4 *tnodes4 = a.tnodes4;
   const AmsTri *tris = a.tris;
   if (LDS) {
-    stage_scene(a.tnodes, a.n_tnodes, a.tris, a.n_tris, ams_smem);
-    tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
-    tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
+    // the hierarchy the queries walk (four-wide when given) and the triangles
+    const int n4 = tnodes4 ? 8 * a.n_tnodes4 : 2 * a.n_tnodes;
+    stage_scene(tnodes4 ? reinterpret_cast<const float4 *>(a.tnodes4) : reinterpret_cast<const float4 *>(a.tnodes), n4,
+                a.tris, a.n_tris, ams_smem);
+    if (tnodes4) tnodes4 = reinterpret_cast<const AmsNode4 *>(ams_smem);
+    else tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
+    tris = reinterpret_cast<const AmsTri *>(ams_smem + n4);
   }
   __shared__ DedupTable dd;
   dedup_init(dd);
@@ -596,7 +717,7 @@ __global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
           torig = tri_orig;
         } else {
           tt = INFINITY;
-          slot = closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+          slot = scene_closest(tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
         }
         float ts = tt;
         int rx = rx_query(a, ray, ts, ct);
@@ -828,11 +949,16 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
   const DiscoverArgs &a = ra.d;
   extern __shared__ float4 ams_smem[];
   const AmsNode *tnodes = a.tnodes;
+  const AmsNode4 *tnodes4 = a.tnodes4;
   const AmsTri *tris = a.tris;
   if (LDS) {
-    stage_scene(a.tnodes, a.n_tnodes, a.tris, a.n_tris, ams_smem);
-    tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
-    tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
+    // the hierarchy the queries walk (four-wide when given) and the triangles
+    const int n4 = tnodes4 ? 8 * a.n_tnodes4 : 2 * a.n_tnodes;
+    stage_scene(tnodes4 ? reinterpret_cast<const float4 *>(a.tnodes4) : reinterpret_cast<const float4 *>(a.tnodes), n4,
+                a.tris, a.n_tris, ams_smem);
+    if (tnodes4) tnodes4 = reinterpret_cast<const AmsNode4 *>(ams_smem);
+    else tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
+    tris = reinterpret_cast<const AmsTri *>(ams_smem + n4);
   }
   __shared__ DedupTable dd;
   dedup_init(dd);
@@ -879,7 +1005,7 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
             torig = tri_orig;
           } else {
             tt = INFINITY;
-            sl = closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+            sl = scene_closest(tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
           }
           float ts = tt;
           int rx = rx_query(a, ray, ts, ct);
@@ -984,10 +1110,12 @@ struct EvalArgs {
 template <bool LDS>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) k_evaluate(EvalArgs a) {
   extern __shared__ float4 ams_smem[];
+  // Backward legs are short bounded segments: the binary walk, whose loop
+  // fits the 80 registers of this kernel (the four-wide one spills here).
   const AmsNode *tnodes = a.tnodes;
   const AmsTri *tris = a.tris;
   if (LDS) {
-    stage_scene(a.tnodes, a.n_tnodes, a.tris, a.n_tris, ams_smem);
+    stage_scene(reinterpret_cast<const float4 *>(a.tnodes), 2 * a.n_tnodes, a.tris, a.n_tris, ams_smem);
     tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
     tris = reinterpret_cast<const AmsTri *>(ams_smem + 2 * a.n_tnodes);
   }
@@ -1338,10 +1466,14 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
     B.nodes.push_back(n);
   }
   s->n_tnodes = (int)B.nodes.size();
+  std::vector<AmsNode4> N4;
+  s->depth4 = ams_bvh_collapse4(B, &N4);
+  s->n_tnodes4 = (int)N4.size();
   std::vector<float> mp(19 * (size_t)std::max(n_mat, 1), 0.0f), rp(std::max(n_mat, 1), 0.0f);
   for (int k = 0; k < 19 * n_mat; k++) mp[k] = mat_prop[k];
   for (int k = 0; k < n_mat; k++) rp[k] = refl_phase ? refl_phase[k] : 0.0f;
   if (dev_upload(&s->d_tnodes, B.nodes.data(), sizeof(AmsNode) * B.nodes.size()) ||
+      dev_upload(&s->d_tnodes4, N4.data(), sizeof(AmsNode4) * N4.size()) ||
       dev_upload(&s->d_tris, tris.data(), sizeof(AmsTri) * tris.size()) ||
       dev_upload(&s->d_aux, aux.data(), sizeof(AmsTriAux) * aux.size()) ||
       dev_upload(&s->d_planes, planes.data(), sizeof(AmsPlane) * planes.size()) ||
@@ -1553,10 +1685,20 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   DiscoverArgs da;
   da.tnodes = s->d_tnodes;
   da.n_tnodes = s->n_tnodes;
+  // Scene hierarchy of the discovery walk.  Scenes that fit in LDS (flags
+  // bit 4 keeps them in global memory) walk the binary hierarchy there: its
+  // node loads are cheap and its loop is the shorter one.  Larger scenes walk
+  // the four-wide hierarchy, a quarter as many dependent L2 loads.  Bit 10
+  // forces the binary walk, bit 11 the four-wide one.
+  const size_t eval_scene_lds = (size_t)s->n_tnodes * sizeof(AmsNode) + (size_t)s->n_tri * sizeof(AmsTri);
+  const bool eval_lds = eval_scene_lds <= (size_t)kSceneLds && !(o->flags & 16);
+  const bool wide = s->d_tnodes4 && !(o->flags & 1024) && ((o->flags & 2048) || !eval_lds);
+  da.tnodes4 = wide ? s->d_tnodes4 : nullptr;
+  da.n_tnodes4 = wide ? s->n_tnodes4 : 0;
   da.tris = s->d_tris;
   da.n_tris = s->n_tri;
-  // LDS-staged scene for small scenes (flags bit 4 keeps it in global memory)
-  const size_t scene_lds = (size_t)s->n_tnodes * sizeof(AmsNode) + (size_t)s->n_tri * sizeof(AmsTri);
+  const size_t scene_lds = (wide ? (size_t)s->n_tnodes4 * sizeof(AmsNode4) : (size_t)s->n_tnodes * sizeof(AmsNode)) +
+                           (size_t)s->n_tri * sizeof(AmsTri);
   const bool use_lds = scene_lds <= (size_t)kSceneLds && !(o->flags & 16);
   const unsigned lds_bytes = use_lds ? (unsigned)scene_lds : 0u;
   da.aux = s->d_aux;
@@ -1922,7 +2064,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         eb.unit_tile += u0;
         eb.unit_begin += u0;
         eb.unit_end += u0;
-        if (use_lds) k_evaluate<true><<<dim3(bpt, un), block, lds_bytes, st>>>(eb);
+        if (eval_lds) k_evaluate<true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
         else k_evaluate<false><<<dim3(bpt, un), block, 0, st>>>(eb);
         HIPCHK(hipGetLastError());
         S.n_evaluate_launches++;

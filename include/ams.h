@@ -55,6 +55,10 @@ typedef struct {
                              bit 8: no analytic pre-check of backward chains (every chain is
                                     traced leg by leg);
                              bit 9: sort record keys at 128 bits even when they pack into 64;
+                             bits 10, 11: forward queries walk the binary threaded hierarchy
+                                     (bit 10) or the four-wide one (bit 11); by default the
+                                     binary one for scenes staged in LDS and the four-wide one
+                                     otherwise (same hits either way);
                              bit 7: keep tracing a branch with no reflection left even when it
                                     can no longer reach the Rx array */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
@@ -71,6 +75,7 @@ typedef struct {
   int32_t launch_x, launch_y, samples_per_launch, reserve_size;
 } ams_trace_opts;
 
+/* *_nodes count 32-byte node boxes tested (a four-wide node is four) */
 typedef struct {
   int64_t rays, records_raw, records_unique, paths, tasks, rounds, chains, successes, overflow;
   int64_t fwd_queries, fwd_nodes, fwd_tris, fwd_spheres;
@@ -151,6 +156,13 @@ int ams_to_bf16(const float *x, void *y, int64_t n, void *stream);
  * nodes {lo[3] f32, miss i32, hi[3] f32, info i32}, order_out the leaf order */
 int ams_host_bvh_build(const double *lo, const double *hi, int32_t n, int32_t leaf_max, void *nodes_out,
                        int32_t nodes_cap, int32_t *order_out, int32_t *n_nodes_out);
+/* the four-wide hierarchy the kernels walk, collapsed from the same binary
+ * tree: nodes4_out gets 128-byte nodes {lo x[4], lo y[4], lo z[4], hi x[4],
+ * hi y[4], hi z[4] f32, child[4] i32, pad[4]}; child >= 0 is a node, < 0 a
+ * leaf -1 - ((first << 3) | count) into order_out, INT32_MIN an unused slot.
+ * No counterpart in the reference (its hierarchy is built inside OptiX). */
+int ams_host_bvh4_build(const double *lo, const double *hi, int32_t n, int32_t leaf_max, void *nodes4_out,
+                        int32_t nodes4_cap, int32_t *order_out, int32_t *n_nodes4_out);
 /* plane id of every triangle as ams_scene_plane_ids reports it */
 int ams_host_plane_ids(const float *tri, int32_t n_tri, int32_t *plane_out);
 

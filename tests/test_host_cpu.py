@@ -106,6 +106,117 @@ def test_threaded_bvh_closest_hit_equals_brute_force():
         assert bi == bbi
 
 
+def _bvh4(lo, hi, leaf_max=4):
+    from automs_amd import _lib
+    L = _lib.lib()
+    n = lo.shape[0]
+    nodes = np.zeros((max(n, 1) + 1, 32), np.float32)
+    order = np.zeros(max(n, 1), np.int32)
+    nn = C.c_int32()
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    _lib.check(L.ams_host_bvh4_build(C.c_void_p(lo.ctypes.data), C.c_void_p(hi.ctypes.data), n, leaf_max,
+                                     C.c_void_p(nodes.ctypes.data), nodes.shape[0],
+                                     C.c_void_p(order.ctypes.data), C.c_void_p(C.addressof(nn))), "bvh4")
+    nodes = nodes[: nn.value]
+    blo = nodes[:, 0:12].reshape(-1, 3, 4).transpose(0, 2, 1)  # [node][child][axis]
+    bhi = nodes[:, 12:24].reshape(-1, 3, 4).transpose(0, 2, 1)
+    child = nodes.view(np.int32)[:, 24:28]
+    return blo, bhi, child, order
+
+
+EMPTY = -(1 << 31)
+
+
+def _walk4(blo, bhi, child, order, tri, o, d, stack=12):
+    """The kernels' four-wide walk: children sorted by entry distance, the
+    nearest continued, the others pushed; leaves tested when reached."""
+    inv = 1 / np.where(d == 0, 1e-30, d)
+    best, bi, visits = np.inf, -1, 0
+    stk, cur, depth = [], 0, 0
+    while True:
+        nxt = None
+        if cur >= 0:
+            visits += 1
+            ent = []
+            for c in range(4):
+                if child[cur, c] == EMPTY:
+                    continue
+                t0 = np.minimum((blo[cur, c] - o) * inv, (bhi[cur, c] - o) * inv).max()
+                t1 = np.maximum((blo[cur, c] - o) * inv, (bhi[cur, c] - o) * inv).min()
+                t0 = max(t0, 1e-5)
+                if t0 <= min(t1, best) * (1 + 4e-7):
+                    ent.append((t0, child[cur, c]))
+            ent.sort(key=lambda e: e[0])
+            for _, c in reversed(ent[1:]):
+                stk.append(c)
+            depth = max(depth, len(stk))
+            if ent:
+                nxt = ent[0][1]
+        else:
+            code = -1 - cur
+            for p in order[(code >> 3):(code >> 3) + (code & 7)]:
+                t = _mt(o, d, *tri[p])
+                if t is not None and 1e-5 <= t < best:
+                    best, bi = t, p
+        if nxt is not None:
+            cur = nxt
+        elif stk:
+            cur = stk.pop()
+        else:
+            break
+    assert depth <= stack
+    return bi, visits
+
+
+@pytest.mark.parametrize("T", [0, 1, 3, 200, 1500])
+def test_four_wide_bvh_structure_and_closest_hit(T):
+    rng = np.random.default_rng(7 + T)
+    tri = (rng.uniform(-4, 4, (T, 1, 3)) + rng.normal(0, 0.5, (T, 3, 3)))
+    lo, hi = (tri.min(1), tri.max(1)) if T else (np.zeros((0, 3)), np.zeros((0, 3)))
+    blo, bhi, child, order = _bvh4(lo, hi)
+    n4 = child.shape[0]
+    assert n4 >= 1
+    # a tree rooted at 0: every node but the root referenced exactly once
+    inner = child[child >= 0]
+    assert ((inner > 0) & (inner < n4)).all()
+    assert sorted(inner.tolist()) == list(range(1, n4))
+    # leaves cover every primitive exactly once, within their boxes; child
+    # boxes of an inner child lie inside that child's box
+    seen = []
+    for i in range(n4):
+        for c in range(4):
+            r = child[i, c]
+            if r == EMPTY:
+                continue
+            if r < 0:
+                code = -1 - r
+                first, cnt = code >> 3, code & 7
+                assert 1 <= cnt <= 4
+                for p in order[first:first + cnt]:
+                    seen.append(p)
+                    assert (blo[i, c] <= lo[p] + 1e-6).all() and (bhi[i, c] >= hi[p] - 1e-6).all()
+            else:
+                for c2 in range(4):
+                    if child[r, c2] != EMPTY:
+                        assert (blo[i, c] <= blo[r, c2]).all() and (bhi[i, c] >= bhi[r, c2]).all()
+    assert sorted(seen) == list(range(T))
+    if T == 0:
+        assert (child == EMPTY).all()
+        return
+    for _ in range(100):
+        o = rng.uniform(-5, 5, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        bi, _ = _walk4(blo, bhi, child, order, tri, o, d)
+        bbest, bbi = np.inf, -1
+        for p in range(T):
+            t = _mt(o, d, *tri[p])
+            if t is not None and 1e-5 <= t < bbest:
+                bbest, bbi = t, p
+        assert bi == bbi
+
+
 def test_plane_ids_match_oracle():
     from automs_amd import _lib, scenes
     from oracle import oracle as O

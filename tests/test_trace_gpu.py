@@ -262,3 +262,32 @@ def test_culling_leaves_results_unchanged(small_room, trace_scene, flags):
     assert sa["successes"] == sb["successes"]
     assert rel < 1e-6
     assert sa["fwd_queries"] + sa["bwd_queries"] <= sb["fwd_queries"] + sb["bwd_queries"]
+
+
+@pytest.mark.parametrize("base", [2048, 16])
+def test_four_wide_walk_matches_binary(small_room, trace_scene, base):
+    """The four-wide hierarchy walk returns the binary walk's hits (flags bit
+    10 forces the binary walk), with the scene staged in LDS (bit 11 forces
+    the four-wide walk there) and read from global memory (bit 4, where it is
+    the default): identical record sets and successes, the same H, and fewer
+    node boxes tested."""
+    import torch
+
+    from automs_amd.tracer import trace
+    w = small_room
+    _flush(f"case: four-wide vs binary walk base flags={base}")
+    out = {}
+    binary = (base & ~2048) | 1024
+    for f in (base, binary):
+        res, st, keys = trace(trace_scene, w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
+                              w.gridsize, flags=f, keys_cap=1 << 20)
+        torch.cuda.synchronize()
+        out[f] = (res.double().cpu().numpy(), _keyset(keys), st)
+    (Ha, ka, sa), (Hb, kb, sb) = out[base], out[binary]
+    rel = np.linalg.norm(Ha - Hb) / np.linalg.norm(Hb)
+    _flush(f"  keys {len(ka)} vs {len(kb)} successes {sa['successes']} vs {sb['successes']} nodes "
+           f"{sa['fwd_nodes']} vs {sb['fwd_nodes']} H rel {rel:.1e}")
+    assert ka == kb
+    assert sa["successes"] == sb["successes"]
+    assert rel < 1e-6
+    assert sa["fwd_nodes"] < sb["fwd_nodes"]
