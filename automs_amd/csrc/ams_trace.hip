@@ -583,6 +583,14 @@ __global__ void k_fib_keys(long long n, unsigned *__restrict__ keys, unsigned *_
 // instances): the traversal is a chain of dependent node loads, and from LDS
 // each costs a fraction of an L2 hit.  Larger scenes are read from L2/HBM.
 constexpr int kSceneLds = 32 * 1024;
+// Discovery launch shapes.  The binary walk: workgroups of kDiscBlock
+// threads with kDiscRays rays each (the workgroup's record filter sees 2048
+// rays) at 72 registers, 7 waves per SIMD.  The four-wide walk keeps its
+// stack in registers: one ray per thread in workgroups of 256.
+constexpr int kDiscBlock = 512;
+constexpr int kDiscRays = 4;
+constexpr int kDiscWideBlock = 256;
+constexpr int kDiscWideRays = 1;
 static_assert(sizeof(AmsNode) == 2 * sizeof(float4) && sizeof(AmsNode4) == 8 * sizeof(float4) &&
                   sizeof(AmsTri) == 3 * sizeof(float4),
               "stage_scene copies nodes and triangles as float4");
@@ -604,7 +612,7 @@ struct DiscoverArgs {
   const AmsTri *tris;
   int n_tris;
   const AmsTriAux *aux;
-  const float *mat_prop;
+  const float *mat_prop;  // [n_mat][19]
   const AmsNode *snodes;
   int n_snodes;
   const float4 *sph;
@@ -625,6 +633,29 @@ struct DiscoverArgs {
   unsigned long long cap;
   unsigned long long *counters;  // nodes, tris, spheres, queries, records(raw)
 };
+
+// Scene data the discovery kernels read per query: the hierarchy walked
+// (four-wide when given) and the triangles, copied into LDS once per
+// workgroup by the LDS instances; otherwise the views point at global
+// memory (the normals and the material table are always read from there).
+struct SceneView {
+  const AmsNode *tnodes;
+  const AmsNode4 *tnodes4;
+  const AmsTri *tris;
+};
+template <bool LDS>
+__device__ __forceinline__ SceneView discovery_scene(const DiscoverArgs &a, float4 *smem) {
+  SceneView v{a.tnodes, a.tnodes4, a.tris};
+  if (LDS) {
+    const int n4 = a.tnodes4 ? 8 * a.n_tnodes4 : 2 * a.n_tnodes;
+    stage_scene(a.tnodes4 ? reinterpret_cast<const float4 *>(a.tnodes4) : reinterpret_cast<const float4 *>(a.tnodes),
+                n4, a.tris, a.n_tris, smem);
+    if (a.tnodes4) v.tnodes4 = reinterpret_cast<const AmsNode4 *>(smem);
+    else v.tnodes = reinterpret_cast<const AmsNode *>(smem);
+    v.tris = reinterpret_cast<const AmsTri *>(smem + n4);
+  }
+  return v;
+}
 
 __device__ __forceinline__ int rx_query(const DiscoverArgs &a, const Ray &ray, float &ts, Counters &ct) {
   return a.lat.on ? closest_sphere_lattice(a.lat, a.sph_o, ray, AMS_TMIN, ts, a.acc2, a.r2, ct)
@@ -653,134 +684,143 @@ struct Branch {
   uint64_t khi, klo;
 };
 
-template <bool LDS>
-__global__ void __launch_bounds__(256) k_discover(DiscoverArgs a) {
+template <bool LDS, bool WIDE>
+__device__ __forceinline__ void discover_rays(const DiscoverArgs &a, const int rays_per_thread) {
   extern __shared__ float4 ams_smem[];
-  const AmsNode *tnodes = a.tnodes;
-  const AmsNode4 *tnodes4 = a.tnodes4;
-  const AmsTri *tris = a.tris;
-  if (LDS) {
-    // the hierarchy the queries walk (four-wide when given) and the triangles
-    const int n4 = tnodes4 ? 8 * a.n_tnodes4 : 2 * a.n_tnodes;
-    stage_scene(tnodes4 ? reinterpret_cast<const float4 *>(a.tnodes4) : reinterpret_cast<const float4 *>(a.tnodes), n4,
-                a.tris, a.n_tris, ams_smem);
-    if (tnodes4) tnodes4 = reinterpret_cast<const AmsNode4 *>(ams_smem);
-    else tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
-    tris = reinterpret_cast<const AmsTri *>(ams_smem + n4);
-  }
+  const SceneView sv = discovery_scene<LDS>(a, ams_smem);
+  const AmsNode *tnodes = sv.tnodes;
+  const AmsNode4 *tnodes4 = sv.tnodes4;
+  const AmsTri *tris = sv.tris;
   __shared__ DedupTable dd;
   dedup_init(dd);
-  long long r = a.ray_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   Counters ct;
-  bool active = r < a.ray_end;
   unsigned nrec = 0;
-  if (active) {
-    long long e, k;
-    if (a.per_element) {
-      e = r / a.fib_n;
-      k = a.perm[r % a.fib_n];
-    } else {
-      long long rr2 = a.perm[r];
-      e = rr2 % a.n_src;
-      k = rr2;
-    }
-    double dx, dy, dz;
-    fib_dir(k, a.fib_n, dx, dy, dz);
-    Branch st[AMS_MAX_HIT];
-    int sp = 0;
-    Branch cur;
-    cur.o = f3(a.src[3 * e], a.src[3 * e + 1], a.src[3 * e + 2]);
-    cur.d = f3((float)dx, (float)dy, (float)dz);
-    cur.ttl = a.K + 1;
-    cur.b = 0;
-    cur.nref = 0;
-    cur.last_pl = -1;
-    cur.khi = (0xFFFFFull << 36) | (0xFFFFFull << 16) | 0xFFFFull;
-    cur.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
-    uint64_t last_hi = ~0ull, last_lo = ~0ull;  // previous record of this ray
-    bool more = true;
-    while (more) {
-      // After an Rx hit the ray goes on in the same direction, so its next
-      // scene hit is the one already found (tri_rem further on); only the
-      // sphere query is repeated (flags bit 3 repeats both).
-      bool tri_cached = false;
-      float tri_rem = 0.0f;
-      int tri_slot = -1, tri_orig = 0;
-      while (cur.ttl > 0) {
-        Ray ray;
-        ray_setup(ray, cur.o, cur.d);
-        float tt;
-        int torig, slot;
-        if (tri_cached) {
-          tt = tri_rem;
-          slot = tri_slot;
-          torig = tri_orig;
-        } else {
-          tt = INFINITY;
-          slot = scene_closest(tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
-        }
-        float ts = tt;
-        int rx = rx_query(a, ray, ts, ct);
-        ct.queries++;
-        tri_cached = false;
-        if (rx >= 0 && ts < tt) {
-          if (!(a.flags & 8)) {
-            tri_cached = true;
-            tri_rem = tt - ts;
-            tri_slot = slot;
-            tri_orig = torig;
-          }
-          // Rx hit: record (key, tile) and continue from the closest approach
-          int h = a.K + 1 - cur.ttl;
-          uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
-          uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
-          uint64_t lo = (cur.klo & ~0xFFFFFull) | tile;
-          if ((hi != last_hi || lo != last_lo) && dedup_first(dd, hi, lo)) {
-            unsigned long long idx = wave_append(a.out_count);
-            nrec++;
-            if (idx < a.cap) {
-              a.out_hi[idx] = hi;
-              a.out_lo[idx] = lo;
-            }
-          }
-          last_hi = hi;
-          last_lo = lo;
-          cur.o = cur.o + cur.d * ts;
-          cur.last_pl = -1;
-          continue;
-        }
-        if (slot < 0) break;  // miss, optixPathTracer.cu:276-279
-        const AmsTri tr = tris[slot];
-        int pl = __float_as_int(tr.b.w);
-        cur.o = cur.o + cur.d * tt;
-        cur.last_pl = pl;
-        if (cur.ttl > 1) {
-          // optixPathTracer.cu:294-328
-          float4 nm = a.aux[torig].n_maxedge;
-          float3 N0 = f3(nm.x, nm.y, nm.z);
-          float3 N = dot3(N0, cur.d) > 0.0f ? N0 * -1.0f : N0;  // faceforward(N0, -d, N0)
-          int m = __float_as_int(tr.c.w);
-          int th = theta_bin(cur.d, N);
-          bool pen = a.mat_prop[th + 19 * m] < 0.0f;
-          cur.ttl--;
-          int h = a.K + 1 - cur.ttl;
-          if (pen && sp < AMS_MAX_HIT) st[sp++] = cur;  // transmitted parent continues later
-          cur.b |= 1 << (h - 1);
-          cur.d = norm3(cur.d - N * (2.0f * dot3(cur.d, N)));
-          if (cur.nref < AMS_MAX_HIT) key_set_plane(cur.khi, cur.klo, cur.nref, (uint32_t)pl);
-          cur.nref++;
-        }
-        // ttl == 1: scene is pass-through (optixPathTracer.cu:294)
-        if (cur.ttl == 1 && !(a.flags & 128) && !rx_reachable(a, cur.o, cur.d)) break;
+  // rays_per_thread passes, each over blockDim.x consecutive rays
+  for (int pass = 0; pass < rays_per_thread; pass++) {
+    const long long r = a.ray_begin + ((long long)blockIdx.x * rays_per_thread + pass) * blockDim.x + threadIdx.x;
+    const bool active = r < a.ray_end;
+    if (active) {
+      long long e, k;
+      if (a.per_element) {
+        e = r / a.fib_n;
+        k = a.perm[r % a.fib_n];
+      } else {
+        long long rr2 = a.perm[r];
+        e = rr2 % a.n_src;
+        k = rr2;
       }
-      if (sp > 0) cur = st[--sp];
-      else more = false;
+      double dx, dy, dz;
+      fib_dir(k, a.fib_n, dx, dy, dz);
+      Branch st[AMS_MAX_HIT];
+      int sp = 0;
+      Branch cur;
+      cur.o = f3(a.src[3 * e], a.src[3 * e + 1], a.src[3 * e + 2]);
+      cur.d = f3((float)dx, (float)dy, (float)dz);
+      cur.ttl = a.K + 1;
+      cur.b = 0;
+      cur.nref = 0;
+      cur.last_pl = -1;
+      cur.khi = (0xFFFFFull << 36) | (0xFFFFFull << 16) | 0xFFFFull;
+      cur.klo = (0xFull << 60) | (0xFFFFFull << 40) | (0xFFFFFull << 20);
+      uint64_t last_hi = ~0ull, last_lo = ~0ull;  // previous record of this ray
+      bool more = true;
+      while (more) {
+        // After an Rx hit the ray goes on in the same direction, so its next
+        // scene hit is the one already found (tri_rem further on); only the
+        // sphere query is repeated (flags bit 3 repeats both).
+        bool tri_cached = false;
+        float tri_rem = 0.0f;
+        int tri_slot = -1, tri_orig = 0;
+        while (cur.ttl > 0) {
+          Ray ray;
+          ray_setup(ray, cur.o, cur.d);
+          float tt;
+          int torig, slot;
+          if (tri_cached) {
+            tt = tri_rem;
+            slot = tri_slot;
+            torig = tri_orig;
+          } else {
+            tt = INFINITY;
+            slot = WIDE ? scene_closest(tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct)
+                        : closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+          }
+          float ts = tt;
+          int rx = rx_query(a, ray, ts, ct);
+          ct.queries++;
+          tri_cached = false;
+          if (rx >= 0 && ts < tt) {
+            if (!(a.flags & 8)) {
+              tri_cached = true;
+              tri_rem = tt - ts;
+              tri_slot = slot;
+              tri_orig = torig;
+            }
+            // Rx hit: record (key, tile) and continue from the closest approach
+            int h = a.K + 1 - cur.ttl;
+            uint32_t tile = ams_tile_of(rx, a.n1_dst, a.gs);
+            uint64_t hi = (cur.khi & ((1ull << 56) - 1)) | ((uint64_t)h << 61) | ((uint64_t)cur.b << 56);
+            uint64_t lo = (cur.klo & ~0xFFFFFull) | tile;
+            if ((hi != last_hi || lo != last_lo) && dedup_first(dd, hi, lo)) {
+              unsigned long long idx = wave_append(a.out_count);
+              nrec++;
+              if (idx < a.cap) {
+                a.out_hi[idx] = hi;
+                a.out_lo[idx] = lo;
+              }
+            }
+            last_hi = hi;
+            last_lo = lo;
+            cur.o = cur.o + cur.d * ts;
+            cur.last_pl = -1;
+            continue;
+          }
+          if (slot < 0) break;  // miss, optixPathTracer.cu:276-279
+          const AmsTri tr = tris[slot];
+          int pl = __float_as_int(tr.b.w);
+          cur.o = cur.o + cur.d * tt;
+          cur.last_pl = pl;
+          if (cur.ttl > 1) {
+            // optixPathTracer.cu:294-328
+            float4 nm = a.aux[torig].n_maxedge;
+            float3 N0 = f3(nm.x, nm.y, nm.z);
+            float3 N = dot3(N0, cur.d) > 0.0f ? N0 * -1.0f : N0;  // faceforward(N0, -d, N0)
+            int m = __float_as_int(tr.c.w);
+            int th = theta_bin(cur.d, N);
+            bool pen = a.mat_prop[th + 19 * m] < 0.0f;
+            cur.ttl--;
+            int h = a.K + 1 - cur.ttl;
+            if (pen && sp < AMS_MAX_HIT) st[sp++] = cur;  // transmitted parent continues later
+            cur.b |= 1 << (h - 1);
+            cur.d = norm3(cur.d - N * (2.0f * dot3(cur.d, N)));
+            if (cur.nref < AMS_MAX_HIT) key_set_plane(cur.khi, cur.klo, cur.nref, (uint32_t)pl);
+            cur.nref++;
+          }
+          // ttl == 1: scene is pass-through (optixPathTracer.cu:294)
+          if (cur.ttl == 1 && !(a.flags & 128) && !rx_reachable(a, cur.o, cur.d)) break;
+        }
+        if (sp > 0) cur = st[--sp];
+        else more = false;
+      }
     }
   }
   wave_add_counters(a.counters, ct);
   unsigned long long x = nrec;
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
   if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
+}
+
+// Forward discovery (optixPathTracer.cu:142-190, one sample per ray): each
+// thread walks the branch trees of its rays with a local stack; the two
+// launch shapes above.
+template <bool LDS>
+__global__ void __launch_bounds__(kDiscBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
+k_discover(DiscoverArgs a) {
+  discover_rays<LDS, false>(a, kDiscRays);
+}
+template <bool LDS>
+__global__ void __launch_bounds__(kDiscWideBlock) k_discover_wide(DiscoverArgs a) {
+  discover_rays<LDS, true>(a, kDiscWideRays);
 }
 
 // Wavefront-persistent form of the same discovery: the grid is sized to the
@@ -948,18 +988,10 @@ template <bool LDS>
 __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
   const DiscoverArgs &a = ra.d;
   extern __shared__ float4 ams_smem[];
-  const AmsNode *tnodes = a.tnodes;
-  const AmsNode4 *tnodes4 = a.tnodes4;
-  const AmsTri *tris = a.tris;
-  if (LDS) {
-    // the hierarchy the queries walk (four-wide when given) and the triangles
-    const int n4 = tnodes4 ? 8 * a.n_tnodes4 : 2 * a.n_tnodes;
-    stage_scene(tnodes4 ? reinterpret_cast<const float4 *>(a.tnodes4) : reinterpret_cast<const float4 *>(a.tnodes), n4,
-                a.tris, a.n_tris, ams_smem);
-    if (tnodes4) tnodes4 = reinterpret_cast<const AmsNode4 *>(ams_smem);
-    else tnodes = reinterpret_cast<const AmsNode *>(ams_smem);
-    tris = reinterpret_cast<const AmsTri *>(ams_smem + n4);
-  }
+  const SceneView sv = discovery_scene<LDS>(a, ams_smem);
+  const AmsNode *tnodes = sv.tnodes;
+  const AmsNode4 *tnodes4 = sv.tnodes4;
+  const AmsTri *tris = sv.tris;
   __shared__ DedupTable dd;
   dedup_init(dd);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1845,14 +1877,18 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     da.ray_begin = rb;
     da.ray_end = re;
     unsigned blocks = (unsigned)((re - rb + 255) / 256);
+    const long long per_block = wide ? (long long)kDiscWideBlock * kDiscWideRays : (long long)kDiscBlock * kDiscRays;
+    const unsigned dblocks = (unsigned)((re - rb + per_block - 1) / per_block);
     HIPCHK(hipEventRecord(e0, st));
     if (o->flags & 4) {
       HIPCHK(hipMemsetAsync(d_next, 0, 8, st));
       unsigned pblocks = std::min<unsigned>(blocks, (unsigned)(n_cu * 8));
       k_discover_persistent<<<pblocks, 256, 0, st>>>(da, d_next);
     } else {
-      if (use_lds) k_discover<true><<<blocks, 256, lds_bytes, st>>>(da);
-      else k_discover<false><<<blocks, 256, 0, st>>>(da);
+      if (wide && use_lds) k_discover_wide<true><<<dblocks, kDiscWideBlock, lds_bytes, st>>>(da);
+      else if (wide) k_discover_wide<false><<<dblocks, kDiscWideBlock, 0, st>>>(da);
+      else if (use_lds) k_discover<true><<<dblocks, kDiscBlock, lds_bytes, st>>>(da);
+      else k_discover<false><<<dblocks, kDiscBlock, 0, st>>>(da);
     }
     HIPCHK(hipGetLastError());
     S.n_discover_launches++;
