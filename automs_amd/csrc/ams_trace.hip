@@ -1659,18 +1659,35 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   HIPCHK(hipMemcpyAsync(d_src, tx->pos, sizeof(float) * 3 * n_src, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_dst, rx->pos, sizeof(float) * 3 * n_dst, hipMemcpyHostToDevice, st));
   const float dst_acc = std::max(rx->radius, AMS_ACC_MIN);
-  std::vector<double> slo(3 * (size_t)n_dst), shi(3 * (size_t)n_dst);
-  for (int j = 0; j < n_dst; j++)
-    for (int k = 0; k < 3; k++) {
-      slo[3 * j + k] = (double)rx->pos[3 * j + k] - dst_acc;
-      shi[3 * j + k] = (double)rx->pos[3 * j + k] + dst_acc;
-    }
+  // regular Rx lattice: sphere candidates found without the hierarchy (flags bit 5 keeps it)
+  RxLattice lat = rx_lattice(rx, dst_acc, !(o->flags & 32));
+  // The sphere hierarchy is built only for arrays that are not a lattice (a
+  // host build of 4 ms for 4096 Rx and 84 ms for 65,536, on every call);
+  // otherwise one node that no ray enters stands in for it.
   AmsBvhBuild SB;
-  ams_bvh_build(n_dst, slo.data(), shi.data(), AMS_LEAF_MAX, &SB);
-  std::vector<float4> sph(n_dst);
-  for (int q = 0; q < n_dst; q++) {
-    int j = SB.order[q];
-    sph[q] = make_float4(rx->pos[3 * j], rx->pos[3 * j + 1], rx->pos[3 * j + 2], bits_f(j));
+  std::vector<float4> sph(std::max(n_dst, 1), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  if (!lat.on) {
+    std::vector<double> slo(3 * (size_t)n_dst), shi(3 * (size_t)n_dst);
+    for (int j = 0; j < n_dst; j++)
+      for (int k = 0; k < 3; k++) {
+        slo[3 * j + k] = (double)rx->pos[3 * j + k] - dst_acc;
+        shi[3 * j + k] = (double)rx->pos[3 * j + k] + dst_acc;
+      }
+    ams_bvh_build(n_dst, slo.data(), shi.data(), AMS_LEAF_MAX, &SB);
+    for (int q = 0; q < n_dst; q++) {
+      int j = SB.order[q];
+      sph[q] = make_float4(rx->pos[3 * j], rx->pos[3 * j + 1], rx->pos[3 * j + 2], bits_f(j));
+    }
+  }
+  if (SB.nodes.empty()) {
+    AmsNode n;
+    for (int k = 0; k < 3; k++) {
+      n.lo[k] = 1.0f;
+      n.hi[k] = -1.0f;
+    }
+    n.miss = 1;
+    n.info = 0;
+    SB.nodes.push_back(n);
   }
   AmsNode *d_snodes = (AmsNode *)sc.get("snodes", sizeof(AmsNode) * SB.nodes.size());
   float4 *d_sph = (float4 *)sc.get("sph", sizeof(float4) * sph.size());
@@ -1679,8 +1696,6 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   HIPCHK(hipMemcpyAsync(d_snodes, SB.nodes.data(), sizeof(AmsNode) * SB.nodes.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_sph, sph.data(), sizeof(float4) * sph.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
-  // regular Rx lattice: sphere candidates found without the hierarchy (flags bit 5 keeps it)
-  RxLattice lat = rx_lattice(rx, dst_acc, !(o->flags & 32));
   float4 *d_sph_o = nullptr;
   if (lat.on) {
     std::vector<float4> sph_o(n_dst);
