@@ -114,6 +114,59 @@ def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_ref
     return result, stats, keys
 
 
+_job_streams: dict = {}
+
+
+def trace_many(jobs: list, device: str = "cuda") -> list:
+    """Trace independent channel sets concurrently.  Each job is a dict of
+    trace()'s arguments (ts, tx, rx, wave_len, max_reflection, fib_n, and any
+    keywords); each runs from its own host thread on its own stream, so the
+    host steps of one trace (record sort, path list, expansion rounds) and the
+    tails of its kernels overlap the kernels of the others.  Every job needs
+    its own TraceScene handle: a handle's scratch serves one call at a time.
+    Results are those of calling trace() on each job in turn; the caller's
+    current stream is ordered after all of them on return."""
+    import threading
+
+    if len({id(j["ts"]) for j in jobs}) != len(jobs):
+        raise ValueError("trace_many: every job needs its own TraceScene handle")
+    dev = torch.device(device)
+    if dev.index is None:
+        dev = torch.device(dev.type, torch.cuda.current_device())
+    caller = torch.cuda.current_stream(dev)
+    streams = _job_streams.setdefault(dev.index, [])
+    while len(streams) < len(jobs):
+        streams.append(torch.cuda.Stream(dev))
+    jobs = [dict(j) for j in jobs]
+    for j in jobs:  # results zero-filled on the caller's stream come first
+        if j.get("result") is None:
+            j["result"] = torch.zeros(j["tx"].n, j["rx"].n, 2, dtype=torch.float32, device=dev)
+    out: list = [None] * len(jobs)
+    err: list = [None] * len(jobs)
+
+    def run(k):
+        try:
+            s = streams[k]
+            s.wait_stream(caller)
+            jobs[k]["result"].record_stream(s)
+            with torch.cuda.device(dev), torch.cuda.stream(s):
+                out[k] = trace(device=str(dev), **jobs[k])
+        except BaseException as e:  # re-raised in the caller's thread
+            err[k] = e
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(len(jobs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for e in err:
+        if e is not None:
+            raise e
+    for k in range(len(jobs)):
+        caller.wait_stream(streams[k])
+    return out
+
+
 def trace_workload(w: Workload, ts: Optional[TraceScene] = None, **kw):
     ts = ts or TraceScene(w.scene)
     res, stats, keys = trace(ts, w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,

@@ -291,3 +291,36 @@ def test_four_wide_walk_matches_binary(small_room, trace_scene, base):
     assert sa["successes"] == sb["successes"]
     assert rel < 1e-6
     assert sa["fwd_nodes"] < sb["fwd_nodes"]
+
+
+def test_trace_many_matches_sequential(gpu):
+    """trace_many runs independent traces from separate host threads on
+    separate streams: every counter equals that of calling trace() on each job
+    in turn and H agrees to FP32 rounding (three metasurface traces, reduced
+    ray counts); jobs sharing a scene handle are refused."""
+    import torch
+
+    from automs_amd import scenes
+    from automs_amd.tracer import TraceScene, trace, trace_many
+    _flush("case: trace_many vs sequential (room_boxes_ms, fib 2^14, ms fib 2^18)")
+    tr = scenes.room_boxes_ms(fib_n=1 << 14, ms_fib_n=1 << 18)
+    hs = {k: TraceScene(w.scene) for k, w in tr.items()}
+
+    def job(k):
+        w = tr[k]
+        return dict(ts=hs[k], tx=w.tx, rx=w.rx, wave_len=w.wave_len, max_reflection=w.max_reflection,
+                    fib_n=w.fib_n, per_element=w.per_element, gridsize=w.gridsize)
+    seq = {k: trace(**job(k)) for k in tr}
+    torch.cuda.synchronize()
+    for rep in range(2):
+        par = dict(zip(tr, trace_many([job(k) for k in tr])))
+        torch.cuda.synchronize()
+        for k in tr:
+            # H sums FP64 atomics in no fixed order: equal to rounding
+            rel = float((seq[k][0].double() - par[k][0].double()).norm() / seq[k][0].double().norm())
+            assert rel < 1e-6, (rep, k, rel)
+            for f in ("records_unique", "paths", "tasks", "chains", "successes", "fwd_queries", "bwd_queries"):
+                assert seq[k][1][f] == par[k][1][f], (rep, k, f)
+        _flush(f"  rep {rep}: identical counters, H within 1e-6 for {list(tr)}")
+    with pytest.raises(ValueError):
+        trace_many([job("G"), job("G")])
