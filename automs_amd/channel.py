@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from .scenes import ArraySpec, Scene
-from .tracer import TraceScene, trace
+from .tracer import TraceScene, trace, trace_many
 
 # ---------------------------------------------------------------- scene I/O --
 
@@ -185,6 +185,35 @@ def init(op_path: Optional[str] = None, seed: int = 5) -> None:
 
 # -------------------------------------------------------- build_channel -----
 
+def _channel_job(src, dst, src_dir, dst_dir, src_bptype: int, dst_bptype: int, wave_len: float,
+                 scene_folder: Optional[str] = None, max_reflection: int = 4, dst_radius: float = 0.06,
+                 src_radius: float = 0.5e-5, samples_per_launch: int = 1, reserved_size: int = 100,
+                 blackhole_buffer=(), reflector_buffer=(), door_buffer=(), dst_gridsize: int = 5,
+                 scene: Optional[Scene] = None, rays: Optional[int] = None, per_element: bool = False,
+                 refl_phase: float = 0.0) -> Tuple[Scene, dict]:
+    """The scene and the trace() arguments (without the handle) of one
+    build_channel call."""
+    if scene is None:
+        if scene_folder is None:
+            raise ValueError("build_channel needs scene_folder or scene")
+        scene = load_scene_folder(scene_folder, blackhole_buffer, reflector_buffer, door_buffer, refl_phase)
+    else:
+        scene = scene.with_extra([t for b in blackhole_buffer for t in b], 0.0, "blackhole", refl_phase)
+        scene = scene.with_extra([t for b in reflector_buffer for t in b], 1.0, "reflector", refl_phase)
+        scene = scene.with_extra([t for b in door_buffer for t in b], -0.7, "door", refl_phase)
+    s, d = _np(src), _np(dst)
+    tx = ArraySpec(s, _np(src_dir).reshape(3), int(src_bptype), float(src_radius))
+    rx = ArraySpec(d, _np(dst_dir).reshape(3), int(dst_bptype), float(dst_radius))
+    if rays is None:
+        job = dict(tx=tx, rx=rx, wave_len=wave_len, max_reflection=int(max_reflection), fib_n=0, per_element=False,
+                   gridsize=int(dst_gridsize),
+                   reference_launch=(100, 100, int(samples_per_launch), int(reserved_size)))
+    else:
+        job = dict(tx=tx, rx=rx, wave_len=wave_len, max_reflection=int(max_reflection), fib_n=int(rays),
+                   per_element=per_element, gridsize=int(dst_gridsize))
+    return scene, job
+
+
 def build_channel(src: torch.Tensor, dst: torch.Tensor, src_dir, dst_dir, src_bptype: int, dst_bptype: int,
                   wave_len: float, scene_folder: Optional[str] = None, max_reflection: int = 4,
                   dst_radius: float = 0.06, src_radius: float = 0.5e-5, samples_per_launch: int = 1,
@@ -197,26 +226,14 @@ def build_channel(src: torch.Tensor, dst: torch.Tensor, src_dir, dst_dir, src_bp
     rays None: the reference's own launch (go.py:739-765), 100 x 100 threads
     x samples_per_launch TEA/LCG rays with reserved_size records per thread;
     rays given: that many Fibonacci rays (per src element if per_element)."""
-    if scene is None:
-        if scene_folder is None:
-            raise ValueError("build_channel needs scene_folder or scene")
-        scene = load_scene_folder(scene_folder, blackhole_buffer, reflector_buffer, door_buffer, refl_phase)
-    else:
-        scene = scene.with_extra([t for b in blackhole_buffer for t in b], 0.0, "blackhole", refl_phase)
-        scene = scene.with_extra([t for b in reflector_buffer for t in b], 1.0, "reflector", refl_phase)
-        scene = scene.with_extra([t for b in door_buffer for t in b], -0.7, "door", refl_phase)
-    s, d = _np(src), _np(dst)
-    tx = ArraySpec(s, _np(src_dir).reshape(3), int(src_bptype), float(src_radius))
-    rx = ArraySpec(d, _np(dst_dir).reshape(3), int(dst_bptype), float(dst_radius))
-    result = torch.zeros(tx.n, rx.n, 2, dtype=torch.float32, device=device)
-    ts = TraceScene(scene)
+    sc, job = _channel_job(src, dst, src_dir, dst_dir, src_bptype, dst_bptype, wave_len, scene_folder,
+                           max_reflection, dst_radius, src_radius, samples_per_launch, reserved_size,
+                           blackhole_buffer, reflector_buffer, door_buffer, dst_gridsize, scene, rays, per_element,
+                           refl_phase)
+    result = torch.zeros(job["tx"].n, job["rx"].n, 2, dtype=torch.float32, device=device)
+    ts = TraceScene(sc)
     try:
-        if rays is None:
-            trace(ts, tx, rx, wave_len, int(max_reflection), 0, per_element=False, gridsize=int(dst_gridsize),
-                  result=result, reference_launch=(100, 100, int(samples_per_launch), int(reserved_size)))
-        else:
-            trace(ts, tx, rx, wave_len, int(max_reflection), int(rays), per_element=per_element,
-                  gridsize=int(dst_gridsize), result=result)
+        trace(ts, result=result, device=device, **job)
     finally:
         ts.close()
     return torch.complex(result[..., 0], result[..., 1])
@@ -234,19 +251,32 @@ def build_channels(src: torch.Tensor, dst: torch.Tensor, ms_list, ap_dir, scene_
     n = len(ms_list)
     kw = dict(scene_folder=scene_folder, max_reflection=max_reflection, reflector_buffer=reflector_buffer,
               door_buffer=door_buffer, scene=scene, refl_phase=refl_phase)
-    ch = {"src_to_dst": build_channel(src, dst, ap_dir, dir_no, 2, 0, wave_len,
-                                      blackhole_buffer=[m.blackhole for m in ms_list], rays=rays,
-                                      per_element=per_element, **kw)}
+    spec = {"src_to_dst": _channel_job(src, dst, ap_dir, dir_no, 2, 0, wave_len,
+                                       blackhole_buffer=[m.blackhole for m in ms_list], rays=rays,
+                                       per_element=per_element, **kw)}
     for i in range(n):
         others = [ms_list[k].blackhole for k in range(n) if k != i]
-        ch["src_to_ms{%d}" % i] = build_channel(src, ms_list[i].pos, ap_dir, ms_list[i].recv_dir, 2, 1, wave_len,
-                                                blackhole_buffer=others, rays=rays, per_element=per_element,
-                                                **kw)
-        ch["ms{%d}_to_dst" % i] = build_channel(ms_list[i].pos, dst, ms_list[i].send_dir, dir_no, 1, 0, wave_len,
-                                                blackhole_buffer=others, rays=ms_rays or rays, **kw)
+        spec["src_to_ms{%d}" % i] = _channel_job(src, ms_list[i].pos, ap_dir, ms_list[i].recv_dir, 2, 1, wave_len,
+                                                 blackhole_buffer=others, rays=rays, per_element=per_element,
+                                                 **kw)
+        spec["ms{%d}_to_dst" % i] = _channel_job(ms_list[i].pos, dst, ms_list[i].send_dir, dir_no, 1, 0, wave_len,
+                                                 blackhole_buffer=others, rays=ms_rays or rays, **kw)
         for j in range(i + 1, n):
             oth = [ms_list[k].blackhole for k in range(n) if k not in (i, j)]
-            ch["ms{%d}_to_ms{%d}" % (i, j)] = build_channel(ms_list[i].pos, ms_list[j].pos, ms_list[i].send_dir,
-                                                            ms_list[j].recv_dir, 1, 1, wave_len,
-                                                            blackhole_buffer=oth, rays=ms_rays or rays, **kw)
-    return ch
+            spec["ms{%d}_to_ms{%d}" % (i, j)] = _channel_job(ms_list[i].pos, ms_list[j].pos, ms_list[i].send_dir,
+                                                             ms_list[j].recv_dir, 1, 1, wave_len,
+                                                             blackhole_buffer=oth, rays=ms_rays or rays, **kw)
+    # every channel set is an independent trace with its own scene: build the
+    # scene hierarchies on host threads and trace the sets concurrently
+    from concurrent.futures import ThreadPoolExecutor
+
+    names = list(spec)
+    with ThreadPoolExecutor(max_workers=min(8, len(names))) as ex:
+        handles = list(ex.map(lambda k: TraceScene(spec[k][0]), names))
+    try:
+        res = trace_many([dict(ts=h, **spec[k][1]) for k, h in zip(names, handles)], max_concurrent=4)
+    finally:
+        torch.cuda.synchronize()
+        for h in handles:
+            h.close()
+    return {k: torch.complex(r[0][..., 0], r[0][..., 1]) for k, r in zip(names, res)}

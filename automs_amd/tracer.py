@@ -117,15 +117,16 @@ def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_ref
 _job_streams: dict = {}
 
 
-def trace_many(jobs: list, device: str = "cuda") -> list:
+def trace_many(jobs: list, device: str = "cuda", max_concurrent: Optional[int] = None) -> list:
     """Trace independent channel sets concurrently.  Each job is a dict of
     trace()'s arguments (ts, tx, rx, wave_len, max_reflection, fib_n, and any
-    keywords); each runs from its own host thread on its own stream, so the
-    host steps of one trace (record sort, path list, expansion rounds) and the
-    tails of its kernels overlap the kernels of the others.  Every job needs
-    its own TraceScene handle: a handle's scratch serves one call at a time.
-    Results are those of calling trace() on each job in turn; the caller's
-    current stream is ordered after all of them on return."""
+    keywords); at most max_concurrent (default: all) run at once, each from
+    its own host thread on its own stream, so the host steps of one trace
+    (record sort, path list, expansion rounds) and the tails of its kernels
+    overlap the kernels of the others.  Every job needs its own TraceScene
+    handle: a handle's scratch serves one call at a time.  Results are those
+    of calling trace() on each job in turn; the caller's current stream is
+    ordered after all of them on return."""
     import threading
 
     if len({id(j["ts"]) for j in jobs}) != len(jobs):
@@ -133,9 +134,10 @@ def trace_many(jobs: list, device: str = "cuda") -> list:
     dev = torch.device(device)
     if dev.index is None:
         dev = torch.device(dev.type, torch.cuda.current_device())
+    workers = max(1, min(len(jobs), max_concurrent or len(jobs)))
     caller = torch.cuda.current_stream(dev)
     streams = _job_streams.setdefault(dev.index, [])
-    while len(streams) < len(jobs):
+    while len(streams) < workers:
         streams.append(torch.cuda.Stream(dev))
     jobs = [dict(j) for j in jobs]
     for j in jobs:  # results zero-filled on the caller's stream come first
@@ -143,18 +145,26 @@ def trace_many(jobs: list, device: str = "cuda") -> list:
             j["result"] = torch.zeros(j["tx"].n, j["rx"].n, 2, dtype=torch.float32, device=dev)
     out: list = [None] * len(jobs)
     err: list = [None] * len(jobs)
+    lock = threading.Lock()
+    nxt = [0]
 
-    def run(k):
-        try:
-            s = streams[k]
-            s.wait_stream(caller)
-            jobs[k]["result"].record_stream(s)
-            with torch.cuda.device(dev), torch.cuda.stream(s):
-                out[k] = trace(device=str(dev), **jobs[k])
-        except BaseException as e:  # re-raised in the caller's thread
-            err[k] = e
+    def run(w):
+        s = streams[w]
+        s.wait_stream(caller)
+        with torch.cuda.device(dev), torch.cuda.stream(s):
+            while True:
+                with lock:
+                    k = nxt[0]
+                    nxt[0] += 1
+                if k >= len(jobs):
+                    return
+                try:
+                    jobs[k]["result"].record_stream(s)
+                    out[k] = trace(device=str(dev), **jobs[k])
+                except BaseException as e:  # re-raised in the caller's thread
+                    err[k] = e
 
-    threads = [threading.Thread(target=run, args=(k,)) for k in range(len(jobs))]
+    threads = [threading.Thread(target=run, args=(w,)) for w in range(workers)]
     for t in threads:
         t.start()
     for t in threads:
@@ -162,8 +172,8 @@ def trace_many(jobs: list, device: str = "cuda") -> list:
     for e in err:
         if e is not None:
             raise e
-    for k in range(len(jobs)):
-        caller.wait_stream(streams[k])
+    for w in range(workers):
+        caller.wait_stream(streams[w])
     return out
 
 
