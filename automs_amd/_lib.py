@@ -43,6 +43,7 @@ class AmsTraceOpts(C.Structure):
         ("launch_y", C.c_int32),
         ("samples_per_launch", C.c_int32),
         ("reserve_size", C.c_int32),
+        ("task_info_path", C.c_char_p),
     ]
 
 

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_set>
@@ -594,6 +595,9 @@ constexpr int kDiscWideRays = 1;
 // Evaluation: at most kEvalBlock lanes per workgroup (256 measured 7% faster
 // than 512 and 16% faster than 128 on the room traces)
 constexpr int kEvalBlock = 256;
+// records of one round the task export may hold (52 bytes each)
+constexpr size_t kMaxTaskInfo = size_t(1) << 26;
+static_assert(sizeof(ams_task_info) == 52, "ams_task_info keeps the 52-byte layout of the reference's record");
 static_assert(sizeof(AmsNode) == 2 * sizeof(float4) && sizeof(AmsNode4) == 8 * sizeof(float4) &&
                   sizeof(AmsTri) == 3 * sizeof(float4),
               "stage_scene copies nodes and triangles as float4");
@@ -1140,9 +1144,12 @@ struct EvalArgs {
   double *H;                  // [n_src][n_dst][2]
   unsigned char *task_ok;
   unsigned long long *counters;  // nodes, tris, spheres, queries, chains, successes
+  ams_task_info *info;           // task export of this round (EXPORT instances), [src][task][gs2]
+  long long info_nt;             // tasks of this round
+  int gs2;                       // Rx per tile
 };
 
-template <bool LDS>
+template <bool LDS, bool EXPORT>
 __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_evaluate(EvalArgs a) {
   extern __shared__ float4 ams_smem[];
   // Backward legs are short bounded segments: the binary walk, whose loop
@@ -1176,6 +1183,9 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
   }
   for (int t = tb; t < te; t++) {
     bool ok = false;
+    float x_loss = 1.0f, x_launch = 0.0f, x_phase = 0.0f, x_recv = 0.0f;  // task export of the chain
+    int x_hits = 0;
+    int x_idx[AMS_MAX_HIT] = {0, 0, 0, 0, 0};
     if (active) {
       int p = a.task_path[t];
       uint64_t khi = a.path_hi[p], klo = a.path_lo[p];
@@ -1196,6 +1206,8 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
       float3 o = r3;
       float A = beam_pattern(d, a.dst_dir, a.dst_bptype);
       float D = 0.0f, Dres = 0.0f, far = 1e6f, psi = 0.0f;
+      // task export (optixPathTracer.cu:243-250): cosines as the reference forms them
+      if (EXPORT) x_recv = dot3(d, a.dst_dir) / (sqrtf(dot3(d, d)) * sqrtf(dot3(a.dst_dir, a.dst_dir)));
       int ttl = h + 1, ri = 0, last_pl = -1;
       nchain++;
       // Unfold the path analytically before any query: from the Rx towards
@@ -1268,6 +1280,11 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
                  (4.0f * (float)M_PI * (Dres + D));
             double cyc = L / a.wave_len_d;
             float ang = (float)(2.0 * M_PI * (cyc - floor(cyc))) - psi;
+            if (EXPORT) {  // optixPathTracer.cu:407-412
+              x_loss *= a.wave_len / (4.0f * (float)M_PI * (Dres + D));
+              x_launch = -dot3(d, a.src_dir) / (sqrtf(dot3(d, d)) * sqrtf(dot3(a.src_dir, a.src_dir)));
+              x_phase = (float)(2.0 * M_PI * cyc) - psi;
+            }
             float sn, cs;
             sincosf(ang, &sn, &cs);
             acc_re += (double)(A * cs);
@@ -1289,6 +1306,7 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
         int m = __float_as_int(tr.c.w);
         int th = theta_bin(d, N);
         float loss = fabsf(a.mat_prop[th + 19 * m]);
+        if (EXPORT && x_hits < AMS_MAX_HIT) x_idx[x_hits++] = th + 19 * m;  // optixPathTracer.cu:378-383
         ttl--;
         int k = h + 1 - ttl;
         bool refl = (bm >> (k - 1)) & 1;
@@ -1303,6 +1321,7 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
           Dres = 0.0f;
         }
         A *= loss * fspl;
+        if (EXPORT) x_loss *= fspl;
         Dres += D;
         float fk = 2.0f * nm.w * nm.w / a.wave_len;
         far = D > fk ? fk : 1e6f;
@@ -1312,6 +1331,18 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
           psi += a.refl_phase[m];
         }
       }
+    }
+    if (EXPORT && active) {
+      ams_task_info &rec = a.info[((long long)i * a.info_nt + t) * a.gs2 + q];
+      rec.successful = ok ? 1 : 0;
+      rec.launch_angle = x_launch;
+      rec.receive_angle = x_recv;
+      rec.loss = x_loss;
+      rec.phase = x_phase;
+      rec.src = i;
+      rec.dst = j;
+      rec.hit_cnt = x_hits;
+      for (int h2 = 0; h2 < AMS_MAX_HIT; h2++) rec.hit_loss_idx[h2] = x_idx[h2];
     }
     if (ok) nsucc++;
     unsigned long long any = __ballot(ok);
@@ -2019,6 +2050,15 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   if (!d_H) return AMS_ERR_ALLOC;
   HIPCHK(hipMemsetAsync(d_H, 0, sizeof(double) * 2 * (size_t)n_src * n_dst, st));
   HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
+  // task export file, created for every call as the reference's path.dat (optixPathTracer.cpp:831)
+  std::unique_ptr<FILE, int (*)(FILE *)> info_file(nullptr, &std::fclose);
+  if (o->task_info_path) {
+    info_file.reset(std::fopen(o->task_info_path, "wb"));
+    if (!info_file) {
+      set_error(std::string("task export: cannot open ") + o->task_info_path);
+      return AMS_ERR_ARG;
+    }
+  }
   if (n_paths > 0) {
     uint64_t *d_phi = (uint64_t *)sc.get("p_hi", 8 * (size_t)n_paths), *d_plo = (uint64_t *)sc.get("p_lo", 8 * (size_t)n_paths);
     if (!d_phi || !d_plo) return AMS_ERR_ALLOC;
@@ -2111,6 +2151,21 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
       ea.unit_begin = d_u + nu;
       ea.unit_end = d_u + 2 * nu;
       ea.task_ok = d_ok;
+      // task export of this round: [src][task][Rx in tile], zero-filled
+      size_t n_info = 0;
+      ea.info = nullptr;
+      ea.info_nt = (long long)nt;
+      ea.gs2 = gs * gs;
+      if (info_file) {
+        n_info = nt * (size_t)n_src * (size_t)(gs * gs);
+        if (n_info > kMaxTaskInfo) {
+          set_error("task export of one round exceeds 2^26 records (about 3.5 GB); trace fewer rays");
+          return AMS_ERR_ARG;
+        }
+        ea.info = (ams_task_info *)sc.get("t_info", n_info * sizeof(ams_task_info));
+        if (!ea.info) return AMS_ERR_ALLOC;
+        HIPCHK(hipMemsetAsync(ea.info, 0, n_info * sizeof(ams_task_info), st));
+      }
       HIPCHK(hipEventRecord(e0, st));
       for (int u0 = 0; u0 < nu; u0 += 65535) {
         int un = std::min(65535, nu - u0);
@@ -2118,8 +2173,14 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         eb.unit_tile += u0;
         eb.unit_begin += u0;
         eb.unit_end += u0;
-        if (eval_lds) k_evaluate<true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
-        else k_evaluate<false><<<dim3(bpt, un), block, 0, st>>>(eb);
+        if (eb.info) {
+          if (eval_lds) k_evaluate<true, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+          else k_evaluate<false, true><<<dim3(bpt, un), block, 0, st>>>(eb);
+        } else if (eval_lds) {
+          k_evaluate<true, false><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+        } else {
+          k_evaluate<false, false><<<dim3(bpt, un), block, 0, st>>>(eb);
+        }
         HIPCHK(hipGetLastError());
         S.n_evaluate_launches++;
       }
@@ -2130,6 +2191,14 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
       eval_ms += ms;
+      if (info_file && n_info > 0) {  // optixPathTracer.cpp:244-256
+        std::vector<ams_task_info> buf(n_info);
+        HIPCHK(hipMemcpy(buf.data(), ea.info, n_info * sizeof(ams_task_info), hipMemcpyDeviceToHost));
+        if (std::fwrite(buf.data(), sizeof(ams_task_info), n_info, info_file.get()) != n_info) {
+          set_error("task export: write failed");
+          return AMS_ERR_ARG;
+        }
+      }
       // neighbour expansion, optixPathTracer.cpp:264-284 (linear bounds only)
       std::vector<int> np_, nt_;
       for (size_t k = 0; k < nt; k++) {

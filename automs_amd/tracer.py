@@ -5,6 +5,7 @@ optixPathTracer.cpp:781-870).  Compute runs in libams.so on the GPU.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Dict, Optional
 
 import numpy as np
@@ -67,14 +68,18 @@ def _array(spec: ArraySpec, keep: list) -> _lib.AmsArray:
 def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_reflection: int, fib_n: int,
           per_element: bool = True, gridsize: int = 5, flags: int = 0, result: Optional[torch.Tensor] = None,
           keys_cap: int = 0, tasks_per_unit: int = 0, seed_keys=None, device: str = "cuda",
-          reference_launch: Optional[tuple] = None):
+          reference_launch: Optional[tuple] = None, task_info_path: Optional[str] = None):
     """Trace Tx -> Rx and accumulate into result (f32 [n_tx, n_rx, 2] on the
     GPU, zero-initialised if None).  Returns (result, stats dict, keys or None).
 
     reference_launch = (launch_x, launch_y, samples_per_launch, reserve_size)
     selects the reference's own discovery (optixPathTracer.cu:142-190: TEA/LCG
     random rays per launch index, per-tile min-hit gating, reserve cap)
-    instead of the Fibonacci ray set; fib_n is then ignored."""
+    instead of the Fibonacci ray set; fib_n is then ignored.
+
+    task_info_path writes one record per backward chain of every expansion
+    round to that file, as the reference's path.dat (EXPORT_TASKINFO,
+    optixPathTracer.cpp:244-256); read_task_info() loads it."""
     if result is None:
         result = torch.zeros(tx.n, rx.n, 2, dtype=torch.float32, device=device)
     if result.dtype != torch.float32 or not result.is_contiguous() or not result.is_cuda or \
@@ -95,6 +100,8 @@ def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_ref
         o.discovery_mode = 1
         o.launch_x, o.launch_y, o.samples_per_launch, o.reserve_size = lx, ly, spl, res
     o.stream = C.c_void_p(torch.cuda.current_stream(result.device).cuda_stream)
+    if task_info_path is not None:
+        o.task_info_path = os.fsencode(task_info_path)
     keys = None
     if keys_cap > 0:
         keys = np.zeros((keys_cap, 2), dtype=np.uint64)
@@ -112,6 +119,19 @@ def trace(ts: TraceScene, tx: ArraySpec, rx: ArraySpec, wave_len: float, max_ref
     if keys is not None:
         keys = keys[: min(keys_cap, stats["records_unique"])]
     return result, stats, keys
+
+
+# ams_task_info / ExportedTaskInfo (optixPathTracer.h:40-52): 52 bytes
+TASK_INFO_DTYPE = np.dtype([("successful", np.uint8), ("pad_", np.uint8, 3), ("launch_angle", np.float32),
+                            ("receive_angle", np.float32), ("loss", np.float32), ("phase", np.float32),
+                            ("src", np.int32), ("dst", np.int32), ("hit_cnt", np.int32),
+                            ("hit_loss_idx", np.int32, 5)])
+assert TASK_INFO_DTYPE.itemsize == 52
+
+
+def read_task_info(path: str) -> np.ndarray:
+    """Records written by trace(task_info_path=...), in file order."""
+    return np.fromfile(path, dtype=TASK_INFO_DTYPE)
 
 
 _job_streams: dict = {}

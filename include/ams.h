@@ -73,7 +73,33 @@ typedef struct {
                                  random Tx and stratified directions, per-thread min_hc gating
                                  and reserve_size records (optixPathTracer.cu:142-190) */
   int32_t launch_x, launch_y, samples_per_launch, reserve_size;
+  const char *task_info_path; /* optional: write one ams_task_info per backward chain of every
+                                 expansion round to this file (the reference's path.dat
+                                 under EXPORT_TASKINFO, optixPathTracer.cpp:244-256, :831) */
 } ams_trace_opts;
+
+/* One backward chain (optixPathTracer.h:40-52, 52 bytes in that layout).
+ * Records of a round are ordered [src][task][Rx in tile] as the reference's
+ * globalTaskId (optixPathTracer.cu:199), rounds one after another.  Fields
+ * follow optixPathTracer.cu:243-250, :378-383, :407-412:
+ *   successful     the chain reached its Tx element;
+ *   receive_angle  cos(initial direction, dst_dir);
+ *   launch_angle   cos(-final direction, src_dir), set on success;
+ *   loss           the free-space factors only: each Fraunhofer split's
+ *                  lambda / (4 pi D) and the final lambda / (4 pi D) at the Tx;
+ *   phase          2 pi (path length) / lambda minus the reflection phases,
+ *                  unwrapped, set on success;
+ *   hit_cnt, hit_loss_idx  scene hits on the way, theta_bin + 19 * material.
+ * Entries with no Rx element (tiles at the array's edge) and chains the
+ * analytic pre-check skips hold zeros apart from src, dst and receive_angle
+ * (the reference leaves the former unwritten). */
+typedef struct {
+  uint8_t successful;
+  uint8_t pad_[3];
+  float launch_angle, receive_angle, loss, phase;
+  int32_t src, dst, hit_cnt;
+  int32_t hit_loss_idx[5];
+} ams_task_info;
 
 /* *_nodes count 32-byte node boxes tested (a four-wide node is four) */
 typedef struct {

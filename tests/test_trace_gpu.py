@@ -324,3 +324,50 @@ def test_trace_many_matches_sequential(gpu):
         _flush(f"  rep {rep}: identical counters, H within 1e-6 for {list(tr)}")
     with pytest.raises(ValueError):
         trace_many([job("G"), job("G")])
+
+
+def test_task_export_rebuilds_H(small_room, trace_scene, tmp_path):
+    """The task export (the reference's path.dat under EXPORT_TASKINFO,
+    optixPathTracer.cpp:244-256): one 52-byte record per backward chain of
+    every round, [src][task][Rx in tile]; the successful records equal the
+    success counter, and H rebuilt from them alone (beam patterns of the two
+    cosines x |reflectivity| of every recorded hit x loss x e^{-j phase})
+    matches the traced H.  The phase is an unwrapped FP32 value (about 1e4
+    rad here), so the rebuild agrees to 3e-3.  Exporting leaves H unchanged."""
+    import torch
+
+    from automs_amd.tracer import read_task_info, trace
+    from oracle import oracle as O
+    w = small_room
+    _flush("case: task export rebuilds H (room_boxes, fib 2^12)")
+    path = tmp_path / "path.dat"
+    res0, st0, _ = trace(trace_scene, w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element, w.gridsize)
+    res, st, _ = trace(trace_scene, w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element, w.gridsize,
+                       task_info_path=str(path))
+    torch.cuda.synchronize()
+    H0 = res0.double().cpu().numpy()
+    H = res.double().cpu().numpy()
+    assert np.linalg.norm(H - H0) <= 1e-6 * np.linalg.norm(H0)
+    rec = read_task_info(str(path))
+    assert len(rec) == st["tasks"] * w.tx.n * w.gridsize ** 2
+    ok = rec[rec["successful"] == 1]
+    assert len(ok) == st["successes"]
+    assert (ok["hit_cnt"] <= w.max_reflection).all()
+
+    def bp(c, t):
+        if t == 0:
+            return np.ones(len(c))
+        return np.array([O.beam_pattern([float(x), float(np.sqrt(max(0.0, 1.0 - float(x) ** 2))), 0.0],
+                                        [1.0, 0.0, 0.0], t) for x in c])
+    gam = np.abs(np.asarray(w.scene.mat_prop, np.float64).reshape(-1))
+    srcs = [0, w.tx.n - 1]  # rebuild the rows of two Tx elements
+    sub = ok[np.isin(ok["src"], srcs)]
+    hit = np.arange(5)[None, :] < sub["hit_cnt"][:, None]
+    refl = np.where(hit, gam[np.where(hit, sub["hit_loss_idx"], 0)], 1.0).prod(axis=1)
+    amp = bp(sub["receive_angle"], w.rx.bptype) * bp(sub["launch_angle"], w.tx.bptype) * sub["loss"] * refl
+    Hr = np.zeros(H.shape[:2], np.complex128)
+    np.add.at(Hr, (sub["src"], sub["dst"]), amp * np.exp(-1j * sub["phase"].astype(np.float64)))
+    Hc = (H[..., 0] + 1j * H[..., 1])[srcs]
+    rel = np.linalg.norm(Hr[srcs] - Hc) / np.linalg.norm(Hc)
+    _flush(f"  records {len(rec)}, successful {len(ok)}, rebuilt {len(sub)} of Tx {srcs}: H rel {rel:.2e}")
+    assert rel < 3e-3
