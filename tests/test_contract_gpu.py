@@ -222,3 +222,36 @@ def test_metasurface_cascades_vs_reference_enumeration(gpu, reflex):
     rel = (Y - ref).norm() / ref.norm()
     _flush(f"  rel err {rel:.2e}")
     assert rel < 3e-5
+
+
+def test_postprocess_quantises_and_matches_oracle(gpu):
+    """postprocess (go.py:89-97): AP phases become the nearest multiples of
+    pi/2, metasurface phases are wrapped to [0, max_phase); the loss it
+    returns equals the complex128 oracle's at the quantised phases (1e-4)."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    from oracle import contract_ref as R
+    _flush("case: postprocess")
+    Hd, G, F = _channels(K=128, N=512, seed=21)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (8, 16), None, None, [], max_phase=300.0)]
+    th_ms = (torch.rand(1, 128, generator=torch.Generator().manual_seed(4)) - 0.3) * 6 * math.pi
+    model = PhaseMapModel("t", ch, ms, code_num=8, objective="logsnr", ms_theta=[th_ms])
+    ap0 = model.antenna_theta.detach().cpu().double().clone()
+    with torch.no_grad():
+        loss = model.postprocess()
+    ap = model.antenna_theta.detach().cpu().double()
+    msq = model.ms_theta[0].detach().cpu().double().reshape(-1)
+    q = ap / (math.pi / 2)
+    assert (q - q.round()).abs().max() < 1e-5
+    assert (ap - ap0).abs().max() <= math.pi / 4 + 1e-5
+    mp = 300 / 180 * math.pi
+    assert msq.min() >= 0 and msq.max() < mp + 1e-5
+    wrapped = torch.remainder(th_ms.double().reshape(-1), mp)
+    diff = (msq - wrapped).abs()
+    assert torch.minimum(diff, mp - diff).max() < 1e-4  # circular: FP32 and FP64 may wrap differently at mp
+    Y = R.received(model.antenna_A.cpu(), ap, Hd, [G], [F], [msq], [mp])
+    lo = R.loss_logsnr(Y, 1, model.n0)
+    rel = abs(loss.item() - lo.item()) / abs(lo.item())
+    _flush(f"  loss {loss.item():.6f} oracle {lo.item():.6f} rel {rel:.1e}")
+    assert rel < 1e-4
