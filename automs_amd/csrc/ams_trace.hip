@@ -1130,7 +1130,7 @@ struct EvalArgs {
   int n_src;
   const float *dst;
   int n0_dst, n1_dst, n_dst, gs, n1g, flags;
-  double wave_len_d;
+  double wave_len_d, inv_wave_len_d;
   float wave_len;
   float src_r2, src_acc2;
   float3 src_dir, dst_dir;
@@ -1202,7 +1202,8 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
       }
       double ex = I.x - (double)r3.x, ey = I.y - (double)r3.y, ez = I.z - (double)r3.z;
       double L = sqrt(ex * ex + ey * ey + ez * ez);
-      float3 d = f3((float)(ex / L), (float)(ey / L), (float)(ez / L));
+      const double il = 1.0 / L;
+      float3 d = f3((float)(ex * il), (float)(ey * il), (float)(ez * il));
       float3 o = r3;
       float A = beam_pattern(d, a.dst_dir, a.dst_bptype);
       float D = 0.0f, Dres = 0.0f, far = 1e6f, psi = 0.0f;
@@ -1278,7 +1279,7 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
             D += bs;
             A *= beam_pattern(d * -1.0f, a.src_dir, a.src_bptype) * a.wave_len /
                  (4.0f * (float)M_PI * (Dres + D));
-            double cyc = L / a.wave_len_d;
+            double cyc = L * a.inv_wave_len_d;
             float ang = (float)(2.0 * M_PI * (cyc - floor(cyc))) - psi;
             if (EXPORT) {  // optixPathTracer.cu:407-412
               x_loss *= a.wave_len / (4.0f * (float)M_PI * (Dres + D));
@@ -2103,6 +2104,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ea.n1g = n1g;
     ea.flags = o->flags;
     ea.wave_len_d = o->wave_len;
+    ea.inv_wave_len_d = 1.0 / o->wave_len;
     ea.wave_len = (float)o->wave_len;
     ea.src_r2 = tx->radius * tx->radius;
     float sacc = std::max(tx->radius, AMS_ACC_MIN);
