@@ -539,6 +539,16 @@ __device__ __forceinline__ void fib_dir(long long k, long long n, double &x, dou
   y = rr * sn;
 }
 
+// Direction table of the Fibonacci ray set in ray-slot order: slot s holds
+// fib_dir(perm[s]) rounded to f32, exactly the direction discovery traces.
+__global__ void k_fib_dirs(long long n, const unsigned *__restrict__ perm, float4 *__restrict__ dirs) {
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double x, y, z;
+  fib_dir(perm[s], n, x, y, z);
+  dirs[s] = make_float4((float)x, (float)y, (float)z, 0.0f);
+}
+
 // Hilbert index of (x, y) on a 2^16 grid.
 __device__ __forceinline__ unsigned hilbert16(unsigned x, unsigned y) {
   unsigned d = 0;
@@ -634,6 +644,7 @@ struct DiscoverArgs {
   int per_element;
   long long ray_begin, ray_end;
   const unsigned *perm;  // slot -> direction (per-element) or ray (distributed) index
+  const float4 *dirs;    // slot -> f32 direction of fib_dir(perm[slot])
   int flags;             // ams_trace_opts.flags
   uint64_t *out_hi, *out_lo;
   unsigned long long *out_count;
@@ -707,22 +718,20 @@ __device__ __forceinline__ void discover_rays(const DiscoverArgs &a, const int r
     const long long r = a.ray_begin + ((long long)blockIdx.x * rays_per_thread + pass) * blockDim.x + threadIdx.x;
     const bool active = r < a.ray_end;
     if (active) {
-      long long e, k;
+      long long e, slot;
       if (a.per_element) {
         e = r / a.fib_n;
-        k = a.perm[r % a.fib_n];
+        slot = r % a.fib_n;
       } else {
-        long long rr2 = a.perm[r];
-        e = rr2 % a.n_src;
-        k = rr2;
+        e = a.perm[r] % a.n_src;
+        slot = r;
       }
-      double dx, dy, dz;
-      fib_dir(k, a.fib_n, dx, dy, dz);
+      const float4 dv = a.dirs[slot];
       Branch st[AMS_MAX_HIT];
       int sp = 0;
       Branch cur;
       cur.o = f3(a.src[3 * e], a.src[3 * e + 1], a.src[3 * e + 2]);
-      cur.d = f3((float)dx, (float)dy, (float)dz);
+      cur.d = f3(dv.x, dv.y, dv.z);
       cur.ttl = a.K + 1;
       cur.b = 0;
       cur.nref = 0;
@@ -1818,10 +1827,15 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     void *tmp = sc.get("perm_tmp", tb);
     if (!tmp) return AMS_ERR_ALLOC;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, k2, idx, perm, (int)n, 0, 32, st));
+    float4 *dirs = (float4 *)sc.get("dirs", sizeof(float4) * (size_t)n);
+    if (!dirs) return AMS_ERR_ALLOC;
+    k_fib_dirs<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, perm, dirs);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     s->perm_n = n;
   }
   da.perm = (const unsigned *)sc.get("perm", 4 * (size_t)o->fib_n);
+  da.dirs = (const float4 *)sc.get("dirs", sizeof(float4) * (size_t)std::max<long long>(1, o->fib_n));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
