@@ -1139,7 +1139,7 @@ struct EvalArgs {
   int n_src;
   const float *dst;
   int n0_dst, n1_dst, n_dst, gs, n1g, flags;
-  double wave_len_d, inv_wave_len_d;
+  double inv_wave_len_d;  // 1 / wave_len in FP64
   float wave_len;
   float src_r2, src_acc2;
   float3 src_dir, dst_dir;
@@ -2117,7 +2117,6 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ea.gs = gs;
     ea.n1g = n1g;
     ea.flags = o->flags;
-    ea.wave_len_d = o->wave_len;
     ea.inv_wave_len_d = 1.0 / o->wave_len;
     ea.wave_len = (float)o->wave_len;
     ea.src_r2 = tx->radius * tx->radius;
