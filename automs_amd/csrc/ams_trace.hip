@@ -53,7 +53,9 @@ struct Ray {
 __device__ __forceinline__ void ray_setup(Ray &r, float3 o, float3 d) {
   r.o = o;
   r.d = d;
-  r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  // hardware reciprocals (1 ulp): the slab test's margin covers their error,
+  // and the shear below needs only to be the same for every triangle
+  r.inv = f3(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
   float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
   int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
   int kx = kz == 2 ? 0 : kz + 1;
@@ -66,10 +68,10 @@ __device__ __forceinline__ void ray_setup(Ray &r, float3 o, float3 d) {
   r.kx = kx;
   r.ky = ky;
   r.kz = kz;
-  float dz = sel3(d, kz);
-  r.sx = sel3(d, kx) / dz;
-  r.sy = sel3(d, ky) / dz;
-  r.sz = 1.0f / dz;
+  const float iz = __builtin_amdgcn_rcpf(sel3(d, kz));
+  r.sx = sel3(d, kx) * iz;
+  r.sy = sel3(d, ky) * iz;
+  r.sz = iz;
 }
 
 // Watertight test; returns true and t if the hit lies in [tmin, tmax].
@@ -107,8 +109,10 @@ __device__ __forceinline__ bool box_test(const Ray &r, float4 lo, float4 hi, flo
   float tz0 = (lo.z - r.o.z) * r.inv.z, tz1 = (hi.z - r.o.z) * r.inv.z;
   float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
   float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-  // conservative margin of 2 gamma(3) on the far distance (Ize 2013)
-  return t0 <= t1 * 1.0000004f;
+  // conservative margin on the far distance (Ize 2013): each distance is off
+  // by at most 4 units of the last place (subtraction, the 1-ulp reciprocal,
+  // the product), so 2 x 4 units plus slack
+  return t0 <= t1 * 1.000001f;
 }
 
 struct Counters {  // per-lane counts of one launch (32 bits per lane suffice)
@@ -164,7 +168,7 @@ __device__ __forceinline__ float box_entry(const Ray &r, float lx, float ly, flo
   float tz0 = (lz - r.o.z) * r.inv.z, tz1 = (hz - r.o.z) * r.inv.z;
   float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
   float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-  return t0 <= t1 * 1.0000004f ? t0 : INFINITY;
+  return t0 <= t1 * 1.000001f ? t0 : INFINITY;
 }
 __device__ __forceinline__ void entry_swap(float &ea, int &ca, float &eb, int &cb) {
   if (eb < ea) {
@@ -336,7 +340,10 @@ struct RxLattice {
 
 __device__ __forceinline__ void lattice_clip(float x0, float dx, float lo, float hi, float &ta, float &tb) {
   if (fabsf(dx) > 1e-30f) {
-    float t1 = (lo - x0) / dx, t2 = (hi - x0) / dx;
+    // a 1-ulp reciprocal: its error moves the clip by far less than the
+    // lattice's 0.1 mm slack
+    const float r = __builtin_amdgcn_rcpf(dx);
+    float t1 = (lo - x0) * r, t2 = (hi - x0) * r;
     ta = fmaxf(ta, fminf(t1, t2));
     tb = fminf(tb, fmaxf(t1, t2));
   } else if (x0 < lo || x0 > hi) {
