@@ -99,6 +99,7 @@ int ams_scene_destroy(ams_scene *s) {
   ams::dfree(s->d_tris);
   ams::dfree(s->d_aux);
   ams::dfree(s->d_planes);
+  ams::dfree(s->d_planes_f);
   ams::dfree(s->d_pbox);
   ams::dfree(s->d_tnodes4);
   ams::dfree(s->d_mat_prop);

@@ -71,6 +71,7 @@ struct ams_scene {
   AmsTri *d_tris = nullptr;
   AmsTriAux *d_aux = nullptr;
   AmsPlane *d_planes = nullptr;
+  float4 *d_planes_f = nullptr;  // the same planes rounded to f32: (n, d)
   AmsPlaneBox *d_pbox = nullptr;
   float *d_mat_prop = nullptr;
   float *d_refl_phase = nullptr;

@@ -1141,6 +1141,7 @@ struct EvalArgs {
   const float *mat_prop;
   const float *refl_phase;
   const AmsPlane *planes;
+  const float4 *planes_f;  // the planes in f32, (n, d)
   const AmsPlaneBox *pbox;
   const float *src;
   int n_src;
@@ -1237,10 +1238,11 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
         bool geo = true;
         for (int q = 0; q < nref && geo; q++) {
           const uint32_t pid = key_get_plane(khi, klo, nref - 1 - q);
-          const AmsPlane P = a.planes[pid];
-          float3 pn = f3((float)P.n[0], (float)P.n[1], (float)P.n[2]);
+          const float4 P = a.planes_f[pid];
+          float3 pn = f3(P.x, P.y, P.z);
           float nd = dot3(pn, pd);
-          float tp = nd != 0.0f ? ((float)P.d - dot3(pn, po)) / nd : -1.0f;
+          // 1-ulp reciprocal: far inside the check's 0.1 mm / 1 mm slack
+          float tp = nd != 0.0f ? (P.w - dot3(pn, po)) * __builtin_amdgcn_rcpf(nd) : -1.0f;
           if (!(tp > -1e-4f)) {
             geo = false;
             break;
@@ -1275,10 +1277,10 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
           tlim = bs;
         } else if ((bm >> (h + 1 - ttl)) & 1) {
           if (ri >= nref) break;
-          const AmsPlane P = a.planes[key_get_plane(khi, klo, nref - 1 - ri)];
-          float3 pn = f3((float)P.n[0], (float)P.n[1], (float)P.n[2]);
+          const float4 P = a.planes_f[key_get_plane(khi, klo, nref - 1 - ri)];
+          float3 pn = f3(P.x, P.y, P.z);
           float nd = dot3(pn, d);
-          float tp = nd != 0.0f ? ((float)P.d - dot3(pn, o)) / nd : -1.0f;
+          float tp = nd != 0.0f ? (P.w - dot3(pn, o)) * __builtin_amdgcn_rcpf(nd) : -1.0f;
           if (!(tp > 0.0f)) break;  // the plane is not ahead
           tlim = tp * 1.001f + 1e-3f;
         }
@@ -1552,6 +1554,9 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
   std::vector<AmsNode4> N4;
   s->depth4 = ams_bvh_collapse4(B, &N4);
   s->n_tnodes4 = (int)N4.size();
+  std::vector<float4> planes_f(planes.size());
+  for (size_t k = 0; k < planes.size(); k++)
+    planes_f[k] = make_float4((float)planes[k].n[0], (float)planes[k].n[1], (float)planes[k].n[2], (float)planes[k].d);
   std::vector<float> mp(19 * (size_t)std::max(n_mat, 1), 0.0f), rp(std::max(n_mat, 1), 0.0f);
   for (int k = 0; k < 19 * n_mat; k++) mp[k] = mat_prop[k];
   for (int k = 0; k < n_mat; k++) rp[k] = refl_phase ? refl_phase[k] : 0.0f;
@@ -1560,6 +1565,7 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
       dev_upload(&s->d_tris, tris.data(), sizeof(AmsTri) * tris.size()) ||
       dev_upload(&s->d_aux, aux.data(), sizeof(AmsTriAux) * aux.size()) ||
       dev_upload(&s->d_planes, planes.data(), sizeof(AmsPlane) * planes.size()) ||
+      dev_upload(&s->d_planes_f, planes_f.data(), sizeof(float4) * planes_f.size()) ||
       dev_upload(&s->d_pbox, pbox.data(), sizeof(AmsPlaneBox) * pbox.size()) ||
       dev_upload(&s->d_mat_prop, mp.data(), sizeof(float) * mp.size()) ||
       dev_upload(&s->d_refl_phase, rp.data(), sizeof(float) * rp.size()))
@@ -2114,6 +2120,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ea.mat_prop = s->d_mat_prop;
     ea.refl_phase = s->d_refl_phase;
     ea.planes = s->d_planes;
+    ea.planes_f = s->d_planes_f;
     ea.pbox = s->d_pbox;
     ea.src = d_src;
     ea.n_src = n_src;
