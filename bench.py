@@ -31,8 +31,8 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # (calibrated against a 1 GiB device copy); the table and the commands that
 # produced it are in profiles/hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("room_boxes_ms", "k_discover"): {"bytes": 214.4e6 + 179.6e6, "l2_hit": 0.945},
-    ("room_boxes_ms", "k_evaluate"): {"bytes": 2.2e6 + 43.0e6, "l2_hit": 0.860},
+    ("room_boxes_ms", "k_discover"): {"bytes": 213.9e6 + 178.4e6, "l2_hit": 0.945},
+    ("room_boxes_ms", "k_evaluate"): {"bytes": 2.2e6 + 43.0e6, "l2_hit": 0.862},
     ("office_batched", "k_cgemm"): {"bytes": 1547.4e6 + 134.2e6, "l2_hit": 0.782, "mfma_util": 0.592},
 }
 
