@@ -359,3 +359,24 @@ def test_surrogate_placements_and_features():
     r = f[5]
     i, j = np.unravel_index(np.argmin(r), r.shape)
     assert i == 63 and abs(j - 32) <= 1
+
+
+def test_task_info_record_layout():
+    """The task export record: the numpy dtype tracer.read_task_info uses has
+    the field offsets of ams_task_info (include/ams.h), the 52-byte layout of
+    the reference's ExportedTaskInfo (optixPathTracer.h:40-52: a bool padded
+    to 4 bytes, four floats, three ints, five ints)."""
+    from automs_amd.tracer import TASK_INFO_DTYPE
+
+    class Rec(C.Structure):
+        _fields_ = [("successful", C.c_uint8), ("pad_", C.c_uint8 * 3), ("launch_angle", C.c_float),
+                    ("receive_angle", C.c_float), ("loss", C.c_float), ("phase", C.c_float), ("src", C.c_int32),
+                    ("dst", C.c_int32), ("hit_cnt", C.c_int32), ("hit_loss_idx", C.c_int32 * 5)]
+    assert C.sizeof(Rec) == TASK_INFO_DTYPE.itemsize == 52
+    for name, *_ in Rec._fields_:
+        assert getattr(Rec, name).offset == TASK_INFO_DTYPE.fields[name][1], name
+    header = (ROOT / "include" / "ams.h").read_text()
+    block = header[header.index("typedef struct {\n  uint8_t successful;"):]
+    block = block[: block.index("} ams_task_info;")]
+    for name, *_ in Rec._fields_:
+        assert name.rstrip("_") in block, name

@@ -322,6 +322,10 @@ def test_trace_many_matches_sequential(gpu):
             for f in ("records_unique", "paths", "tasks", "chains", "successes", "fwd_queries", "bwd_queries"):
                 assert seq[k][1][f] == par[k][1][f], (rep, k, f)
         _flush(f"  rep {rep}: identical counters, H within 1e-6 for {list(tr)}")
+    lim = dict(zip(tr, trace_many([job(k) for k in tr], max_concurrent=2)))  # three jobs on two workers
+    torch.cuda.synchronize()
+    for k in tr:
+        assert lim[k][1]["successes"] == seq[k][1]["successes"], k
     with pytest.raises(ValueError):
         trace_many([job("G"), job("G")])
 
