@@ -33,6 +33,8 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 MEASURED_TRAFFIC = {
     ("room_boxes_ms", "k_discover"): {"bytes": 213.9e6 + 178.4e6, "l2_hit": 0.945},
     ("room_boxes_ms", "k_evaluate"): {"bytes": 2.2e6 + 43.0e6, "l2_hit": 0.862},
+    ("office", "k_discover"): {"bytes": 523.8e6 + 283.4e6, "l2_hit": 0.926},
+    ("office", "k_evaluate"): {"bytes": 8.9e6 + 255.2e6, "l2_hit": 0.953},
     ("office_batched", "k_cgemm"): {"bytes": 1547.4e6 + 134.2e6, "l2_hit": 0.782, "mfma_util": 0.592},
 }
 
@@ -51,6 +53,9 @@ def workload_traces(name: str):
         return {"H": scenes.room_boxes()}
     if name == "room_boxes_ms":
         return scenes.room_boxes_ms()
+    if name == "office":  # 100,046 triangles, 5 reflections: H_d, G0, F0 of one metasurface, 2^24 rays each
+        tr = scenes.office()
+        return {k: tr[k] for k in ("H_d", "G0", "F0")}
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -58,6 +63,16 @@ def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int, office: bool = 
     """FP64 oracle (OpenMP) on the same scene with fewer rays per trace."""
     from automs_amd import scenes
     from oracle import oracle as O
+    if name == "office":
+        w = scenes.office(fib_n=1 << 10, ms_fib_n=1 << 12)["H_d"]
+        t0 = time.perf_counter()
+        _, st, _ = O.trace_workload(w)
+        dt = time.perf_counter() - t0
+        q = st["fwd_queries"] + st["bwd_queries"]
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        return {"value": q / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+                "sample": f"office H_d ({w.scene.n_tri} triangles, 64 x 65,536) with {w.fib_n} discovery rays "
+                          f"({q} closest-hit queries in {dt:.1f} s; FP64 oracle, OpenMP)"}
     if name == "room_boxes":
         traces = {"H": scenes.room_boxes(fib_n=sample_fib)}
     else:
@@ -182,7 +197,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="room_boxes_ms", choices=["room_boxes", "room_boxes_ms"])
+    ap.add_argument("--workload", default="room_boxes_ms", choices=["room_boxes", "room_boxes_ms", "office"])
     ap.add_argument("--opt-steps", type=int, default=500)
     ap.add_argument("--no-opt", action="store_true")
     ap.add_argument("--sequential", action="store_true",
