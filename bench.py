@@ -167,6 +167,19 @@ def contraction_office(iters: int = 20) -> dict:
         ref = A_h.to(torch.complex128) @ Fs.to(torch.complex128)
         err = (Y[:, cols].cpu().to(torch.complex128) - ref).abs().pow(2).sum(0).sqrt() / \
             ref.abs().pow(2).sum(0).sqrt().clamp_min(1e-300)
+        # the same product with the GPU's FP32 library GEMM (complex64 matmul)
+        Ag = A.to(torch.complex64).contiguous()
+        Yl = Ag @ Fcat
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(iters):
+            Yl = Ag @ Fcat
+        e1.record()
+        torch.cuda.synchronize()
+        t_lib = e0.elapsed_time(e1) / iters * 1e-3
+        err_lib = (Yl[:, cols].cpu().to(torch.complex128) - ref).abs().pow(2).sum(0).sqrt() / \
+            ref.abs().pow(2).sum(0).sqrt().clamp_min(1e-300)
+        del Yl
         Fcpu = Fcat[:, : min(N, 8192)].cpu().contiguous()
         A_h @ Fcpu
         t0 = time.perf_counter()
@@ -179,8 +192,11 @@ def contraction_office(iters: int = 20) -> dict:
     host = {"seconds_per_contraction": t_cpu, "threads": torch.get_num_threads(),
             "kind": "torch complex64 matmul on the host (FP32), timed on 8192 Rx columns and scaled to N",
             "gpu_speedup": t_cpu / t_fwd}
+    lib = {"ms": t_lib * 1e3, "per_rx_rel_err_max": float(err_lib.max()), "speedup_of_mfma_path": t_lib / t_fwd,
+           "kind": "torch complex64 matmul on the GPU (the FP32 library GEMM), same operands"}
     return {"shape": {"M": M, "K": Kc, "N": N}, "fwd_gemm_ms": t_fwd * 1e3,
             "rx_channels_per_s": M * N / t_fwd, "parity_vs_fp64": parity, "host_baseline": host,
+            "gpu_fp32_library_gemm": lib,
             "roofline": {"bound": "mfma", "achieved": 3 * useful / t_fwd / 1e12, "peak": MFMA_BF16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": 3 * useful / t_fwd / 2.5e15, "traffic": tr_meas["bytes"],
                          "mfma_busy_pmc": tr_meas["mfma_util"]},
