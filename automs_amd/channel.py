@@ -272,7 +272,18 @@ def build_channels(src: torch.Tensor, dst: torch.Tensor, ms_list, ap_dir, scene_
 
     names = list(spec)
     with ThreadPoolExecutor(max_workers=min(8, len(names))) as ex:
-        handles = list(ex.map(lambda k: TraceScene(spec[k][0]), names))
+        futures = [ex.submit(TraceScene, spec[k][0]) for k in names]
+    handles = []
+    err = None
+    for f in futures:  # keep every handle that was built, so that all of them are closed
+        try:
+            handles.append(f.result())
+        except BaseException as e:
+            err = err or e
+    if err is not None:
+        for h in handles:
+            h.close()
+        raise err
     try:
         res = trace_many([dict(ts=h, **spec[k][1]) for k, h in zip(names, handles)], max_concurrent=4)
     finally:
