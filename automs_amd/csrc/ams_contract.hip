@@ -16,6 +16,7 @@
 // contiguous along the reduction index.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
 #include <string>
 
@@ -307,7 +308,10 @@ __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, c
   }
 }
 
-// Ordered sum of split partials: C = sum_z part[z]  (deterministic).
+// Ordered sum of split partials: C = sum_z part[z]  (deterministic), the
+// splits' loads issued four at a time.  k_reduce_parts2 takes two complex
+// elements per lane (16-byte loads) when rows have even length; both add
+// the splits in the same order, so their sums are identical.
 __global__ void k_reduce_parts(const float *__restrict__ part, int splits, int Ra, int Rb, float *__restrict__ C,
                                long long ldc) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -315,6 +319,7 @@ __global__ void k_reduce_parts(const float *__restrict__ part, int splits, int R
   if (t >= n) return;
   int row = (int)(t / Rb), col = (int)(t % Rb);
   float re = 0.0f, im = 0.0f;
+#pragma unroll 4
   for (int z = 0; z < splits; z++) {
     const float *p = part + (long long)z * 2 * n + 2 * t;
     re += p[0];
@@ -322,6 +327,25 @@ __global__ void k_reduce_parts(const float *__restrict__ part, int splits, int R
   }
   C[2 * ((long long)row * ldc + col)] = re;
   C[2 * ((long long)row * ldc + col) + 1] = im;
+}
+
+__global__ void k_reduce_parts2(const float *__restrict__ part, int splits, int Ra, int Rb, float *__restrict__ C,
+                                long long ldc) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // pair index
+  long long n = (long long)Ra * Rb;
+  if (2 * t >= n) return;
+  long long e = 2 * t;
+  int row = (int)(e / Rb), col = (int)(e % Rb);
+  float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 4
+  for (int z = 0; z < splits; z++) {
+    float4 p = *(const float4 *)(part + (long long)z * 2 * n + 2 * e);
+    s.x += p.x;
+    s.y += p.y;
+    s.z += p.z;
+    s.w += p.w;
+  }
+  *(float4 *)(C + 2 * ((long long)row * ldc + col)) = s;
 }
 
 // Plain FP32 complex GEMM C = A . B^T on the vector ALU (comparison path).
@@ -447,7 +471,13 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && splits > 1) {
     long long n = (long long)Ra * Rb;
-    k_reduce_parts<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, Ra, Rb, C, ldc);
+    // pairs stay in one row and 16-byte aligned when Rb and ldc are even
+    // (the workspace and C start 16-byte aligned)
+    bool pairs = (Rb % 2 == 0) && (ldc % 2 == 0) && ((uintptr_t)C % 16 == 0) && ((uintptr_t)part % 16 == 0);
+    if (pairs)
+      k_reduce_parts2<<<(unsigned)((n / 2 + 255) / 256), 256, 0, st>>>(part, splits, Ra, Rb, C, ldc);
+    else
+      k_reduce_parts<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, Ra, Rb, C, ldc);
     e = hipGetLastError();
   }
   if (e != hipSuccess) {

@@ -38,16 +38,20 @@ __device__ __forceinline__ float cap_of(float2 y, float n0) {
   return log2f(1.0f + a * a / n0);
 }
 
-// partial[g][block] = (S1, S2): mode 0: S1 = sum log2(1+max SNR); mode 1:
-// S1 = sum max cap', S2 = sum of squares of non-max cap'.
+// Coverage side output of one Rx: 10 log10 of the best code's power.
+__device__ __forceinline__ double power_db(float p) { return (double)(10.0f * log10f(p + 1e-30f)); }
+
+// partial[g][block] = (S1, S2, S3): mode 0: S1 = sum log2(1+max SNR); mode 1:
+// S1 = sum max cap', S2 = sum of squares of non-max cap'.  S3 (when
+// want_cov) = sum of power_db(max_c |y|^2).
 __global__ void __launch_bounds__(256) k_obj_fwd(const float2 *__restrict__ Y, int C, int N, float n0, int mode,
-                                                 const unsigned char *__restrict__ rowmask,
+                                                 const unsigned char *__restrict__ rowmask, int want_cov,
                                                  double *__restrict__ partial, int *__restrict__ argmax) {
   __shared__ double sh[8];
   int g = blockIdx.y;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   const float2 *Yg = Y + (size_t)g * C * N;
-  double s1 = 0.0, s2 = 0.0;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
   if (r < N) {
     if (mode == 0) {
       float best = -1.0f;
@@ -61,17 +65,21 @@ __global__ void __launch_bounds__(256) k_obj_fwd(const float2 *__restrict__ Y, i
         }
       }
       s1 = (double)log2f(1.0f + best / n0);
+      if (want_cov) s3 = power_db(best);
       if (argmax) argmax[(size_t)g * N + r] = bi;
     } else {
-      float best = -1.0f;
+      float best = -1.0f, pmax = 0.0f;
       int bi = 0;
       for (int c = 0; c < C; c++) {
-        float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
+        float2 y = Yg[(size_t)c * N + r];
+        pmax = fmaxf(pmax, y.x * y.x + y.y * y.y);
+        float v = rowmask && rowmask[c] ? 0.0f : cap_of(y, n0);
         if (v > best) {
           best = v;
           bi = c;
         }
       }
+      if (want_cov) s3 = power_db(pmax);
       for (int c = 0; c < C; c++) {
         float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
         if (v != best) s2 += (double)v * v;
@@ -82,26 +90,32 @@ __global__ void __launch_bounds__(256) k_obj_fwd(const float2 *__restrict__ Y, i
   }
   double t1 = block_sum(s1, sh);
   double t2 = block_sum(s2, sh);
+  double t3 = want_cov ? block_sum(s3, sh) : 0.0;
   if (threadIdx.x == 0) {
-    partial[2 * ((size_t)g * gridDim.x + blockIdx.x)] = t1;
-    partial[2 * ((size_t)g * gridDim.x + blockIdx.x) + 1] = t2;
+    double *p = partial + 3 * ((size_t)g * gridDim.x + blockIdx.x);
+    p[0] = t1;
+    p[1] = t2;
+    p[2] = t3;
   }
 }
 
-// loss[g] from the partials, in block order (deterministic).
+// loss[g] (and cov[g]) from the partials, in block order (deterministic).
 __global__ void k_obj_finish(const double *__restrict__ partial, int nb, int N, int mode, float w,
-                             double *__restrict__ sums, float *__restrict__ loss) {
+                             double *__restrict__ sums, float *__restrict__ loss, float *__restrict__ cov) {
   int g = blockIdx.x;
   if (threadIdx.x != 0) return;
-  double s1 = 0.0, s2 = 0.0;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (int b = 0; b < nb; b++) {
-    s1 += partial[2 * ((size_t)g * nb + b)];
-    s2 += partial[2 * ((size_t)g * nb + b) + 1];
+    const double *p = partial + 3 * ((size_t)g * nb + b);
+    s1 += p[0];
+    s2 += p[1];
+    s3 += p[2];
   }
   sums[2 * g] = s1;
   sums[2 * g + 1] = s2;
   if (mode == 0) loss[g] = (float)(-s1 / N);
   else loss[g] = (float)(-(double)w * s1 + sqrt(s2));
+  if (cov) cov[g] = (float)(s3 / N);
 }
 
 __global__ void __launch_bounds__(256) k_obj_bwd(const float2 *__restrict__ Y, int C, int N, float n0, int mode,
@@ -170,19 +184,74 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
   p[i] = p[i] - (lr / bc1) * (mi / denom);
 }
 
+// Device-side state of the fused Adam (AMS_ADAM_STATE_BYTES): the step
+// count, the two bias corrections of the current step, and the best-loss
+// tracking of go.py:320-327 (the loss of the parameters before the update
+// is compared with the best so far; when lower, those parameters are kept).
+struct AdamState {
+  int32_t step;
+  float bc1;
+  float bc2_sqrt;
+  int32_t better;
+  float best_loss;
+  int32_t best_valid;
+  int32_t pad[2];
+};
+
+// One lane: the count, the bias corrections (the arithmetic of the host path:
+// double pow, rounded to float once), the best-loss decision and the history.
+__global__ void k_adam_count(AdamState *st, float b1, float b2, const float *loss, float *hist, long long hist_cap) {
+  if (threadIdx.x != 0) return;
+  int32_t t = st->step;
+  if (loss) {
+    float l = *loss;
+    float best = st->best_valid ? st->best_loss : INFINITY;
+    int better = l < best;
+    st->better = better;
+    if (better) {
+      st->best_loss = l;
+      st->best_valid = 1;
+    }
+    if (hist && t < hist_cap) hist[t] = l;
+  } else {
+    st->better = 0;
+  }
+  t += 1;
+  st->step = t;
+  st->bc1 = (float)(1.0 - pow((double)b1, (double)t));
+  st->bc2_sqrt = (float)sqrt(1.0 - pow((double)b2, (double)t));
+}
+
+__global__ void k_adam_dev(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                           float *__restrict__ v, long long n, float lr, float b1, float b2, float eps,
+                           const AdamState *__restrict__ st, float *__restrict__ best_p) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float bc1 = st->bc1, bc2_sqrt = st->bc2_sqrt;
+  float pi = p[i];
+  if (best_p && st->better) best_p[i] = pi;
+  float gi = g[i];
+  float mi = m[i] + (1.0f - b1) * (gi - m[i]);
+  float vi = v[i] * b2 + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float denom = sqrtf(vi) / bc2_sqrt + eps;
+  p[i] = pi - (lr / bc1) * (mi / denom);
+}
+
 }  // namespace
 
 extern "C" {
 
 int64_t ams_objective_workspace_bytes(int32_t G, int32_t N) {
   int64_t nb = (N + 255) / 256;
-  return (int64_t)G * nb * 16 + (int64_t)G * 16 + (int64_t)G * N * 4;
+  return (int64_t)G * nb * 24 + (int64_t)G * 16 + (int64_t)G * N * 4;
 }
 
-// Y complex f32 [G][C][N]; loss f32 [G]; workspace from
+// Y complex f32 [G][C][N]; loss f32 [G]; cov f32 [G] or null; workspace from
 // ams_objective_workspace_bytes (keeps argmax and sums for the backward).
-int ams_objective_fwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
-                      const unsigned char *rowmask, float *loss, void *workspace, void *stream) {
+int ams_objective_fwd_cov(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                          const unsigned char *rowmask, float *loss, float *cov, void *workspace, void *stream) {
   if (!Y || !loss || !workspace || G <= 0 || C <= 0 || N <= 0 || (mode != 0 && mode != 1) || !(n0 > 0)) {
     ams::set_error("ams_objective_fwd: bad arguments");
     return AMS_ERR_ARG;
@@ -190,16 +259,21 @@ int ams_objective_fwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t m
   hipStream_t st = (hipStream_t)stream;
   int nb = (N + 255) / 256;
   double *partial = (double *)workspace;
-  double *sums = partial + 2 * (size_t)G * nb;
+  double *sums = partial + 3 * (size_t)G * nb;
   int *argmax = (int *)(sums + 2 * (size_t)G);
-  k_obj_fwd<<<dim3(nb, G), 256, 0, st>>>((const float2 *)Y, C, N, n0, mode, rowmask, partial, argmax);
-  k_obj_finish<<<G, 64, 0, st>>>(partial, nb, N, mode, w, sums, loss);
+  k_obj_fwd<<<dim3(nb, G), 256, 0, st>>>((const float2 *)Y, C, N, n0, mode, rowmask, cov ? 1 : 0, partial, argmax);
+  k_obj_finish<<<G, 64, 0, st>>>(partial, nb, N, mode, w, sums, loss, cov);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ams::set_error(std::string("ams_objective_fwd: ") + hipGetErrorString(e));
     return AMS_ERR_HIP;
   }
   return AMS_OK;
+}
+
+int ams_objective_fwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                      const unsigned char *rowmask, float *loss, void *workspace, void *stream) {
+  return ams_objective_fwd_cov(Y, G, C, N, mode, n0, w, rowmask, loss, nullptr, workspace, stream);
 }
 
 // gY complex f32 [G][C][N] = gloss[g] * dL_g/dY (workspace of the forward).
@@ -213,7 +287,7 @@ int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t m
   hipStream_t st = (hipStream_t)stream;
   int nb = (N + 255) / 256;
   const double *partial = (const double *)workspace;
-  const double *sums = partial + 2 * (size_t)G * nb;
+  const double *sums = partial + 3 * (size_t)G * nb;
   const int *argmax = (const int *)(sums + 2 * (size_t)G);
   k_obj_bwd<<<dim3(nb, G), 256, 0, st>>>((const float2 *)Y, C, N, n0, mode, w, rowmask, argmax, sums, gloss,
                                          (float2 *)gY);
@@ -238,6 +312,28 @@ int ams_adam_step(float *param, const float *grad, float *m, float *v, int64_t n
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ams::set_error(std::string("ams_adam_step: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_adam_step_dev(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
+                      float beta2, float eps, void *step_state, const float *loss, float *best_param,
+                      float *loss_hist, int64_t hist_cap, void *stream) {
+  if (!param || !grad || !m || !v || !step_state || n <= 0 || (best_param && !loss) || (loss_hist && !loss) ||
+      hist_cap < 0) {
+    ams::set_error("ams_adam_step_dev: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  static_assert(sizeof(AdamState) == AMS_ADAM_STATE_BYTES, "AdamState layout");
+  hipStream_t st = (hipStream_t)stream;
+  AdamState *as = (AdamState *)step_state;
+  k_adam_count<<<1, 64, 0, st>>>(as, beta1, beta2, loss, loss_hist, (long long)hist_cap);
+  k_adam_dev<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(param, grad, m, v, n, lr, beta1, beta2, eps, as,
+                                                          best_param);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_adam_step_dev: ") + hipGetErrorString(e));
     return AMS_ERR_HIP;
   }
   return AMS_OK;

@@ -1,0 +1,255 @@
+// Fused phase-map operand of one metasurface with single-bounce chains, and
+// its backward (go.py:107-163 for one metasurface, max_ms_reflex = 1).
+//
+// Rows r = m*Cn + c (map m, code c) of the stacked operand A [nm*Cn][K+M]:
+//   A[r][k]     = WG[c][k] * u[m][k]          k < K,   WG = W . G
+//   A[r][K + j] = W[c][j]
+//   W[c][j] = amp[c][j] e^{-i th_ap[c][j]}          (go.py:163)
+//   u[m][k] = ms_amp[k] e^{i rem(th_ms[m][k], P)}   (go.py:131, P = max phase)
+// so that y = A . [F ; H_d] is the received signal of every (map, code).
+//
+// The backward takes gA in torch's convention for complex tensors (the
+// gradient with respect to the conjugate, as torch's autograd hands it to a
+// complex input) and returns the gradients of the real phases:
+//   gu[m][k]  = sum_c gA[r][k] conj(WG[c][k])        g_ms = Im(conj(u) gu)
+//   gWG[c][k] = sum_m gA[r][k] conj(u[m][k])
+//   gW[c][j]  = sum_m gA[r][K+j] + sum_k gWG[c][k] conj(G[j][k])
+//   g_ap      = -Im(conj(W) gW)
+// (d u / d th = i u and d W / d th = -i W; the remainder passes the gradient
+// through unchanged, as torch.remainder's backward does).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/ams.h"
+#include "ams_internal.h"
+
+namespace {
+
+constexpr int kPmBlock = 256;
+constexpr int kPmMaxM = 4096;  // AP elements (LDS row of W: 32 KB)
+constexpr int kPmJ = 16;       // AP elements per pass of the gW reduction
+constexpr int kPmApBlock = 1024;
+constexpr int kPmMsK = 64;  // metasurface elements per workgroup of k_pm_bwd_ms
+constexpr int kPmMsC = 4;   // code groups (waves) per workgroup of k_pm_bwd_ms
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// torch.remainder for floats: the sign of the divisor
+__device__ __forceinline__ float remainder_like_torch(float a, float b) {
+  float m = fmodf(a, b);
+  if (m != 0.0f && ((b < 0.0f) != (m < 0.0f))) m += b;
+  return m;
+}
+
+__device__ __forceinline__ float2 ap_weight(float amp, float th) {
+  float s, c;
+  sincosf(th, &s, &c);
+  return make_float2(amp * c, -amp * s);
+}
+
+__device__ __forceinline__ float2 ms_coef(float a, float th, float max_phase) {
+  float s, c;
+  sincosf(remainder_like_torch(th, max_phase), &s, &c);
+  return make_float2(a * c, a * s);
+}
+
+// grid (ceil(K / 256), Cn): a workgroup holds W[c][:] in LDS and forms
+// WG[c][k] for 256 elements k, then A for every map.  Workgroups of column
+// block 0 also write the W columns of A.
+__global__ void __launch_bounds__(kPmBlock) k_pm_operand(const float *__restrict__ th_ap,
+                                                         const float *__restrict__ amp,
+                                                         const float *__restrict__ th_ms,
+                                                         const float *__restrict__ ms_amp,
+                                                         const float2 *__restrict__ G, int Cn, int M, int K, int nm,
+                                                         float max_phase, float2 *__restrict__ A,
+                                                         float2 *__restrict__ WG, float2 *__restrict__ U) {
+  extern __shared__ float2 w_row[];
+  const int c = blockIdx.y;
+  const long long Kc = (long long)K + M;
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    float2 w = ap_weight(amp[(size_t)c * M + j], th_ap[(size_t)c * M + j]);
+    w_row[j] = w;
+    if (blockIdx.x == 0)
+      for (int m = 0; m < nm; m++) A[(size_t)(m * Cn + c) * Kc + K + j] = w;
+  }
+  __syncthreads();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  float2 acc = make_float2(0.0f, 0.0f);
+  for (int j = 0; j < M; j++) {
+    float2 w = w_row[j], g = G[(size_t)j * K + k];
+    acc.x += w.x * g.x - w.y * g.y;
+    acc.y += w.x * g.y + w.y * g.x;
+  }
+  WG[(size_t)c * K + k] = acc;
+  const float a = ms_amp[k];
+  for (int m = 0; m < nm; m++) {
+    float2 u = ms_coef(a, th_ms[(size_t)m * K + k], max_phase);
+    if (c == 0) U[(size_t)m * K + k] = u;
+    A[(size_t)(m * Cn + c) * Kc + k] = cmul(acc, u);
+  }
+}
+
+// Workgroup = 64 metasurface elements k (one per lane) x 4 waves, wave w
+// summing codes c = w, w+4, ...; the four partial sums are added in wave
+// order.  g_ms for every map, then gWG when the AP phases need a gradient.
+__global__ void __launch_bounds__(kPmMsK *kPmMsC) k_pm_bwd_ms(const float2 *__restrict__ gA,
+                                                              const float2 *__restrict__ WG,
+                                                              const float2 *__restrict__ U, int Cn, int M, int K,
+                                                              int nm, float *__restrict__ g_ms,
+                                                              float2 *__restrict__ gWG) {
+  __shared__ float2 red[kPmMsC][kPmMsK];
+  const int kl = threadIdx.x % kPmMsK, cg = threadIdx.x / kPmMsK;
+  const int k = blockIdx.x * kPmMsK + kl;
+  const bool ok = k < K;
+  const long long Kc = (long long)K + M;
+  for (int m = 0; m < nm; m++) {
+    float2 gu = make_float2(0.0f, 0.0f);
+    if (ok)
+      for (int c = cg; c < Cn; c += kPmMsC) {
+        float2 t = cmulc(gA[(size_t)(m * Cn + c) * Kc + k], WG[(size_t)c * K + k]);
+        gu.x += t.x;
+        gu.y += t.y;
+      }
+    red[cg][kl] = gu;
+    __syncthreads();
+    if (cg == 0 && ok) {
+      float2 s = red[0][kl];
+      for (int q = 1; q < kPmMsC; q++) {
+        s.x += red[q][kl].x;
+        s.y += red[q][kl].y;
+      }
+      float2 u = U[(size_t)m * K + k];
+      g_ms[(size_t)m * K + k] = u.x * s.y - u.y * s.x;
+    }
+    __syncthreads();
+  }
+  if (!gWG || !ok) return;
+  for (int c = cg; c < Cn; c += kPmMsC) {
+    float2 s = make_float2(0.0f, 0.0f);
+    for (int m = 0; m < nm; m++) {
+      float2 t = cmulc(gA[(size_t)(m * Cn + c) * Kc + k], U[(size_t)m * K + k]);
+      s.x += t.x;
+      s.y += t.y;
+    }
+    gWG[(size_t)c * K + k] = s;
+  }
+}
+
+// One workgroup of 1024 lanes per code c: gW[c][j] = sum_k gWG[c][k]
+// conj(G[j][k]) (kPmJ elements j per pass, a wave reduction then an ordered
+// sum over the 16 waves), plus the direct columns, then g_ap[c][j].
+__global__ void __launch_bounds__(kPmApBlock) k_pm_bwd_ap(const float2 *__restrict__ gA,
+                                                          const float2 *__restrict__ gWG,
+                                                          const float2 *__restrict__ G,
+                                                          const float *__restrict__ th_ap,
+                                                          const float *__restrict__ amp, int Cn, int M, int K,
+                                                          int nm, float *__restrict__ g_ap) {
+  __shared__ float2 red[kPmApBlock / 64][kPmJ];
+  const int c = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long Kc = (long long)K + M;
+  for (int j0 = 0; j0 < M; j0 += kPmJ) {
+    float2 acc[kPmJ];
+#pragma unroll
+    for (int q = 0; q < kPmJ; q++) acc[q] = make_float2(0.0f, 0.0f);
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+      float2 gw = gWG[(size_t)c * K + k];
+#pragma unroll
+      for (int q = 0; q < kPmJ; q++) {
+        if (j0 + q < M) {
+          float2 t = cmulc(gw, G[(size_t)(j0 + q) * K + k]);
+          acc[q].x += t.x;
+          acc[q].y += t.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPmJ; q++) {
+      float x = acc[q].x, y = acc[q].y;
+      for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_down(x, off);
+        y += __shfl_down(y, off);
+      }
+      if (lane == 0) red[wave][q] = make_float2(x, y);
+    }
+    __syncthreads();
+    if (threadIdx.x < kPmJ && j0 + (int)threadIdx.x < M) {
+      const int j = j0 + threadIdx.x;
+      float2 s = make_float2(0.0f, 0.0f);
+      for (int w = 0; w < kPmApBlock / 64; w++) {
+        s.x += red[w][threadIdx.x].x;
+        s.y += red[w][threadIdx.x].y;
+      }
+      for (int m = 0; m < nm; m++) {
+        float2 d = gA[(size_t)(m * Cn + c) * Kc + K + j];
+        s.x += d.x;
+        s.y += d.y;
+      }
+      float2 w = ap_weight(amp[(size_t)c * M + j], th_ap[(size_t)c * M + j]);
+      g_ap[(size_t)c * M + j] = -(w.x * s.y - w.y * s.x);
+    }
+    __syncthreads();
+  }
+}
+
+bool pm_dims_ok(int32_t Cn, int32_t M, int32_t K, int32_t nm) {
+  return Cn > 0 && M > 0 && M <= kPmMaxM && K > 0 && nm > 0 && (long long)nm * Cn <= (1LL << 20) &&
+         (long long)K <= (1LL << 24);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ams_pm_operand(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp, const float *G,
+                   int32_t Cn, int32_t M, int32_t K, int32_t nm, float max_phase, float *A, float *WG, float *U,
+                   void *stream) {
+  if (!th_ap || !amp || !th_ms || !ms_amp || !G || !A || !WG || !U || !pm_dims_ok(Cn, M, K, nm) ||
+      !(max_phase > 0.0f)) {
+    ams::set_error("ams_pm_operand: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  dim3 grid((unsigned)((K + kPmBlock - 1) / kPmBlock), (unsigned)Cn);
+  k_pm_operand<<<grid, kPmBlock, (size_t)M * sizeof(float2), (hipStream_t)stream>>>(
+      th_ap, amp, th_ms, ms_amp, (const float2 *)G, Cn, M, K, nm, max_phase, (float2 *)A, (float2 *)WG,
+      (float2 *)U);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_pm_operand: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_pm_operand_bwd(const float *gA, const float *th_ap, const float *amp, const float *G, const float *WG,
+                       const float *U, int32_t Cn, int32_t M, int32_t K, int32_t nm, float *g_ms, float *g_ap,
+                       float *gWG, void *stream) {
+  if (!gA || !WG || !U || !g_ms || !pm_dims_ok(Cn, M, K, nm) || (g_ap && (!th_ap || !amp || !G || !gWG))) {
+    ams::set_error("ams_pm_operand_bwd: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  k_pm_bwd_ms<<<(unsigned)((K + kPmMsK - 1) / kPmMsK), kPmMsK * kPmMsC, 0, st>>>(
+      (const float2 *)gA, (const float2 *)WG, (const float2 *)U, Cn, M, K, nm, g_ms, g_ap ? (float2 *)gWG : nullptr);
+  if (g_ap)
+    k_pm_bwd_ap<<<(unsigned)Cn, kPmApBlock, 0, st>>>((const float2 *)gA, (const float2 *)gWG, (const float2 *)G,
+                                                    th_ap, amp, Cn, M, K, nm, g_ap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_pm_operand_bwd: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+}  // extern "C"

@@ -97,6 +97,14 @@ class ChannelOperand:
         return self.frag_fwd.numel() + self.frag_bwd.numel()
 
 
+def split_count(Ra: int, Rb: int, Kd: int, cap: int = 64) -> int:
+    """Reduction splits that give a short-and-wide product about 1024 waves
+    of 32 x 64 output tiles (a few rows of codes against thousands of Rx
+    leave the GPU's 256 CUs idle otherwise); 1 when the tiles alone do."""
+    tiles = ((Ra + 31) // 32) * ((Rb + 63) // 64)
+    return max(1, min(cap, (Kd + 31) // 32, 1024 // max(1, tiles)))
+
+
 class Contract(torch.autograd.Function):
     """Y = A . [F ; H_d]  (A: [R, Kc] complex64) with the split-bf16 MFMA GEMM;
     backward gA = gY . [F ; H_d]^H on the same kernels."""
@@ -107,7 +115,7 @@ class Contract(torch.autograd.Function):
         R = A.shape[0]
         fa = frag_pack(A, R, op.Kc, op.Kc, 1)
         ctx.op = op
-        return cgemm_frag(fa, op.frag_fwd, R, op.N, op.Kc, conj_b=False)
+        return cgemm_frag(fa, op.frag_fwd, R, op.N, op.Kc, conj_b=False, splits=split_count(R, op.N, op.Kc))
 
     @staticmethod
     def backward(ctx, gY: torch.Tensor):
@@ -115,8 +123,51 @@ class Contract(torch.autograd.Function):
         gY = gY.contiguous()
         R = gY.shape[0]
         fg = frag_pack(gY, R, op.N, op.N, 1)
-        gA = cgemm_frag(fg, op.frag_bwd, R, op.Kc, op.N, conj_b=True, splits=op.bwd_splits)
+        splits = max(op.bwd_splits, split_count(R, op.Kc, op.N))
+        gA = cgemm_frag(fg, op.frag_bwd, R, op.Kc, op.N, conj_b=True, splits=splits)
         return gA, None
+
+
+class PhaseOperand(torch.autograd.Function):
+    """The stacked operand A [n_maps * C, K + M] of one metasurface with
+    single-bounce chains (go.py:107-163 with max_ms_reflex = 1) from the
+    phases, in one kernel (ams_pm_operand); the backward turns gA into the
+    phase gradients in two (ams_pm_operand_bwd)."""
+
+    @staticmethod
+    def forward(ctx, th_ap: torch.Tensor, th_ms: torch.Tensor, amp: torch.Tensor, ms_amp: torch.Tensor,
+                G: torch.Tensor, max_phase: float):
+        L = _lib.lib()
+        Cn, M = th_ap.shape
+        nm, K = th_ms.shape
+        dev = th_ap.device
+        th_ap, th_ms = th_ap.contiguous(), th_ms.contiguous()
+        A = torch.empty(nm * Cn, K + M, dtype=torch.complex64, device=dev)
+        WG = torch.empty(Cn, K, dtype=torch.complex64, device=dev)
+        U = torch.empty(nm, K, dtype=torch.complex64, device=dev)
+        _lib.check(L.ams_pm_operand(_ptr(th_ap), _ptr(amp), _ptr(th_ms), _ptr(ms_amp), _ptr(torch.view_as_real(G)),
+                                    Cn, M, K, nm, max_phase, _ptr(A), _ptr(WG), _ptr(U), _stream()),
+                   "ams_pm_operand")
+        ctx.save_for_backward(th_ap, amp, G, WG, U)
+        ctx.dims = (Cn, M, K, nm)
+        return A
+
+    @staticmethod
+    def backward(ctx, gA: torch.Tensor):
+        L = _lib.lib()
+        th_ap, amp, G, WG, U = ctx.saved_tensors
+        Cn, M, K, nm = ctx.dims
+        gA = gA.contiguous()
+        dev = gA.device
+        g_ms = torch.empty(nm, K, dtype=torch.float32, device=dev)
+        need_ap = ctx.needs_input_grad[0]
+        g_ap = torch.empty(Cn, M, dtype=torch.float32, device=dev) if need_ap else None
+        gWG = torch.empty(Cn, K, dtype=torch.complex64, device=dev) if need_ap else None
+        _lib.check(L.ams_pm_operand_bwd(_ptr(gA), _ptr(th_ap), _ptr(amp), _ptr(torch.view_as_real(G)), _ptr(WG),
+                                        _ptr(U), Cn, M, K, nm, _ptr(g_ms), _ptr(g_ap) if need_ap else None,
+                                        _ptr(gWG) if need_ap else None, _stream()),
+                   "ams_pm_operand_bwd")
+        return g_ap, g_ms, None, None, None, None
 
 
 class ContractPlain(torch.autograd.Function):
@@ -135,7 +186,9 @@ class ContractPlain(torch.autograd.Function):
 
 class Objective(torch.autograd.Function):
     """Coverage objective over Y [G*C, N] (G groups of C codes): returns the
-    per-group loss [G].  mode 0: -mean_r log2(1 + max_c |y|^2 / n0);
+    per-group loss [G] and, as a side output without gradient, the per-group
+    coverage [G] (mean over Rx of the best code's power in dB).
+    mode 0: -mean_r log2(1 + max_c |y|^2 / n0);
     mode 1: the reference capacity loss (go.py:214-225)."""
 
     @staticmethod
@@ -147,16 +200,22 @@ class Objective(torch.autograd.Function):
         Cc = GC // groups
         ws = torch.empty(L.ams_objective_workspace_bytes(groups, N), dtype=torch.uint8, device=Y.device)
         loss = torch.empty(groups, dtype=torch.float32, device=Y.device)
+        cov = torch.empty(groups, dtype=torch.float32, device=Y.device)
         rm = rowmask.to(device=Y.device, dtype=torch.uint8).contiguous() if rowmask is not None else None
-        _lib.check(L.ams_objective_fwd(_ptr(Y), groups, Cc, N, mode, n0, w,
-                                       _ptr(rm) if rm is not None else None, _ptr(loss), _ptr(ws), _stream()),
-                   "ams_objective_fwd")
+        _lib.check(L.ams_objective_fwd_cov(_ptr(Y), groups, Cc, N, mode, n0, w,
+                                           _ptr(rm) if rm is not None else None, _ptr(loss), _ptr(cov), _ptr(ws),
+                                           _stream()),
+                   "ams_objective_fwd_cov")
         ctx.save_for_backward(Y)
         ctx.ws, ctx.rm, ctx.cfg = ws, rm, (groups, Cc, N, mode, n0, w)
-        return loss
+        ctx.mark_non_differentiable(cov)
+        ctx.set_materialize_grads(False)
+        return loss, cov
 
     @staticmethod
-    def backward(ctx, gloss: torch.Tensor):
+    def backward(ctx, gloss: torch.Tensor, gcov: torch.Tensor = None):
+        if gloss is None:
+            return None, None, None, None, None, None
         L = _lib.lib()
         (Y,) = ctx.saved_tensors
         groups, Cc, N, mode, n0, w = ctx.cfg
@@ -197,7 +256,7 @@ class PhaseMapModel(nn.Module):
                  code_num: int = 30, w: float = 1.0, objective: str = "logsnr", n0: float = N0_90DBM,
                  n_maps: int = 1, optimize_codebook: bool = True, antenna_theta: Optional[torch.Tensor] = None,
                  ms_theta: Optional[List[torch.Tensor]] = None, plain: bool = False, seed: int = 0,
-                 max_ms_reflex: int = 1):
+                 max_ms_reflex: int = 1, fused: bool = True):
         super().__init__()
         self.max_ms_reflex = max_ms_reflex
         dev = channels["src_to_dst"].device
@@ -225,7 +284,10 @@ class PhaseMapModel(nn.Module):
             self.ms_A.append(torch.ones(K, dtype=torch.float32, device=dev))
             t0 = ms_theta[i] if ms_theta is not None else torch.zeros(n_maps, K)
             self.ms_theta.append(nn.Parameter(t0.to(dev, torch.float32).reshape(n_maps, K).clone()))
-        self.G_list = [channels["src_to_ms{%d}" % i].to(torch.complex64) for i in range(len(ms_list))]
+        self.G_list = [channels["src_to_ms{%d}" % i].to(torch.complex64).contiguous() for i in range(len(ms_list))]
+        # one metasurface: every chain has length 1, and the operand comes from
+        # the fused kernels (PhaseOperand); otherwise from torch ops
+        self.fused_operand = fused and not plain and len(ms_list) == 1
         # metasurface-to-metasurface channels; ms{b}->ms{a} for b > a is the
         # transpose (not the conjugate) of ms{a}->ms{b}, as go.py:147-153 uses it
         self.T = {}
@@ -272,21 +334,23 @@ class PhaseMapModel(nn.Module):
         return self.forward()
 
     def received(self) -> torch.Tensor:
-        A = self.stacked_operand()
+        if self.fused_operand:
+            A = PhaseOperand.apply(self.antenna_theta, self.ms_theta[0], self.antenna_A, self.ms_A[0],
+                                   self.G_list[0], self.ms_list[0].max_phase_rad)
+        else:
+            A = self.stacked_operand()
         fn = ContractPlain if self.plain else Contract
         return fn.apply(A, self.op)
 
     def forward(self) -> torch.Tensor:
         Y = self.received()
-        with torch.no_grad():  # coverage side output: mean over Rx of best-code power (dB)
-            p = (Y.real ** 2 + Y.imag ** 2).reshape(self.n_maps, self.code_num, -1).amax(1)
-            self.last_cov = (10 * torch.log10(p + 1e-30)).mean(1)
         rowmask = None
         if self.mode == 1:
             # go.py:220: rows of cap zeroed where torch.rand(C) < 0.1 (CPU generator)
             rowmask = (torch.rand(self.code_num * self.n_maps) < 0.1) if self.rowmask_fn is None \
                 else self.rowmask_fn()
-        loss = Objective.apply(Y, self.n_maps, self.mode, self.n0, self.w, rowmask)
+        # last_cov: mean over Rx of the best code's power (dB), per map
+        loss, self.last_cov = Objective.apply(Y, self.n_maps, self.mode, self.n0, self.w, rowmask)
         self.loss = loss.sum()
         return self.loss
 
@@ -300,7 +364,9 @@ class PhaseMapModel(nn.Module):
 
 class FusedAdam:
     """torch.optim.Adam semantics (defaults beta=(0.9, 0.999), eps=1e-8) with
-    one fused kernel over all parameters (ams_adam_step)."""
+    one fused kernel over all parameters (ams_adam_step_dev).  The step count
+    and the best-loss tracking live on the device, so a step can be captured
+    in a HIP graph."""
 
     def __init__(self, params, lr: float = 0.3, betas=(0.9, 0.999), eps: float = 1e-8):
         self.params = [p for p in params if p.requires_grad]
@@ -310,35 +376,96 @@ class FusedAdam:
         self.flat = torch.cat([p.detach().reshape(-1) for p in self.params]).contiguous()
         self.m = torch.zeros(n, dtype=torch.float32, device=dev)
         self.v = torch.zeros(n, dtype=torch.float32, device=dev)
+        # steps taken, the step's bias corrections, best-loss tracking
+        self.state = torch.zeros(_lib.AMS_ADAM_STATE_BYTES // 4, dtype=torch.int32, device=dev)
         off = 0
         for p in self.params:  # parameters become views of one flat buffer
             k = p.numel()
             p.data = self.flat[off:off + k].view_as(p)
             off += k
-        self.t = 0
+
+    @property
+    def steps_taken(self) -> int:
+        return int(self.state[0].item())
 
     def zero_grad(self):
         for p in self.params:
             p.grad = None
 
     @torch.no_grad()
-    def step(self):
-        self.t += 1
+    def step(self, loss: Optional[torch.Tensor] = None, best: Optional[torch.Tensor] = None,
+             hist: Optional[torch.Tensor] = None):
+        """One update.  With loss (the device scalar of the parameters before
+        this update), parameters whose loss is the lowest so far are copied
+        to best (a flat buffer like self.flat) and the loss is written to
+        hist[step] (both optional)."""
         g = torch.cat([(p.grad if p.grad is not None else torch.zeros_like(p)).reshape(-1) for p in self.params])
         g = g.to(torch.float32).contiguous()
-        _lib.check(_lib.lib().ams_adam_step(_ptr(self.flat), _ptr(g), _ptr(self.m), _ptr(self.v), self.flat.numel(),
-                                            self.lr, self.b1, self.b2, self.eps, self.t, _stream()),
-                   "ams_adam_step")
+        if loss is not None:
+            loss = loss.detach().to(torch.float32)
+        if best is not None and (best.numel() != self.flat.numel() or best.dtype != torch.float32
+                                 or not best.is_contiguous()):
+            raise ValueError("FusedAdam.step: best must be a contiguous float32 buffer of the parameters' size")
+        _lib.check(_lib.lib().ams_adam_step_dev(
+            _ptr(self.flat), _ptr(g), _ptr(self.m), _ptr(self.v), self.flat.numel(), self.lr, self.b1, self.b2,
+            self.eps, _ptr(self.state), _ptr(loss) if loss is not None else None,
+            _ptr(best) if best is not None else None, _ptr(hist) if hist is not None else None,
+            hist.numel() if hist is not None else 0, _stream()), "ams_adam_step_dev")
 
 
 def optimize_model(model: PhaseMapModel, lr: float = 0.3, step: int = 200, fused: bool = True,
-                   record: bool = True, callback=None):
+                   record: bool = True, callback=None, graph: bool = False, graph_dot: Optional[str] = None):
     """Adam over the model parameters keeping the best-loss parameters
     (go.py:296-336).  The best loss is tracked on the device (no per-step
-    host sync); the loss history is read back once at the end."""
+    host sync; inside the fused Adam when fused); the loss history is read
+    back once at the end.
+
+    graph=True captures one whole step (forward, the hand-written backward
+    and the fused Adam with its tracking) in a HIP graph and replays it:
+    the step's small launches become one graph launch.  It needs the fused
+    Adam and an objective with no host input per step (the log-SNR
+    objective; the capacity objective draws its row mask on the host each
+    step).  graph_dot names a file for the captured graph's nodes (Graphviz
+    DOT, from the runtime's graph dump)."""
     params = [p for p in model.parameters() if p.requires_grad]
-    opt = FusedAdam(params, lr=lr) if fused else torch.optim.Adam(params, lr=lr)
-    best = torch.tensor(float("inf"), device=model.antenna_theta.device)
+    if graph:
+        if not fused:
+            raise ValueError("optimize_model: graph=True needs the fused Adam")
+        if model.mode != 0:
+            raise ValueError("optimize_model: graph=True needs the log-SNR objective (the capacity objective's "
+                             "row mask is drawn on the host every step)")
+    if not fused:
+        return _optimize_torch_adam(model, params, lr, step, record, callback)
+    opt = FusedAdam(params, lr=lr)
+    dev = opt.flat.device
+    best = opt.flat.clone()
+    hist = torch.zeros(max(step, 1), dtype=torch.float32, device=dev) if record else None
+    seed = torch.ones((), dtype=torch.float32, device=dev)  # d loss / d loss, made once
+
+    def body():
+        loss = model()
+        loss.backward(seed)
+        opt.step(loss=loss, best=best, hist=hist)
+        opt.zero_grad()
+
+    if graph:
+        _replay_steps(model, body, step, callback, graph_dot)
+    else:
+        for it in range(step):
+            body()
+            if callback is not None:
+                callback(it, model)
+    with torch.no_grad():
+        opt.flat.copy_(best)
+    if record and step > 0:
+        model.loss_list.extend(hist[:step].cpu().tolist())
+    return model
+
+
+def _optimize_torch_adam(model: PhaseMapModel, params, lr: float, step: int, record: bool, callback):
+    """optimize_model with torch.optim.Adam; best-loss tracking by torch ops."""
+    opt = torch.optim.Adam(params, lr=lr)
+    best = torch.tensor(float("inf"), device=params[0].device)
     best_state = [p.detach().clone() for p in params]
     hist = []
     for it in range(step):
@@ -363,6 +490,61 @@ def optimize_model(model: PhaseMapModel, lr: float = 0.3, step: int = 200, fused
     return model
 
 
+_CAPTURE_STREAMS: Dict[int, torch.cuda.Stream] = {}
+
+
+def _capture_stream(dev) -> torch.cuda.Stream:
+    """One side stream per device for warm-up and capture, made once: a new
+    stream costs the libraries a new workspace each time."""
+    i = torch.device(dev).index
+    i = torch.cuda.current_device() if i is None else i
+    if i not in _CAPTURE_STREAMS:
+        _CAPTURE_STREAMS[i] = torch.cuda.Stream(device=i)
+    return _CAPTURE_STREAMS[i]
+
+
+def _replay_steps(model: PhaseMapModel, body, step: int, callback, graph_dot: Optional[str] = None):
+    """Run body() `step` times with the step captured once and replayed.  The
+    first two steps run eagerly on a side stream (capture needs whatever the
+    step initialises lazily done first); they are real steps of the run.
+    The third is captured on that stream and every step from there on is a
+    replay, so the trajectory is the eager loop's: the same kernels in the
+    same order, all state (step count, best loss, history index) on the
+    device."""
+    dev = model.antenna_theta.device
+    cur = torch.cuda.current_stream(dev)
+    side = _capture_stream(dev)
+    n_eager = min(2, step)
+    for it in range(n_eager):
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):
+            body()
+        cur.wait_stream(side)
+        # drop the step's autograd graph, so the parameters' gradient
+        # accumulators are made afresh on the capture stream
+        model.loss = None
+        if callback is not None:
+            callback(it, model)
+    if step <= n_eager:
+        return
+    g = torch.cuda.CUDAGraph()
+    if graph_dot:
+        g.enable_debug_mode()
+    side.wait_stream(cur)
+    with torch.cuda.stream(side):
+        g.capture_begin()
+        try:
+            body()
+        finally:
+            g.capture_end()
+    if graph_dot:
+        g.debug_dump(graph_dot)
+    for it in range(n_eager, step):
+        g.replay()
+        if callback is not None:
+            callback(it, model)
+
+
 def channels_from_traces(H_d: torch.Tensor, G: torch.Tensor, F: torch.Tensor) -> Dict[str, torch.Tensor]:
     """The channel dictionary of build_channels for one metasurface
     (go.py:1172-1224 key names)."""
@@ -370,11 +552,13 @@ def channels_from_traces(H_d: torch.Tensor, G: torch.Tensor, F: torch.Tensor) ->
 
 
 def time_to_target(channels: Dict[str, torch.Tensor], ms: Metasurface, steps: int = 500, lr: float = 0.3,
-                   code_num: int = 30, seed: int = 0, tol_db: float = 0.1) -> dict:
+                   code_num: int = 30, seed: int = 0, tol_db: float = 0.1, graph: bool = True) -> dict:
     """Optimise the phase map twice from the same start: plain FP32 (vector
     ALU contraction, torch.optim.Adam) and the split-bf16 MFMA path with the
-    fused Adam.  Returns the plain optimum's coverage, and the wall seconds the
-    MFMA path needs to come within tol_db of it (step events on the stream)."""
+    fused Adam (its step captured in a HIP graph when graph is set).  Returns
+    the plain optimum's coverage, and the wall seconds the MFMA path needs to
+    come within tol_db of it (step events on the stream, counted from before
+    the warm-up and capture)."""
     K = channels["src_to_ms{0}"].shape[1]
     M = channels["src_to_dst"].shape[0]
     g = torch.Generator().manual_seed(seed)
@@ -394,13 +578,14 @@ def time_to_target(channels: Dict[str, torch.Tensor], ms: Metasurface, steps: in
             e = torch.cuda.Event(enable_timing=True)
             e.record()
             evs.append(e)
-        optimize_model(model, lr=lr, step=steps, fused=not plain, callback=cb)
+        optimize_model(model, lr=lr, step=steps, fused=not plain, callback=cb, graph=graph and not plain)
         torch.cuda.synchronize()
         cov = torch.stack(covs).reshape(-1).cpu()
         t = [e0.elapsed_time(e) / 1e3 for e in evs]
         best_loss_step = int(torch.tensor(model.loss_list).argmin())
         out[name] = {"cov_db": cov, "t": t, "best_cov_db": float(cov[best_loss_step]),
-                     "final_loss": float(min(model.loss_list)), "seconds": t[-1]}
+                     "final_loss": float(min(model.loss_list)), "seconds": t[-1],
+                     "ms_per_step": (t[-1] - t[0]) / max(1, len(t) - 1) * 1e3}
     target = out["plain_fp32"]["best_cov_db"]
     cov = out["mfma_split_bf16"]["cov_db"]
     hit = (cov >= target - tol_db).nonzero()
@@ -415,6 +600,9 @@ def time_to_target(channels: Dict[str, torch.Tensor], ms: Metasurface, steps: in
         "start_cov_db": float(cov[0]),
         "final_loss_plain": out["plain_fp32"]["final_loss"],
         "final_loss_mfma": out["mfma_split_bf16"]["final_loss"],
+        "mfma_ms_per_step": out["mfma_split_bf16"]["ms_per_step"],
+        "plain_ms_per_step": out["plain_fp32"]["ms_per_step"],
+        "mfma_graph": bool(graph),
     }
 
 

@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--workload", default="room_boxes_ms", choices=["room_boxes", "room_boxes_ms", "office"])
     ap.add_argument("--opt-steps", type=int, default=500)
     ap.add_argument("--no-opt", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="run the phase-map steps eagerly, not as a HIP graph")
     ap.add_argument("--sequential", action="store_true",
                     help="trace H_d, G and F one after the other instead of concurrently (tracer.trace_many)")
     ap.add_argument("--no-contract", action="store_true")
@@ -361,7 +362,7 @@ def main():
             return torch.complex(outs[k][..., 0], outs[k][..., 1])
         ch = channels_from_traces(cplx("H_d"), cplx("G"), cplx("F"))
         ms = Metasurface(None, traces["G"].ms[0]["size"], None, None, [], max_phase=360.0)
-        r = time_to_target(ch, ms, steps=args.opt_steps)
+        r = time_to_target(ch, ms, steps=args.opt_steps, graph=not args.no_graph)
         print(f"phase map: {r}", file=sys.stderr, flush=True)
         line["phase_map"] = {
             "metric": "seconds to within 0.1 dB of the plain FP32 optimum (mean best-code Rx power)",
@@ -369,7 +370,10 @@ def main():
             "adam_steps": args.opt_steps, "mfma_seconds_all_steps": r["mfma_seconds_500"],
             "plain_fp32_seconds_all_steps": r["plain_seconds_500"],
             "plain_optimum_cov_db": r["plain_optimum_cov_db"], "mfma_best_cov_db": r["mfma_best_cov_db"],
-            "start_cov_db": r["start_cov_db"], "codes": 30, "objective": "mean log2(1+SNR), N0 = -90 dBm"}
+            "start_cov_db": r["start_cov_db"], "codes": 30, "objective": "mean log2(1+SNR), N0 = -90 dBm",
+            "mfma_step_hip_graph": r["mfma_graph"], "mfma_ms_per_step": r["mfma_ms_per_step"],
+            "plain_fp32_ms_per_step": r["plain_ms_per_step"],
+            "timing": "HIP events on the stream from before the first step (warm-up and capture included)"}
     if not args.no_contract:
         r = contraction_office(iters=5 if args.profile_pass else 20)
         print(f"contraction: {r}", file=sys.stderr, flush=True)

@@ -153,12 +153,55 @@ int ams_cgemm_f32(const float *A, int64_t lda, const float *B, int64_t ldb, int3
 int64_t ams_objective_workspace_bytes(int32_t G, int32_t N);
 int ams_objective_fwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
                       const unsigned char *rowmask, float *loss, void *workspace, void *stream);
+/* ams_objective_fwd with a coverage side output: the best code's received
+ * power per Rx in dB (go.py:166-181, max over codes), averaged over Rx:
+ * cov[g] = (1/N) sum_r 10 log10(max_c |y|^2 + 1e-30), over all codes
+ * whatever the row mask; cov may be null. */
+int ams_objective_fwd_cov(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                          const unsigned char *rowmask, float *loss, float *cov, void *workspace, void *stream);
 int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
                       const unsigned char *rowmask, const float *gloss, const void *workspace, float *gY,
                       void *stream);
+
+/* Phase-map operand of one metasurface with single-bounce chains, fused
+ * (go.py:107-163 for one metasurface and max_ms_reflex = 1, replacing its
+ * torch ops W = A e^{-j theta}, u = e^{j rem(phi)}, (W G) diag(u), cat).
+ * Rows r = m*Cn + c (map m, code c) of A [nm*Cn][K+M] complex f32:
+ *   A[r][k] = (W G)[c][k] u[m][k] (k < K),  A[r][K+j] = W[c][j],
+ *   W[c][j] = amp[c][j] e^{-i th_ap[c][j]},
+ *   u[m][k] = ms_amp[k] e^{i rem(th_ms[m][k], max_phase)} (torch.remainder).
+ * th_ap, amp f32 [Cn][M]; th_ms f32 [nm][K]; ms_amp f32 [K]; G complex f32
+ * [M][K].  Also writes WG = W G complex [Cn][K] and u complex [nm][K] for the
+ * backward.  M <= 4096. */
+int ams_pm_operand(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp, const float *G,
+                   int32_t Cn, int32_t M, int32_t K, int32_t nm, float max_phase, float *A, float *WG, float *U,
+                   void *stream);
+/* Its backward: gA complex [nm*Cn][K+M] in torch's convention for complex
+ * gradients; g_ms f32 [nm][K] = d loss / d th_ms; when g_ap is not null also
+ * g_ap f32 [Cn][M] = d loss / d th_ap, using gWG (complex [Cn][K]) as
+ * workspace. */
+int ams_pm_operand_bwd(const float *gA, const float *th_ap, const float *amp, const float *G, const float *WG,
+                       const float *U, int32_t Cn, int32_t M, int32_t K, int32_t nm, float *g_ms, float *g_ap,
+                       float *gWG, void *stream);
 /* torch.optim.Adam step (no weight decay), step counted from 1 */
 int ams_adam_step(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
                   float beta2, float eps, int32_t step, void *stream);
+/* The same step with its count kept on the device, and optionally the
+ * best-loss tracking of the training loop (go.py:320-327, torch.optim.Adam's
+ * state['step']).  step_state: AMS_ADAM_STATE_BYTES of device memory, zeroed
+ * before the first step (its first int32 is the number of steps taken).
+ * loss (device f32 scalar, may be null): the loss of param BEFORE this
+ * update; when it is below every loss passed so far (the first one always,
+ * unless NaN), param is copied to best_param (device, n floats; may be null)
+ * before the update.  loss_hist (device, hist_cap floats; may be null)
+ * receives loss at index (steps taken before this one) while below hist_cap.
+ * Each call advances the state on the stream, so the call holds no host value
+ * that changes between steps and can be captured in a HIP graph and replayed.
+ * The parameters follow ams_adam_step's arithmetic. */
+#define AMS_ADAM_STATE_BYTES 32
+int ams_adam_step_dev(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
+                      float beta2, float eps, void *step_state, const float *loss, float *best_param,
+                      float *loss_hist, int64_t hist_cap, void *stream);
 
 /* ---- placement surrogate network (3x3 convolutions, bf16 weights) --------
  * X [B][Cin][H][W] f32, W [Cout][Cin][3][3] bf16, bias [Cout] f32, padding 1;

@@ -176,6 +176,170 @@ def test_fused_adam_matches_torch(gpu):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("objective,n_maps,codebook", [("logsnr", 1, True), ("capacity", 2, True),
+                                                       ("logsnr", 3, False)])
+def test_fused_operand_matches_torch_ops(gpu, objective, n_maps, codebook):
+    """PhaseOperand (ams_pm_operand and its backward) against the same model
+    built from torch ops (fused=False): the operand A within 2e-6 relative,
+    the loss within 1e-5, the phase gradients within 1e-4 relative in norm.
+    A 6 x 6 AP with its corners off (zero amplitude), K = 200 (not a
+    multiple of the 256-wide column blocks), metasurface phases outside
+    [0, 300 deg) so the remainder wraps both ways.  The coverage side output
+    equals torch's mean over Rx of 10 log10(max_c |y|^2 + 1e-30) within
+    1e-4 dB."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel, PhaseOperand
+    _flush(f"case: fused operand objective={objective} n_maps={n_maps} codebook={codebook}")
+    Hd, G, F = _channels(M=36, K=200, N=900, seed=41)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (10, 20), None, None, [], max_phase=300.0)]
+    gth = torch.Generator().manual_seed(5)
+    th_ms = (torch.rand(n_maps, 200, generator=gth) - 0.3) * 6 * math.pi
+    mask = torch.rand(30 * n_maps, generator=torch.Generator().manual_seed(6)) < 0.1
+    out = {}
+    for fused in (False, True):
+        model = PhaseMapModel("t", ch, ms, code_num=30, w=0.5, objective=objective, ms_theta=[th_ms],
+                              n_maps=n_maps, optimize_codebook=codebook, fused=fused)
+        assert model.fused_operand == fused
+        model.rowmask_fn = lambda: mask
+        if fused:
+            A = PhaseOperand.apply(model.antenna_theta, model.ms_theta[0], model.antenna_A, model.ms_A[0],
+                                   model.G_list[0], model.ms_list[0].max_phase_rad)
+        else:
+            A = model.stacked_operand()
+        loss = model()
+        loss.backward()
+        with torch.no_grad():
+            Y = model.received().reshape(n_maps, 30, -1)
+            p = (Y.real ** 2 + Y.imag ** 2).amax(1)
+            cov_t = (10 * torch.log10(p + 1e-30)).mean(1)
+        g_ap = model.antenna_theta.grad.cpu().double() if codebook else None
+        out[fused] = (A.detach().cpu().to(torch.complex128), loss.item(), g_ap,
+                      model.ms_theta[0].grad.cpu().double(), model.last_cov.cpu().double(), cov_t.cpu().double())
+    (A0, l0, ga0, gm0, _, _), (A1, l1, ga1, gm1, cov1, covt1) = out[False], out[True]
+    relA = ((A1 - A0).norm() / A0.norm()).item()
+    rell = abs(l1 - l0) / abs(l0)
+    relm = ((gm1 - gm0).norm() / gm0.norm()).item()
+    rela = ((ga1 - ga0).norm() / ga0.norm()).item() if codebook else 0.0
+    dcov = (cov1 - covt1).abs().max().item()
+    _flush(f"  A rel {relA:.1e} loss rel {rell:.1e} g_ms rel {relm:.1e} g_ap rel {rela:.1e} cov diff {dcov:.1e} dB")
+    assert relA < 2e-6 and rell < 1e-5 and relm < 1e-4 and rela < 1e-4
+    assert dcov < 1e-4
+    if not codebook:
+        assert out[True][2] is None
+
+
+def test_adam_device_count_equals_host_count(gpu):
+    """ams_adam_step_dev (step count kept on the device) against
+    ams_adam_step (count passed from the host) over 6 steps: parameters and
+    moments within 1e-6 (the bias corrections are the same double pow rounded
+    to float once, on the device instead of the host); the count reads 6 and
+    the stored corrections match numpy's to one float ulp.  Its best-loss
+    tracking against the loop of go.py:320-327 restated on the host: losses
+    3, 2, NaN, 2, 1.5, 4 keep the parameters that entered step 5 (a tie and a
+    NaN are not improvements), and the history holds the first hist_cap = 5
+    losses."""
+    from automs_amd import _lib
+    from automs_amd.phasemap import _ptr, _stream
+    L = _lib.lib()
+    g = torch.Generator().manual_seed(1)
+    n = 1537
+    p0 = torch.randn(n, generator=g)
+    grads = [torch.randn(n, generator=g).to(gpu) for _ in range(6)]
+    losses = [3.0, 2.0, float("nan"), 2.0, 1.5, 4.0]
+    pa, pb = p0.clone().to(gpu), p0.clone().to(gpu)
+    ma, va, mb, vb = (torch.zeros(n, device=gpu) for _ in range(4))
+    st = torch.zeros(_lib.AMS_ADAM_STATE_BYTES // 4, dtype=torch.int32, device=gpu)
+    best = torch.full((n,), -7.0, device=gpu)
+    hist = torch.full((5,), -1.0, device=gpu)
+    entering = []
+    _flush("case: adam device count vs host count, n=1537, 6 steps, tracking")
+    for t, (gr, lo) in enumerate(zip(grads, losses), 1):
+        entering.append(pb.clone())
+        ld = torch.tensor(lo, dtype=torch.float32, device=gpu)
+        _lib.check(L.ams_adam_step(_ptr(pa), _ptr(gr), _ptr(ma), _ptr(va), n, 0.3, 0.9, 0.999, 1e-8, t,
+                                   _stream()), "ams_adam_step")
+        _lib.check(L.ams_adam_step_dev(_ptr(pb), _ptr(gr), _ptr(mb), _ptr(vb), n, 0.3, 0.9, 0.999, 1e-8,
+                                       _ptr(st), _ptr(ld), _ptr(best), _ptr(hist), 5, _stream()),
+                   "ams_adam_step_dev")
+    torch.cuda.synchronize()
+    stc = st.cpu()
+    assert int(stc[0]) == 6
+    b1, b2 = np.float64(np.float32(0.9)), np.float64(np.float32(0.999))
+    bc = stc.view(torch.float32)[1:3].numpy()
+    want = np.array([1.0 - b1 ** 6, math.sqrt(1.0 - b2 ** 6)]).astype(np.float32)
+    assert (np.abs(bc - want) <= np.spacing(want)).all(), (bc, want)
+    err = max((pa - pb).abs().max().item(), (ma - mb).abs().max().item(), (va - vb).abs().max().item())
+    _flush(f"  max abs diff {err:.2e} (bitwise equal: {torch.equal(pa, pb)})")
+    assert err <= 1e-6
+    # host restatement of the tracking
+    bl, bi = float("inf"), None
+    for i, lo in enumerate(losses):
+        if lo < bl:
+            bl, bi = lo, i
+    assert bi == 4
+    assert torch.equal(best, entering[bi])
+    assert float(stc.view(torch.float32)[4]) == bl
+    h = hist.cpu().numpy()
+    assert np.array_equal(np.isnan(h), np.isnan(np.array(losses[:5], dtype=np.float32)))
+    assert np.allclose(h[~np.isnan(h)], [3.0, 2.0, 2.0, 1.5])
+    with pytest.raises(RuntimeError):
+        _lib.check(L.ams_adam_step_dev(_ptr(pb), _ptr(grads[0]), _ptr(mb), _ptr(vb), n, 0.3, 0.9, 0.999, 1e-8,
+                                       None, None, None, None, 0, _stream()), "ams_adam_step_dev")
+    with pytest.raises(RuntimeError):  # a best buffer needs a loss
+        _lib.check(L.ams_adam_step_dev(_ptr(pb), _ptr(grads[0]), _ptr(mb), _ptr(vb), n, 0.3, 0.9, 0.999, 1e-8,
+                                       _ptr(st), None, _ptr(best), None, 0, _stream()), "ams_adam_step_dev")
+
+
+def test_graphed_optimiser_matches_eager(gpu, tmp_path):
+    """optimize_model(graph=True) captures one step (forward, best-loss
+    tracking, hand-written backward, fused Adam) and replays it.  Over 12
+    steps from the same start its loss history, per-step coverage and final
+    (best-loss) parameters equal the eager loop's within 1e-6 relative: the
+    same kernels run in the same order.  Between replays the callback runs
+    eager work (a clone, a device sync every third step).  The captured
+    graph holds no memset node.  The capacity objective, whose row mask is
+    drawn on the host each step, is refused."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel, optimize_model
+    # the room workload's shapes: 16 AP elements, a 32 x 32 metasurface, 4096 Rx
+    Hd, G, F = _channels(K=1024, N=4096, seed=31)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (32, 32), None, None, [], max_phase=360.0)]
+    gth = torch.Generator().manual_seed(8)
+    th_ms = torch.rand(1, 1024, generator=gth) * 2 * math.pi
+    th_ap = torch.rand(30, 16, generator=gth) * 2 * math.pi
+    dot = tmp_path / "step.dot"
+    res = {}
+    for graph in (False, True):
+        _flush(f"case: phase-map optimiser 12 steps graph={graph}")
+        model = PhaseMapModel("t", ch, ms, code_num=30, ms_theta=[th_ms], antenna_theta=th_ap)
+        covs = []
+
+        def cb(it, m):
+            covs.append(m.last_cov.detach().clone())
+            if it % 3 == 2:
+                torch.cuda.synchronize()
+        optimize_model(model, lr=0.3, step=12, callback=cb, graph=graph, graph_dot=str(dot) if graph else None)
+        torch.cuda.synchronize()
+        res[graph] = (torch.tensor(model.loss_list, dtype=torch.float64), torch.stack(covs).cpu().double(),
+                      model.antenna_theta.detach().cpu().double(), model.ms_theta[0].detach().cpu().double())
+    for name, a, b in zip(("loss", "cov", "ap", "ms"), res[False], res[True]):
+        rel = ((a - b).norm() / a.norm()).item()
+        _flush(f"  {name}: rel {rel:.2e} (bitwise equal: {torch.equal(a, b)})")
+        assert a.shape == b.shape and rel <= 1e-6
+    assert res[True][0][-1] < res[True][0][0]  # the replays optimise
+    if dot.exists():
+        text = dot.read_text().lower()
+        _flush(f"  graph dump {len(text)} bytes, kernel nodes {text.count('kernel')}")
+        assert "memset" not in text
+    else:
+        _flush("  no graph dump written by this runtime")
+    cap = PhaseMapModel("c", ch, ms, code_num=30, objective="capacity", ms_theta=[th_ms], antenna_theta=th_ap)
+    with pytest.raises(ValueError):
+        optimize_model(cap, step=2, graph=True)
+
+
 def test_batched_maps_match_single(gpu):
     """Four phase maps in one GEMM give the per-map losses of four separate
     models."""
