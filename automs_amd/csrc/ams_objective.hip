@@ -41,55 +41,85 @@ __device__ __forceinline__ float cap_of(float2 y, float n0) {
 // Coverage side output of one Rx: 10 log10 of the best code's power.
 __device__ __forceinline__ double power_db(float p) { return (double)(10.0f * log10f(p + 1e-30f)); }
 
+// Workgroup layout of the objective kernels: 64 Rx (one per lane) x 4 waves,
+// wave q taking codes c = q, q+4, ...; per-Rx results of the four waves are
+// combined through LDS.
+constexpr int kObjRx = 64;
+constexpr int kObjGroups = 4;
+
+// The first (lowest-index) maximum over the four code groups of one Rx,
+// each group's candidate being its own first maximum: the code a single
+// ascending scan with a strict '>' would pick.
+__device__ __forceinline__ void combine_first_max(const float (*val)[kObjRx], const int (*idx)[kObjRx], int rl,
+                                                  float &best, int &bi) {
+  best = val[0][rl];
+  bi = idx[0][rl];
+  for (int q = 1; q < kObjGroups; q++) {
+    float v = val[q][rl];
+    int i = idx[q][rl];
+    if (v > best || (v == best && i < bi)) {
+      best = v;
+      bi = i;
+    }
+  }
+}
+
 // partial[g][block] = (S1, S2, S3): mode 0: S1 = sum log2(1+max SNR); mode 1:
 // S1 = sum max cap', S2 = sum of squares of non-max cap'.  S3 (when
 // want_cov) = sum of power_db(max_c |y|^2).
-__global__ void __launch_bounds__(256) k_obj_fwd(const float2 *__restrict__ Y, int C, int N, float n0, int mode,
-                                                 const unsigned char *__restrict__ rowmask, int want_cov,
-                                                 double *__restrict__ partial, int *__restrict__ argmax) {
+__global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_fwd(const float2 *__restrict__ Y, int C, int N,
+                                                                float n0, int mode,
+                                                                const unsigned char *__restrict__ rowmask,
+                                                                int want_cov, double *__restrict__ partial,
+                                                                int *__restrict__ argmax) {
   __shared__ double sh[8];
-  int g = blockIdx.y;
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float val[kObjGroups][kObjRx];
+  __shared__ int idx[kObjGroups][kObjRx];
+  __shared__ float pw[kObjGroups][kObjRx];
+  const int g = blockIdx.y;
+  const int rl = threadIdx.x % kObjRx, q = threadIdx.x / kObjRx;
+  const int r = blockIdx.x * kObjRx + rl;
+  const bool ok = r < N;
   const float2 *Yg = Y + (size_t)g * C * N;
   double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  if (r < N) {
-    if (mode == 0) {
-      float best = -1.0f;
-      int bi = 0;
-      for (int c = 0; c < C; c++) {
-        float2 y = Yg[(size_t)c * N + r];
-        float s = y.x * y.x + y.y * y.y;
-        if (s > best) {
-          best = s;
-          bi = c;
-        }
+  // pass 1: each group's first maximum (power in mode 0, capacity in mode 1)
+  float best = -1.0f, pmax = 0.0f;
+  int bi = 0;
+  if (ok)
+    for (int c = q; c < C; c += kObjGroups) {
+      float2 y = Yg[(size_t)c * N + r];
+      float p = y.x * y.x + y.y * y.y;
+      pmax = fmaxf(pmax, p);
+      float v = mode == 0 ? p : (rowmask && rowmask[c] ? 0.0f : cap_of(y, n0));
+      if (v > best) {
+        best = v;
+        bi = c;
       }
+    }
+  val[q][rl] = best;
+  idx[q][rl] = bi;
+  pw[q][rl] = pmax;
+  __syncthreads();
+  combine_first_max(val, idx, rl, best, bi);
+  if (ok && q == 0) {
+    if (mode == 0) {
       s1 = (double)log2f(1.0f + best / n0);
       if (want_cov) s3 = power_db(best);
-      if (argmax) argmax[(size_t)g * N + r] = bi;
     } else {
-      float best = -1.0f, pmax = 0.0f;
-      int bi = 0;
-      for (int c = 0; c < C; c++) {
-        float2 y = Yg[(size_t)c * N + r];
-        pmax = fmaxf(pmax, y.x * y.x + y.y * y.y);
-        float v = rowmask && rowmask[c] ? 0.0f : cap_of(y, n0);
-        if (v > best) {
-          best = v;
-          bi = c;
-        }
-      }
-      if (want_cov) s3 = power_db(pmax);
-      for (int c = 0; c < C; c++) {
-        float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
-        if (v != best) s2 += (double)v * v;
-      }
+      float pm = fmaxf(fmaxf(pw[0][rl], pw[1][rl]), fmaxf(pw[2][rl], pw[3][rl]));
       s1 = best;
-      if (argmax) argmax[(size_t)g * N + r] = bi;
+      if (want_cov) s3 = power_db(pm);
     }
+    if (argmax) argmax[(size_t)g * N + r] = bi;
   }
+  // pass 2 (mode 1): squares of the codes whose capacity is not the maximum
+  if (mode != 0 && ok)
+    for (int c = q; c < C; c += kObjGroups) {
+      float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
+      if (v != best) s2 += (double)v * v;
+    }
   double t1 = block_sum(s1, sh);
-  double t2 = block_sum(s2, sh);
+  double t2 = mode != 0 ? block_sum(s2, sh) : 0.0;
   double t3 = want_cov ? block_sum(s3, sh) : 0.0;
   if (threadIdx.x == 0) {
     double *p = partial + 3 * ((size_t)g * gridDim.x + blockIdx.x);
@@ -99,18 +129,25 @@ __global__ void __launch_bounds__(256) k_obj_fwd(const float2 *__restrict__ Y, i
   }
 }
 
-// loss[g] (and cov[g]) from the partials, in block order (deterministic).
-__global__ void k_obj_finish(const double *__restrict__ partial, int nb, int N, int mode, float w,
-                             double *__restrict__ sums, float *__restrict__ loss, float *__restrict__ cov) {
-  int g = blockIdx.x;
-  if (threadIdx.x != 0) return;
+// loss[g] (and cov[g]) from the partials: lane l adds blocks l, l+64, ... in
+// order, then a fixed shuffle tree (deterministic).
+__global__ void __launch_bounds__(64) k_obj_finish(const double *__restrict__ partial, int nb, int N, int mode,
+                                                   float w, double *__restrict__ sums, float *__restrict__ loss,
+                                                   float *__restrict__ cov) {
+  const int g = blockIdx.x;
   double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int b = 0; b < nb; b++) {
+  for (int b = threadIdx.x; b < nb; b += 64) {
     const double *p = partial + 3 * ((size_t)g * nb + b);
     s1 += p[0];
     s2 += p[1];
     s3 += p[2];
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    s1 += __shfl_down(s1, off);
+    s2 += __shfl_down(s2, off);
+    s3 += __shfl_down(s3, off);
+  }
+  if (threadIdx.x != 0) return;
   sums[2 * g] = s1;
   sums[2 * g + 1] = s2;
   if (mode == 0) loss[g] = (float)(-s1 / N);
@@ -118,19 +155,25 @@ __global__ void k_obj_finish(const double *__restrict__ partial, int nb, int N, 
   if (cov) cov[g] = (float)(s3 / N);
 }
 
-__global__ void __launch_bounds__(256) k_obj_bwd(const float2 *__restrict__ Y, int C, int N, float n0, int mode,
-                                                 float w, const unsigned char *__restrict__ rowmask,
-                                                 const int *__restrict__ argmax, const double *__restrict__ sums,
-                                                 const float *__restrict__ gloss, float2 *__restrict__ gY) {
-  int g = blockIdx.y;
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= N) return;
+__global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_bwd(const float2 *__restrict__ Y, int C, int N,
+                                                                float n0, int mode, float w,
+                                                                const unsigned char *__restrict__ rowmask,
+                                                                const int *__restrict__ argmax,
+                                                                const double *__restrict__ sums,
+                                                                const float *__restrict__ gloss,
+                                                                float2 *__restrict__ gY) {
+  __shared__ float val[kObjGroups][kObjRx];
+  const int g = blockIdx.y;
+  const int rl = threadIdx.x % kObjRx, q = threadIdx.x / kObjRx;
+  const int r = blockIdx.x * kObjRx + rl;
+  const bool ok = r < N;
   const float2 *Yg = Y + (size_t)g * C * N;
   float2 *Gg = gY + (size_t)g * C * N;
-  float gl = gloss[g];
-  int bi = argmax[(size_t)g * N + r];
+  const float gl = gloss[g];
+  const int bi = ok ? argmax[(size_t)g * N + r] : 0;
   if (mode == 0) {
-    for (int c = 0; c < C; c++) {
+    if (!ok) return;
+    for (int c = q; c < C; c += kObjGroups) {
       float2 out = make_float2(0.0f, 0.0f);
       if (c == bi) {
         float2 y = Yg[(size_t)c * N + r];
@@ -141,29 +184,35 @@ __global__ void __launch_bounds__(256) k_obj_bwd(const float2 *__restrict__ Y, i
       }
       Gg[(size_t)c * N + r] = out;
     }
-  } else {
-    float fro = (float)sqrt(sums[2 * g + 1]);
-    float best = -1.0f;
-    for (int c = 0; c < C; c++) {
+    return;
+  }
+  // mode 1: the maximum capacity of the Rx (over all four groups) first
+  float best = -1.0f;
+  if (ok)
+    for (int c = q; c < C; c += kObjGroups) {
       float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
       if (v > best) best = v;
     }
-    for (int c = 0; c < C; c++) {
-      float2 out = make_float2(0.0f, 0.0f);
-      if (!(rowmask && rowmask[c])) {
-        float2 y = Yg[(size_t)c * N + r];
-        float v = cap_of(y, n0);
-        float dcap = 0.0f;
-        if (c == bi) dcap -= w;
-        if (v != best && fro > 0.0f) dcap += v / fro;
-        float amp = sqrtf(y.x * y.x + y.y * y.y);
-        float a = amp + kEps32;
-        float dca = kInvLn2 * 2.0f * a / n0 / (1.0f + a * a / n0);  // dcap / d|y|
-        float gabs = gl * dcap * dca;
-        if (amp > 0.0f) out = make_float2(gabs * y.x / amp, gabs * y.y / amp);
-      }
-      Gg[(size_t)c * N + r] = out;
+  val[q][rl] = best;
+  __syncthreads();
+  best = fmaxf(fmaxf(val[0][rl], val[1][rl]), fmaxf(val[2][rl], val[3][rl]));
+  if (!ok) return;
+  const float fro = (float)sqrt(sums[2 * g + 1]);
+  for (int c = q; c < C; c += kObjGroups) {
+    float2 out = make_float2(0.0f, 0.0f);
+    if (!(rowmask && rowmask[c])) {
+      float2 y = Yg[(size_t)c * N + r];
+      float v = cap_of(y, n0);
+      float dcap = 0.0f;
+      if (c == bi) dcap -= w;
+      if (v != best && fro > 0.0f) dcap += v / fro;
+      float amp = sqrtf(y.x * y.x + y.y * y.y);
+      float a = amp + kEps32;
+      float dca = kInvLn2 * 2.0f * a / n0 / (1.0f + a * a / n0);  // dcap / d|y|
+      float gabs = gl * dcap * dca;
+      if (amp > 0.0f) out = make_float2(gabs * y.x / amp, gabs * y.y / amp);
     }
+    Gg[(size_t)c * N + r] = out;
   }
 }
 
@@ -244,7 +293,7 @@ __global__ void k_adam_dev(float *__restrict__ p, const float *__restrict__ g, f
 extern "C" {
 
 int64_t ams_objective_workspace_bytes(int32_t G, int32_t N) {
-  int64_t nb = (N + 255) / 256;
+  int64_t nb = (N + kObjRx - 1) / kObjRx;
   return (int64_t)G * nb * 24 + (int64_t)G * 16 + (int64_t)G * N * 4;
 }
 
@@ -257,11 +306,11 @@ int ams_objective_fwd_cov(const float *Y, int32_t G, int32_t C, int32_t N, int32
     return AMS_ERR_ARG;
   }
   hipStream_t st = (hipStream_t)stream;
-  int nb = (N + 255) / 256;
+  int nb = (N + kObjRx - 1) / kObjRx;
   double *partial = (double *)workspace;
   double *sums = partial + 3 * (size_t)G * nb;
   int *argmax = (int *)(sums + 2 * (size_t)G);
-  k_obj_fwd<<<dim3(nb, G), 256, 0, st>>>((const float2 *)Y, C, N, n0, mode, rowmask, cov ? 1 : 0, partial, argmax);
+  k_obj_fwd<<<dim3(nb, G), kObjRx * kObjGroups, 0, st>>>((const float2 *)Y, C, N, n0, mode, rowmask, cov ? 1 : 0, partial, argmax);
   k_obj_finish<<<G, 64, 0, st>>>(partial, nb, N, mode, w, sums, loss, cov);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -285,11 +334,11 @@ int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t m
     return AMS_ERR_ARG;
   }
   hipStream_t st = (hipStream_t)stream;
-  int nb = (N + 255) / 256;
+  int nb = (N + kObjRx - 1) / kObjRx;
   const double *partial = (const double *)workspace;
   const double *sums = partial + 3 * (size_t)G * nb;
   const int *argmax = (const int *)(sums + 2 * (size_t)G);
-  k_obj_bwd<<<dim3(nb, G), 256, 0, st>>>((const float2 *)Y, C, N, n0, mode, w, rowmask, argmax, sums, gloss,
+  k_obj_bwd<<<dim3(nb, G), kObjRx * kObjGroups, 0, st>>>((const float2 *)Y, C, N, n0, mode, w, rowmask, argmax, sums, gloss,
                                          (float2 *)gY);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

@@ -30,8 +30,7 @@ namespace {
 
 constexpr int kPmBlock = 256;
 constexpr int kPmMaxM = 4096;  // AP elements (LDS row of W: 32 KB)
-constexpr int kPmJ = 16;       // AP elements per pass of the gW reduction
-constexpr int kPmApBlock = 1024;
+constexpr int kPmApBlock = 256;
 constexpr int kPmMsK = 64;  // metasurface elements per workgroup of k_pm_bwd_ms
 constexpr int kPmMsC = 4;   // code groups (waves) per workgroup of k_pm_bwd_ms
 
@@ -145,61 +144,47 @@ __global__ void __launch_bounds__(kPmMsK *kPmMsC) k_pm_bwd_ms(const float2 *__re
   }
 }
 
-// One workgroup of 1024 lanes per code c: gW[c][j] = sum_k gWG[c][k]
-// conj(G[j][k]) (kPmJ elements j per pass, a wave reduction then an ordered
-// sum over the 16 waves), plus the direct columns, then g_ap[c][j].
+// One workgroup of 256 lanes per (AP element j, code c): gW[c][j] =
+// sum_k gWG[c][k] conj(G[j][k]) (lanes stride over k; a wave reduction, then
+// the four waves added in order), plus the direct columns, then g_ap[c][j].
 __global__ void __launch_bounds__(kPmApBlock) k_pm_bwd_ap(const float2 *__restrict__ gA,
                                                           const float2 *__restrict__ gWG,
                                                           const float2 *__restrict__ G,
                                                           const float *__restrict__ th_ap,
                                                           const float *__restrict__ amp, int Cn, int M, int K,
                                                           int nm, float *__restrict__ g_ap) {
-  __shared__ float2 red[kPmApBlock / 64][kPmJ];
-  const int c = blockIdx.x;
+  __shared__ float2 red[kPmApBlock / 64];
+  const int j = blockIdx.x, c = blockIdx.y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long Kc = (long long)K + M;
-  for (int j0 = 0; j0 < M; j0 += kPmJ) {
-    float2 acc[kPmJ];
-#pragma unroll
-    for (int q = 0; q < kPmJ; q++) acc[q] = make_float2(0.0f, 0.0f);
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
-      float2 gw = gWG[(size_t)c * K + k];
-#pragma unroll
-      for (int q = 0; q < kPmJ; q++) {
-        if (j0 + q < M) {
-          float2 t = cmulc(gw, G[(size_t)(j0 + q) * K + k]);
-          acc[q].x += t.x;
-          acc[q].y += t.y;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kPmJ; q++) {
-      float x = acc[q].x, y = acc[q].y;
-      for (int off = 32; off > 0; off >>= 1) {
-        x += __shfl_down(x, off);
-        y += __shfl_down(y, off);
-      }
-      if (lane == 0) red[wave][q] = make_float2(x, y);
-    }
-    __syncthreads();
-    if (threadIdx.x < kPmJ && j0 + (int)threadIdx.x < M) {
-      const int j = j0 + threadIdx.x;
-      float2 s = make_float2(0.0f, 0.0f);
-      for (int w = 0; w < kPmApBlock / 64; w++) {
-        s.x += red[w][threadIdx.x].x;
-        s.y += red[w][threadIdx.x].y;
-      }
-      for (int m = 0; m < nm; m++) {
-        float2 d = gA[(size_t)(m * Cn + c) * Kc + K + j];
-        s.x += d.x;
-        s.y += d.y;
-      }
-      float2 w = ap_weight(amp[(size_t)c * M + j], th_ap[(size_t)c * M + j]);
-      g_ap[(size_t)c * M + j] = -(w.x * s.y - w.y * s.x);
-    }
-    __syncthreads();
+  const float2 *gw = gWG + (size_t)c * K;
+  const float2 *gj = G + (size_t)j * K;
+  float x = 0.0f, y = 0.0f;
+#pragma unroll 4
+  for (int k = threadIdx.x; k < K; k += kPmApBlock) {
+    float2 t = cmulc(gw[k], gj[k]);
+    x += t.x;
+    y += t.y;
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    x += __shfl_down(x, off);
+    y += __shfl_down(y, off);
+  }
+  if (lane == 0) red[wave] = make_float2(x, y);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  float2 s = red[0];
+  for (int w = 1; w < kPmApBlock / 64; w++) {
+    s.x += red[w].x;
+    s.y += red[w].y;
+  }
+  for (int m = 0; m < nm; m++) {
+    float2 d = gA[(size_t)(m * Cn + c) * Kc + K + j];
+    s.x += d.x;
+    s.y += d.y;
+  }
+  float2 w = ap_weight(amp[(size_t)c * M + j], th_ap[(size_t)c * M + j]);
+  g_ap[(size_t)c * M + j] = -(w.x * s.y - w.y * s.x);
 }
 
 bool pm_dims_ok(int32_t Cn, int32_t M, int32_t K, int32_t nm) {
@@ -242,7 +227,7 @@ int ams_pm_operand_bwd(const float *gA, const float *th_ap, const float *amp, co
   k_pm_bwd_ms<<<(unsigned)((K + kPmMsK - 1) / kPmMsK), kPmMsK * kPmMsC, 0, st>>>(
       (const float2 *)gA, (const float2 *)WG, (const float2 *)U, Cn, M, K, nm, g_ms, g_ap ? (float2 *)gWG : nullptr);
   if (g_ap)
-    k_pm_bwd_ap<<<(unsigned)Cn, kPmApBlock, 0, st>>>((const float2 *)gA, (const float2 *)gWG, (const float2 *)G,
+    k_pm_bwd_ap<<<dim3((unsigned)M, (unsigned)Cn), kPmApBlock, 0, st>>>((const float2 *)gA, (const float2 *)gWG, (const float2 *)G,
                                                     th_ap, amp, Cn, M, K, nm, g_ap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

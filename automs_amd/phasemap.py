@@ -566,6 +566,14 @@ def time_to_target(channels: Dict[str, torch.Tensor], ms: Metasurface, steps: in
     th_ap = torch.rand(code_num, M, generator=g) * 2 * math.pi
     out = {}
     for name, plain in (("plain_fp32", True), ("mfma_split_bf16", False)):
+        # untimed: 3 steps of the same optimisation on a throwaway model, so the
+        # process's one-time set-up (code loading, streams, allocator pools)
+        # stays out of the timed run; the timed run does its own warm-up steps
+        # and capture
+        warm = PhaseMapModel(name, channels, [ms], code_num=code_num, ms_theta=[th_ms], antenna_theta=th_ap,
+                             plain=plain)
+        optimize_model(warm, lr=lr, step=3, fused=not plain, graph=graph and not plain)
+        del warm
         model = PhaseMapModel(name, channels, [ms], code_num=code_num, ms_theta=[th_ms], antenna_theta=th_ap,
                               plain=plain)
         covs, evs = [], []

@@ -119,6 +119,43 @@ def test_model_loss_and_gradient_vs_oracle(gpu, objective):
     assert rel_gm < 2e-3
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_objective_ties_mask_and_coverage(gpu, mode):
+    """The objective kernels on Y with exact ties (codes 9 and 17 copy code
+    5, so the first of equal maxima must win across the kernels' code
+    groups), masked rows (mode 1), N = 1000 Rx (not a multiple of the
+    64-Rx workgroups) and two groups (mode 0): loss within 1e-5 relative and
+    gY within 1e-4 relative in norm of the complex128 oracle's autograd
+    (torch's max sends the gradient to the first maximum); the coverage
+    output within 1e-4 dB of the oracle's."""
+    from automs_amd.phasemap import N0_90DBM, Objective
+    from oracle import contract_ref as R
+    groups = 2 if mode == 0 else 1
+    C, N = 30, 1000
+    Y = _crand(groups * C, N, scale=3e-5, seed=51)
+    for g in range(groups):
+        Y[g * C + 9] = Y[g * C + 5]
+        Y[g * C + 17] = Y[g * C + 5]
+    Y = Y.to(torch.complex64)
+    mask = None
+    if mode == 1:
+        mask = torch.zeros(C, dtype=torch.bool)
+        mask[[2, 9, 23]] = True
+    _flush(f"case: objective ties mode={mode} groups={groups}")
+    Yd = Y.to(gpu).requires_grad_(True)
+    loss, cov = Objective.apply(Yd, groups, mode, N0_90DBM, 0.5, mask)
+    loss.sum().backward()
+    Yo = Y.to(torch.complex128).requires_grad_(True)
+    lo = R.loss_logsnr(Yo, groups, N0_90DBM) if mode == 0 else R.loss_capacity(Yo, N0_90DBM, 0.5, mask)
+    lo.backward()
+    rel_l = abs(loss.sum().item() - lo.item()) / abs(lo.item())
+    g_gpu = Yd.grad.cpu().to(torch.complex128)
+    rel_g = ((g_gpu - Yo.grad).norm() / Yo.grad.norm()).item()
+    dcov = (cov.cpu().double() - R.coverage_db(Y.to(torch.complex128), groups)).abs().max().item()
+    _flush(f"  loss rel {rel_l:.1e} gY rel {rel_g:.1e} cov diff {dcov:.1e} dB")
+    assert rel_l < 1e-5 and rel_g < 1e-4 and dcov < 1e-4
+
+
 def test_gradient_vs_finite_differences(gpu):
     """Central differences (h = 1e-3 rad) of the oracle loss at coordinates
     away from ties agree with the GPU gradient to 2e-2 relative."""
