@@ -505,16 +505,15 @@ def _capture_stream(dev) -> torch.cuda.Stream:
 
 def _replay_steps(model: PhaseMapModel, body, step: int, callback, graph_dot: Optional[str] = None):
     """Run body() `step` times with the step captured once and replayed.  The
-    first two steps run eagerly on a side stream (capture needs whatever the
-    step initialises lazily done first); they are real steps of the run.
-    The third is captured on that stream and every step from there on is a
-    replay, so the trajectory is the eager loop's: the same kernels in the
-    same order, all state (step count, best loss, history index) on the
-    device."""
+    first step runs eagerly on a side stream (capture needs whatever the step
+    initialises lazily done first); it is a real step of the run.  The second
+    is captured on that stream and every step from there on is a replay, so
+    the trajectory is the eager loop's: the same kernels in the same order,
+    all state (step count, best loss, history) on the device."""
     dev = model.antenna_theta.device
     cur = torch.cuda.current_stream(dev)
     side = _capture_stream(dev)
-    n_eager = min(2, step)
+    n_eager = min(1, step)
     for it in range(n_eager):
         side.wait_stream(cur)
         with torch.cuda.stream(side):
