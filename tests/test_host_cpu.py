@@ -380,3 +380,27 @@ def test_task_info_record_layout():
     block = block[: block.index("} ams_task_info;")]
     for name, *_ in Rec._fields_:
         assert name.rstrip("_") in block, name
+
+
+def test_fused_adam_flat_gradient_view():
+    """FusedAdam._flat_grad returns a view of the gradients' buffer when they
+    lie back to back in parameter order (as the fused operand's backward
+    writes them), and their concatenation otherwise (separate tensors, the
+    wrong order, a missing gradient)."""
+    import torch
+    from automs_amd.phasemap import FusedAdam
+    a = torch.nn.Parameter(torch.zeros(3, 4))
+    b = torch.nn.Parameter(torch.zeros(2, 5))
+    opt = FusedAdam([a, b])
+    buf = torch.arange(22, dtype=torch.float32)
+    a.grad, b.grad = buf[:12].view(3, 4), buf[12:].view(2, 5)
+    g = opt._flat_grad()
+    assert g.data_ptr() == buf.data_ptr() and torch.equal(g, buf)
+    a.grad, b.grad = buf[10:22].view(3, 4), buf[:10].view(2, 5)  # same buffer, other order
+    g = opt._flat_grad()
+    assert g.data_ptr() != buf.data_ptr()
+    assert torch.equal(g, torch.cat([buf[10:22], buf[:10]]))
+    a.grad, b.grad = torch.ones(3, 4), torch.full((2, 5), 2.0)
+    assert torch.equal(opt._flat_grad(), torch.cat([torch.ones(12), torch.full((10,), 2.0)]))
+    b.grad = None
+    assert torch.equal(opt._flat_grad(), torch.cat([torch.ones(12), torch.zeros(10)]))
