@@ -234,9 +234,10 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 }
 
 // Device-side state of the fused Adam (AMS_ADAM_STATE_BYTES): the step
-// count, the two bias corrections of the current step, and the best-loss
-// tracking of go.py:320-327 (the loss of the parameters before the update
-// is compared with the best so far; when lower, those parameters are kept).
+// count, the bias corrections of the last step, and the best-loss tracking
+// of go.py:320-327 (the loss of the parameters before the update is compared
+// with the best so far; when lower, those parameters are kept).  done counts
+// the workgroups of a launch that have finished; it is zero between launches.
 struct AdamState {
   int32_t step;
   float bc1;
@@ -244,48 +245,57 @@ struct AdamState {
   int32_t better;
   float best_loss;
   int32_t best_valid;
-  int32_t pad[2];
+  uint32_t done;
+  int32_t pad;
 };
 
-// One lane: the count, the bias corrections (the arithmetic of the host path:
-// double pow, rounded to float once), the best-loss decision and the history.
-__global__ void k_adam_count(AdamState *st, float b1, float b2, const float *loss, float *hist, long long hist_cap) {
-  if (threadIdx.x != 0) return;
-  int32_t t = st->step;
-  if (loss) {
-    float l = *loss;
-    float best = st->best_valid ? st->best_loss : INFINITY;
-    int better = l < best;
-    st->better = better;
-    if (better) {
-      st->best_loss = l;
-      st->best_valid = 1;
-    }
-    if (hist && t < hist_cap) hist[t] = l;
-  } else {
-    st->better = 0;
-  }
-  t += 1;
-  st->step = t;
-  st->bc1 = (float)(1.0 - pow((double)b1, (double)t));
-  st->bc2_sqrt = (float)sqrt(1.0 - pow((double)b2, (double)t));
-}
-
-__global__ void k_adam_dev(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                           float *__restrict__ v, long long n, float lr, float b1, float b2, float eps,
-                           const AdamState *__restrict__ st, float *__restrict__ best_p) {
+// One launch per step.  Every lane reads the state the previous step left
+// and derives this step's count, bias corrections (the arithmetic of the
+// host path: double pow, rounded to float once) and best-loss decision
+// itself.  The last workgroup to finish (found through done, which it
+// resets) writes the new state and the history entry; every workgroup has
+// read the old state before it counts itself done, so none can see the new.
+__global__ void __launch_bounds__(256) k_adam_dev(float *__restrict__ p, const float *__restrict__ g,
+                                                  float *__restrict__ m, float *__restrict__ v, long long n,
+                                                  float lr, float b1, float b2, float eps, AdamState *st,
+                                                  const float *__restrict__ loss, float *__restrict__ best_p,
+                                                  float *__restrict__ hist, long long hist_cap) {
+  __shared__ int last;
+  const int32_t t = st->step + 1;
+  const float bc1 = (float)(1.0 - pow((double)b1, (double)t));
+  const float bc2_sqrt = (float)sqrt(1.0 - pow((double)b2, (double)t));
+  const float l = loss ? *loss : 0.0f;
+  const float best = st->best_valid ? st->best_loss : INFINITY;
+  const int better = loss && l < best;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float bc1 = st->bc1, bc2_sqrt = st->bc2_sqrt;
-  float pi = p[i];
-  if (best_p && st->better) best_p[i] = pi;
-  float gi = g[i];
-  float mi = m[i] + (1.0f - b1) * (gi - m[i]);
-  float vi = v[i] * b2 + (1.0f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  float denom = sqrtf(vi) / bc2_sqrt + eps;
-  p[i] = pi - (lr / bc1) * (mi / denom);
+  if (i < n) {
+    float pi = p[i];
+    if (best_p && better) best_p[i] = pi;
+    float gi = g[i];
+    float mi = m[i] + (1.0f - b1) * (gi - m[i]);
+    float vi = v[i] * b2 + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float denom = sqrtf(vi) / bc2_sqrt + eps;
+    p[i] = pi - (lr / bc1) * (mi / denom);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  st->done = 0;
+  st->step = t;
+  st->bc1 = bc1;
+  st->bc2_sqrt = bc2_sqrt;
+  st->better = better;
+  if (better) {
+    st->best_loss = l;
+    st->best_valid = 1;
+  }
+  if (loss && hist && t - 1 < hist_cap) hist[t - 1] = l;
 }
 
 }  // namespace
@@ -377,9 +387,8 @@ int ams_adam_step_dev(float *param, const float *grad, float *m, float *v, int64
   static_assert(sizeof(AdamState) == AMS_ADAM_STATE_BYTES, "AdamState layout");
   hipStream_t st = (hipStream_t)stream;
   AdamState *as = (AdamState *)step_state;
-  k_adam_count<<<1, 64, 0, st>>>(as, beta1, beta2, loss, loss_hist, (long long)hist_cap);
-  k_adam_dev<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(param, grad, m, v, n, lr, beta1, beta2, eps, as,
-                                                          best_param);
+  k_adam_dev<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(param, grad, m, v, n, lr, beta1, beta2, eps, as, loss,
+                                                          best_param, loss_hist, (long long)hist_cap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ams::set_error(std::string("ams_adam_step_dev: ") + hipGetErrorString(e));

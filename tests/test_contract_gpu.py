@@ -267,7 +267,8 @@ def test_fused_operand_matches_torch_ops(gpu, objective, n_maps, codebook):
 
 
 def test_adam_device_count_equals_host_count(gpu):
-    """ams_adam_step_dev (step count kept on the device) against
+    """ams_adam_step_dev (step count kept on the device; 7 workgroups, the
+    last to finish writing the new state) against
     ams_adam_step (count passed from the host) over 6 steps: parameters and
     moments within 1e-6 (the bias corrections are the same double pow rounded
     to float once, on the device instead of the host); the count reads 6 and
@@ -302,6 +303,7 @@ def test_adam_device_count_equals_host_count(gpu):
     torch.cuda.synchronize()
     stc = st.cpu()
     assert int(stc[0]) == 6
+    assert int(stc[6]) == 0  # the finished-workgroup count is back to zero after every launch
     b1, b2 = np.float64(np.float32(0.9)), np.float64(np.float32(0.999))
     bc = stc.view(torch.float32)[1:3].numpy()
     want = np.array([1.0 - b1 ** 6, math.sqrt(1.0 - b2 ** 6)]).astype(np.float32)
