@@ -249,6 +249,24 @@ class Metasurface:
         return self.max_phase / 180 * math.pi
 
 
+def ap_amplitude(code_num: int, M: int) -> torch.Tensor:
+    """go.py:59-66: every AP element at amplitude 1/sqrt(M/36); the 4 corners
+    of a 6x6 AP are off.  [code_num, M] float32."""
+    A = torch.ones(code_num, M, dtype=torch.float32) / math.sqrt(M / 36)
+    if M == 36:
+        A[:, [0, 5, 30, 35]] = 0
+    return A
+
+
+def start_phases(K: int, M: int, code_num: int = 30, seed: int = 0):
+    """The seeded start of the config-1 optimisation: metasurface phases
+    [1, K] and AP phases [code_num, M], uniform on [0, 2 pi)."""
+    g = torch.Generator().manual_seed(seed)
+    th_ms = torch.rand(1, K, generator=g) * 2 * math.pi
+    th_ap = torch.rand(code_num, M, generator=g) * 2 * math.pi
+    return th_ms, th_ap
+
+
 class PhaseMapModel(nn.Module):
     """Codebook + metasurface phase model (go.py:27-245, chains of length <= 1).
 
@@ -276,11 +294,7 @@ class PhaseMapModel(nn.Module):
         M = channels["src_to_dst"].shape[0]
         self.code_num = code_num
         g = torch.Generator(device="cpu").manual_seed(seed)
-        # go.py:59-66: amplitude 1/sqrt(M/36); the 4 corners of a 6x6 AP are off
-        A = torch.ones(code_num, M, dtype=torch.float32) / math.sqrt(M / 36)
-        if M == 36:
-            A[:, [0, 5, 30, 35]] = 0
-        self.register_buffer("antenna_A", A.to(dev))
+        self.register_buffer("antenna_A", ap_amplitude(code_num, M).to(dev))
         th = antenna_theta if antenna_theta is not None else torch.rand(code_num, M, generator=g) * 2 * math.pi
         self.antenna_theta = nn.Parameter(th.to(dev, torch.float32).clone(), requires_grad=optimize_codebook)
         self.ms_theta = nn.ParameterList()
@@ -583,9 +597,7 @@ def time_to_target(channels: Dict[str, torch.Tensor], ms: Metasurface, steps: in
     the warm-up and capture)."""
     K = channels["src_to_ms{0}"].shape[1]
     M = channels["src_to_dst"].shape[0]
-    g = torch.Generator().manual_seed(seed)
-    th_ms = torch.rand(1, K, generator=g) * 2 * math.pi
-    th_ap = torch.rand(code_num, M, generator=g) * 2 * math.pi
+    th_ms, th_ap = start_phases(K, M, code_num, seed)
     out = {}
     for name, plain in (("plain_fp32", True), ("mfma_split_bf16", False)):
         # untimed: 3 steps of the same optimisation on a throwaway model, so the

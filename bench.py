@@ -102,6 +102,51 @@ def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int, office: bool = 
     return out
 
 
+def cpu_phase_map(channels: dict, max_phase_rad: float, target_db: float, tol_db: float = 0.1,
+                  max_steps: int = 120, code_num: int = 30, seed: int = 0) -> dict:
+    """Host baseline of the phase-map optimisation: the torch-CPU restatement
+    of go.py's Model (oracle.contract_ref: H_d + G diag(u) F, then the AP
+    codebook) with torch.optim.Adam (lr 0.3), on all host threads, in FP32
+    (go.py's precision) and FP64, from the same seeded start as the GPU run.
+    Runs until the mean best-code power is within tol_db of target_db (the
+    plain FP32 GPU optimum) or max_steps; the same step counting as the GPU
+    measurement (a step's coverage is that of its forward)."""
+    import torch
+
+    from automs_amd.phasemap import N0_90DBM, ap_amplitude, start_phases
+    from oracle import contract_ref as R
+    Hd = channels["src_to_dst"].cpu()
+    G = channels["src_to_ms{0}"].cpu()
+    F = channels["ms{0}_to_dst"].cpu()
+    M, K = G.shape
+    th_ms0, th_ap0 = start_phases(K, M, code_num, seed)
+    amp = ap_amplitude(code_num, M)
+    out = {"threads": torch.get_num_threads(), "kind": "port",
+           "what": "torch-CPU restatement of go.py's Model + torch.optim.Adam (oracle.contract_ref)"}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        cdt = torch.complex64 if dt == torch.float32 else torch.complex128
+        Hd_, G_, F_ = Hd.to(cdt), G.to(cdt), F.to(cdt)
+        ta = th_ap0.to(dt).clone().requires_grad_(True)
+        tm = th_ms0.reshape(-1).to(dt).clone().requires_grad_(True)
+        opt = torch.optim.Adam([ta, tm], lr=0.3)
+        hit = None
+        t0 = time.perf_counter()
+        for it in range(max_steps):
+            Y = R.received(amp, ta, Hd_, [G_], [F_], [tm], [max_phase_rad], dtype=dt)
+            loss = R.loss_logsnr(Y, 1, N0_90DBM)
+            if float(R.coverage_db(Y.detach())[0]) >= target_db - tol_db:
+                hit = it + 1
+                break
+            opt.zero_grad()
+            loss.backward()
+            opt.step()
+        dt_s = time.perf_counter() - t0
+        n = hit if hit is not None else max_steps
+        out[name] = {"steps_to_0p1dB": hit, "seconds_to_0p1dB": dt_s if hit is not None else None,
+                     "seconds_per_step": dt_s / n, "steps_run": n}
+    return out
+
+
 def contraction_office(iters: int = 20) -> dict:
     """Config-3 shape on traced office channels: 4 phase maps x 64 AP beams
     (M = 256) against [F0 ; H_d] (K = 2304 + 64, N = 65,536).  Times the
@@ -374,6 +419,10 @@ def main():
             "mfma_step_hip_graph": r["mfma_graph"], "mfma_ms_per_step": r["mfma_ms_per_step"],
             "plain_fp32_ms_per_step": r["plain_ms_per_step"],
             "timing": "HIP events on the stream from before the first step (warm-up and capture included)"}
+        if not args.no_cpu_baseline:
+            r_cpu = cpu_phase_map(ch, ms.max_phase_rad, r["plain_optimum_cov_db"])
+            print(f"phase map host baseline: {r_cpu}", file=sys.stderr, flush=True)
+            line["phase_map"]["cpu_baseline"] = r_cpu
     if not args.no_contract:
         r = contraction_office(iters=5 if args.profile_pass else 20)
         print(f"contraction: {r}", file=sys.stderr, flush=True)

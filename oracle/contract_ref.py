@@ -20,20 +20,23 @@ EPS32 = float(torch.finfo(torch.float32).eps)
 
 def received(antenna_A: torch.Tensor, antenna_theta: torch.Tensor, H_d: torch.Tensor,
              G_list: Sequence[torch.Tensor], F_list: Sequence[torch.Tensor], ms_theta: Sequence[torch.Tensor],
-             max_phase: Sequence[float], ms_A: Optional[Sequence[torch.Tensor]] = None) -> torch.Tensor:
+             max_phase: Sequence[float], ms_A: Optional[Sequence[torch.Tensor]] = None,
+             dtype: torch.dtype = torch.float64) -> torch.Tensor:
     """y = W (H_d + sum_i G_i diag(u_i) F_i), go.py:107-164; ms_theta[i] is
-    [K_i] or [maps, K_i] (then y is [maps * C, N])."""
-    W = antenna_A.to(torch.float64) * torch.exp(-1j * antenna_theta.to(torch.float64))
+    [K_i] or [maps, K_i] (then y is [maps * C, N]).  dtype float64 (the
+    oracle) or float32 (go.py's own precision, for the host baseline)."""
+    cdt = torch.complex128 if dtype == torch.float64 else torch.complex64
+    W = antenna_A.to(dtype) * torch.exp(-1j * antenna_theta.to(dtype))
     maps = ms_theta[0].reshape(-1, G_list[0].shape[1]).shape[0] if len(ms_theta) else 1
     out = []
     for m in range(maps):
-        ch = H_d.to(torch.complex128)
+        ch = H_d.to(cdt)
         for i in range(len(G_list)):
-            th = ms_theta[i].reshape(maps, -1)[m].to(torch.float64)
+            th = ms_theta[i].reshape(maps, -1)[m].to(dtype)
             real_phase = torch.remainder(th, max_phase[i])  # go.py:131
-            A = ms_A[i].to(torch.float64) if ms_A is not None else torch.ones_like(real_phase)
+            A = ms_A[i].to(dtype) if ms_A is not None else torch.ones_like(real_phase)
             u = A * torch.exp(1j * real_phase)
-            ch = ch + torch.matmul(G_list[i].to(torch.complex128) * u, F_list[i].to(torch.complex128))
+            ch = ch + torch.matmul(G_list[i].to(cdt) * u, F_list[i].to(cdt))
         out.append(torch.matmul(W, ch))
     return torch.cat(out, 0)
 

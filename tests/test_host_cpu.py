@@ -404,3 +404,33 @@ def test_fused_adam_flat_gradient_view():
     assert torch.equal(opt._flat_grad(), torch.cat([torch.ones(12), torch.full((10,), 2.0)]))
     b.grad = None
     assert torch.equal(opt._flat_grad(), torch.cat([torch.ones(12), torch.zeros(10)]))
+
+
+def test_contraction_oracle_fp32_mode_and_start_helpers():
+    """The oracle's FP32 mode (the host baseline of the phase-map optimiser)
+    returns complex64 within 2e-6 of its FP64 result; ap_amplitude follows
+    go.py:59-66 (1/sqrt(M/36), the corners of a 6x6 AP off); start_phases
+    is seeded and in [0, 2 pi)."""
+    from automs_amd.phasemap import ap_amplitude, start_phases
+    from oracle import contract_ref as R
+    g = torch.Generator().manual_seed(3)
+
+    def crand(*s, scale):
+        return (torch.randn(*s, generator=g, dtype=torch.float64)
+                + 1j * torch.randn(*s, generator=g, dtype=torch.float64)) * scale
+    M, K, N, Cc = 16, 64, 128, 7
+    Hd, G, F = crand(M, N, scale=3e-5), crand(M, K, scale=3e-4), crand(K, N, scale=3e-4)
+    ta = torch.rand(Cc, M, generator=g, dtype=torch.float64) * 2 * math.pi
+    tm = (torch.rand(K, generator=g, dtype=torch.float64) - 0.3) * 6 * math.pi
+    A = ap_amplitude(Cc, M)
+    y64 = R.received(A, ta, Hd, [G], [F], [tm], [5.0])
+    y32 = R.received(A, ta, Hd, [G], [F], [tm], [5.0], dtype=torch.float32)
+    assert y64.dtype == torch.complex128 and y32.dtype == torch.complex64
+    assert ((y32.to(torch.complex128) - y64).norm() / y64.norm()).item() < 2e-6
+    assert torch.allclose(A, torch.full((Cc, M), 1.5))
+    A36 = ap_amplitude(3, 36)
+    assert (A36[:, [0, 5, 30, 35]] == 0).all() and (A36.sum(1) == 32).all()
+    a1, b1 = start_phases(K, M, Cc, seed=4)
+    a2, b2 = start_phases(K, M, Cc, seed=4)
+    assert torch.equal(a1, a2) and torch.equal(b1, b2) and a1.shape == (1, K) and b1.shape == (Cc, M)
+    assert float(a1.min()) >= 0 and float(a1.max()) < 2 * math.pi
