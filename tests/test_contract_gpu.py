@@ -379,6 +379,39 @@ def test_graphed_optimiser_matches_eager(gpu, tmp_path):
         optimize_model(cap, step=2, graph=True)
 
 
+def test_graphed_optimiser_two_metasurfaces(gpu, tmp_path):
+    """optimize_model(graph=True) on a model with two metasurfaces, whose
+    operand comes from torch ops (a library complex GEMM among them): 8
+    replayed steps equal the eager loop within 1e-6 relative, and the
+    captured graph holds no memset node."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel, optimize_model
+    Hd, G0, F0 = _channels(M=16, K=128, N=600, seed=61)
+    _, G1, F1 = _channels(M=16, K=96, N=600, seed=62)
+    ch = {"src_to_dst": Hd, "src_to_ms{0}": G0, "ms{0}_to_dst": F0, "src_to_ms{1}": G1, "ms{1}_to_dst": F1}
+    ch = {k: v.to(torch.complex64).to(gpu) for k, v in ch.items()}
+    ms = [Metasurface(None, (8, 16), None, None, [], max_phase=360.0),
+          Metasurface(None, (8, 12), None, None, [], max_phase=300.0)]
+    gth = torch.Generator().manual_seed(9)
+    th = [torch.rand(1, 128, generator=gth) * 6.0, torch.rand(1, 96, generator=gth) * 6.0]
+    th_ap = torch.rand(30, 16, generator=gth) * 6.0
+    dot = tmp_path / "two.dot"
+    res = {}
+    for graph in (False, True):
+        _flush(f"case: two-metasurface optimiser 8 steps graph={graph}")
+        model = PhaseMapModel("t", ch, ms, code_num=30, ms_theta=th, antenna_theta=th_ap)
+        assert not model.fused_operand
+        optimize_model(model, lr=0.3, step=8, graph=graph, graph_dot=str(dot) if graph else None)
+        torch.cuda.synchronize()
+        res[graph] = [torch.tensor(model.loss_list, dtype=torch.float64)] + \
+            [p.detach().cpu().double().reshape(-1) for p in model.parameters()]
+    for a, b in zip(res[False], res[True]):
+        rel = ((a - b).norm() / a.norm()).item()
+        _flush(f"  rel {rel:.2e}")
+        assert rel <= 1e-6
+    if dot.exists():
+        assert "memset" not in dot.read_text().lower()
+
+
 def test_batched_maps_match_single(gpu):
     """Four phase maps in one GEMM give the per-map losses of four separate
     models."""
