@@ -84,10 +84,83 @@ __device__ __forceinline__ bf16x8 neg_bf(u32x4 v) {
 }
 #define MFMA(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
 
+// One 32-deep reduction block of a wave's MT x NT tiles: the A and B
+// fragments of block kb (rows and columns past the matrix are clamped to the
+// last block; they are never stored).
+template <int MT, int NT>
+__device__ __forceinline__ void load_blocks(const u32x4 *__restrict__ A, const u32x4 *__restrict__ B, int mg,
+                                            int ng, int ra_blocks, int rb_blocks, int n_kb, int kb, int lane,
+                                            u32x4 (&a)[MT][4], u32x4 (&b)[NT][4]) {
+#pragma unroll
+  for (int i = 0; i < MT; i++) {
+    int rb = mg * MT + i;
+    rb = rb < ra_blocks ? rb : ra_blocks - 1;
+    const u32x4 *p = A + ((long long)rb * n_kb + kb) * 256 + lane;
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[i][q] = p[q * 64];
+  }
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    int cb = ng * NT + j;
+    cb = cb < rb_blocks ? cb : rb_blocks - 1;
+    const u32x4 *p = B + ((long long)cb * n_kb + kb) * 256 + lane;
+#pragma unroll
+    for (int q = 0; q < 4; q++) b[j][q] = p[q * 64];
+  }
+}
+
+// The 12 MFMAs per tile pair of one reduction block: 4 real products x
+// (hi.hi, hi.lo, lo.hi), negation as a sign-bit XOR.
+template <int MT, int NT, bool CONJB>
+__device__ __forceinline__ void mac_blocks(const u32x4 (&a)[MT][4], const u32x4 (&b)[NT][4],
+                                           f32x4 (&acc_re)[MT][NT], f32x4 (&acc_im)[MT][NT]) {
+#pragma unroll
+  for (int i = 0; i < MT; i++) {
+    bf16x8 arh = as_bf(a[i][0]), arl = as_bf(a[i][1]), aih = as_bf(a[i][2]), ail = as_bf(a[i][3]);
+    bf16x8 narh = neg_bf(a[i][0]), narl = neg_bf(a[i][1]);
+    bf16x8 naih = neg_bf(a[i][2]), nail = neg_bf(a[i][3]);
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+      bf16x8 brh = as_bf(b[j][0]), brl = as_bf(b[j][1]), bih = as_bf(b[j][2]), bil = as_bf(b[j][3]);
+      f32x4 re = acc_re[i][j], im = acc_im[i][j];
+      // real part: ar*br -/+ ai*bi
+      re = MFMA(arh, brh, re);
+      re = MFMA(arh, brl, re);
+      re = MFMA(arl, brh, re);
+      if (CONJB) {
+        re = MFMA(aih, bih, re);
+        re = MFMA(aih, bil, re);
+        re = MFMA(ail, bih, re);
+      } else {
+        re = MFMA(naih, bih, re);
+        re = MFMA(naih, bil, re);
+        re = MFMA(nail, bih, re);
+      }
+      // imaginary part: ai*br +/- ar*bi
+      im = MFMA(aih, brh, im);
+      im = MFMA(aih, brl, im);
+      im = MFMA(ail, brh, im);
+      if (CONJB) {
+        im = MFMA(narh, bih, im);
+        im = MFMA(narh, bil, im);
+        im = MFMA(narl, bih, im);
+      } else {
+        im = MFMA(arh, bih, im);
+        im = MFMA(arh, bil, im);
+        im = MFMA(arl, bih, im);
+      }
+      acc_re[i][j] = re;
+      acc_im[i][j] = im;
+    }
+  }
+}
+
 // C[Ra][Rb] (complex f32) = A . B^T  (CONJB: A . conj(B)^T) over reduction
 // blocks [kb0, kb1).  One wave owns MT x NT tiles of 16 x 16; waves of a
 // workgroup share the column group (so B blocks are reused through L1/L2).
-// With splits > 1 (gridDim.z), partial sums go to part[z][Ra][Rb].
+// The fragments of the next block are loaded while the current one is
+// multiplied (two register sets, the loop unrolled by two).  With splits > 1
+// (gridDim.z), partial sums go to part[z][Ra][Rb].
 template <int MT, int NT, bool CONJB>
 __global__ void __launch_bounds__(256) k_cgemm(const u32x4 *__restrict__ A, const u32x4 *__restrict__ B, int Ra,
                                                int Rb, int n_kb, int m_groups, int n_groups,
@@ -108,63 +181,16 @@ __global__ void __launch_bounds__(256) k_cgemm(const u32x4 *__restrict__ A, cons
       acc_re[i][j] = (f32x4){0, 0, 0, 0};
       acc_im[i][j] = (f32x4){0, 0, 0, 0};
     }
-  for (int kb = kb0; kb < kb1; kb++) {
-    u32x4 a[MT][4], b[NT][4];
-#pragma unroll
-    for (int i = 0; i < MT; i++) {
-      int rb = mg * MT + i;
-      rb = rb < ra_blocks ? rb : ra_blocks - 1;  // clamped rows are never stored
-      const u32x4 *p = A + ((long long)rb * n_kb + kb) * 256 + lane;
-#pragma unroll
-      for (int q = 0; q < 4; q++) a[i][q] = p[q * 64];
-    }
-#pragma unroll
-    for (int j = 0; j < NT; j++) {
-      int cb = ng * NT + j;
-      cb = cb < rb_blocks ? cb : rb_blocks - 1;
-      const u32x4 *p = B + ((long long)cb * n_kb + kb) * 256 + lane;
-#pragma unroll
-      for (int q = 0; q < 4; q++) b[j][q] = p[q * 64];
-    }
-#pragma unroll
-    for (int i = 0; i < MT; i++) {
-      bf16x8 arh = as_bf(a[i][0]), arl = as_bf(a[i][1]), aih = as_bf(a[i][2]), ail = as_bf(a[i][3]);
-      bf16x8 narh = neg_bf(a[i][0]), narl = neg_bf(a[i][1]);
-      bf16x8 naih = neg_bf(a[i][2]), nail = neg_bf(a[i][3]);
-#pragma unroll
-      for (int j = 0; j < NT; j++) {
-        bf16x8 brh = as_bf(b[j][0]), brl = as_bf(b[j][1]), bih = as_bf(b[j][2]), bil = as_bf(b[j][3]);
-        f32x4 re = acc_re[i][j], im = acc_im[i][j];
-        // real part: ar*br -/+ ai*bi
-        re = MFMA(arh, brh, re);
-        re = MFMA(arh, brl, re);
-        re = MFMA(arl, brh, re);
-        if (CONJB) {
-          re = MFMA(aih, bih, re);
-          re = MFMA(aih, bil, re);
-          re = MFMA(ail, bih, re);
-        } else {
-          re = MFMA(naih, bih, re);
-          re = MFMA(naih, bil, re);
-          re = MFMA(nail, bih, re);
-        }
-        // imaginary part: ai*br +/- ar*bi
-        im = MFMA(aih, brh, im);
-        im = MFMA(aih, brl, im);
-        im = MFMA(ail, brh, im);
-        if (CONJB) {
-          im = MFMA(narh, bih, im);
-          im = MFMA(narh, bil, im);
-          im = MFMA(narl, bih, im);
-        } else {
-          im = MFMA(arh, bih, im);
-          im = MFMA(arh, bil, im);
-          im = MFMA(arl, bih, im);
-        }
-        acc_re[i][j] = re;
-        acc_im[i][j] = im;
-      }
-    }
+  u32x4 a0[MT][4], b0[NT][4], a1[MT][4], b1[NT][4];
+  int kb = kb0;
+  if (kb < kb1) load_blocks<MT, NT>(A, B, mg, ng, ra_blocks, rb_blocks, n_kb, kb, lane, a0, b0);
+  while (kb < kb1) {
+    if (kb + 1 < kb1) load_blocks<MT, NT>(A, B, mg, ng, ra_blocks, rb_blocks, n_kb, kb + 1, lane, a1, b1);
+    mac_blocks<MT, NT, CONJB>(a0, b0, acc_re, acc_im);
+    if (++kb >= kb1) break;
+    if (kb + 1 < kb1) load_blocks<MT, NT>(A, B, mg, ng, ra_blocks, rb_blocks, n_kb, kb + 1, lane, a0, b0);
+    mac_blocks<MT, NT, CONJB>(a1, b1, acc_re, acc_im);
+    ++kb;
   }
   // D layout of 16x16 MFMA: lane l holds rows 4*(l>>4)+v, column l&15
   float *out = splits > 1 ? part + (long long)blockIdx.z * 2 * (long long)Ra * Rb : C;
