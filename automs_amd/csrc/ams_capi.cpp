@@ -95,6 +95,7 @@ int ams_scene_create(const float *tri, int32_t n_tri, const int32_t *mat, int32_
 int ams_scene_destroy(ams_scene *s) {
   if (!s) return AMS_OK;
   s->scratch.release();
+  s->stage.release();
   ams::dfree(s->d_tnodes);
   ams::dfree(s->d_tris);
   ams::dfree(s->d_aux);

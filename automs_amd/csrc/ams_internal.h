@@ -32,6 +32,32 @@ int dev_upload(T **dst, const void *src, size_t bytes) {
 }
 
 // Named device buffers that only grow; reused by later calls on the handle.
+// Page-locked host staging for the per-round uploads and read-backs of a
+// trace: the copies run on the DMA engines and need no compute units, so
+// they do not queue behind other traces' kernels.  Grows as needed; the
+// caller synchronises its stream before asking again.
+struct PinnedStage {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void *get(size_t n) {
+    if (n <= bytes) return p;
+    release();
+    size_t want = n + n / 4;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      set_error("pinned host allocation failed");
+      return nullptr;
+    }
+    bytes = want;
+    return p;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 struct Scratch {
   std::map<std::string, std::pair<void *, size_t>> bufs;
   void *get(const char *name, size_t bytes) {
@@ -56,7 +82,11 @@ struct Scratch {
     for (auto &kv : bufs)
       if (kv.second.first) dfree(kv.second.first);
     bufs.clear();
+    pin.release();
   }
+  // page-locked host bytes for small read-backs (counts, counters)
+  void *host(size_t bytes) { return pin.get(bytes); }
+  PinnedStage pin;
 };
 
 }  // namespace ams
@@ -76,6 +106,7 @@ struct ams_scene {
   float *d_mat_prop = nullptr;
   float *d_refl_phase = nullptr;
   ams::Scratch scratch;
+  ams::PinnedStage stage;  // host side of the per-round task uploads
   long long perm_n = -1;  // fib_n of the cached ray permutation in scratch "perm"
 };
 

@@ -1574,6 +1574,18 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
 }
 
 // (hi, lo) sort + unique in place on device; returns the unique count.
+// Small device-to-host read-back through the scene's page-locked buffer (a
+// DMA copy: it does not wait for compute units held by other streams'
+// kernels), then a wait on the stream.
+static hipError_t read_back(Scratch &sc, void *dst, const void *src, size_t bytes, hipStream_t st) {
+  void *h = sc.host(bytes);
+  if (!h) return hipErrorOutOfMemory;
+  hipError_t e = hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) std::memcpy(dst, h, bytes);
+  return e;
+}
+
 static int sort_unique(Scratch &sc, uint64_t *hi, uint64_t *lo, size_t n, hipStream_t st, size_t *n_out,
                        KeyBits kb = KeyBits{0, 0}) {
   if (n == 0) {
@@ -1600,8 +1612,7 @@ static int sort_unique(Scratch &sc, uint64_t *hi, uint64_t *lo, size_t n, hipStr
     HIPCHK(hipGetLastError());
     HIPCHK(hipcub::DeviceSelect::Flagged(tmp, tb3, c2, flag, c1, cnt, (int)n, st));
     size_t m = 0;
-    HIPCHK(hipMemcpyAsync(&m, cnt, sizeof(m), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(read_back(sc, &m, cnt, sizeof(m), st));
     if (m) k_key_expand<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(c1, m, kb, hi, lo);
     HIPCHK(hipGetLastError());
     *n_out = m;
@@ -1626,8 +1637,7 @@ static int sort_unique(Scratch &sc, uint64_t *hi, uint64_t *lo, size_t n, hipStr
   HIPCHK(hipcub::DeviceSelect::Flagged(tmp, tb3, hi, flag, hi2, cnt, (int)n, st));
   HIPCHK(hipcub::DeviceSelect::Flagged(tmp, tb3, lo, flag, lo2, cnt + 1, (int)n, st));
   size_t hc[2];
-  HIPCHK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(read_back(sc, hc, cnt, sizeof(hc), st));
   HIPCHK(hipMemcpyAsync(hi, hi2, hc[0] * 8, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(lo, lo2, hc[0] * 8, hipMemcpyDeviceToDevice, st));
   *n_out = hc[0];
@@ -1917,8 +1927,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     S.n_discover_launches++;
     HIPCHK(hipEventRecord(e1, st));
     unsigned long long hcnt = 0;
-    HIPCHK(hipMemcpyAsync(&hcnt, d_cnt, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(read_back(sc, &hcnt, d_cnt, 8, st));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     disc_ms += ms;
@@ -1971,8 +1980,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     S.n_discover_launches++;
     HIPCHK(hipEventRecord(e1, st));
     unsigned long long hcnt = 0;
-    HIPCHK(hipMemcpyAsync(&hcnt, d_cnt, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(read_back(sc, &hcnt, d_cnt, 8, st));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     disc_ms += ms;
@@ -2043,10 +2051,14 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     if (rc) return rc;
     acc_hi.resize(m);
     acc_lo.resize(m);
-    if (m) {
-      HIPCHK(hipMemcpyAsync(acc_hi.data(), a_hi, m * 8, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(acc_lo.data(), a_lo, m * 8, hipMemcpyDeviceToHost, st));
+    if (m) {  // through page-locked memory, as read_back
+      uint64_t *h = (uint64_t *)sc.host(16 * m);
+      if (!h) return AMS_ERR_ALLOC;
+      HIPCHK(hipMemcpyAsync(h, a_hi, m * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(h + m, a_lo, m * 8, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
+      std::memcpy(acc_hi.data(), h, m * 8);
+      std::memcpy(acc_lo.data(), h + m, m * 8);
     }
   }
   n_unique_total = acc_hi.size();
@@ -2054,8 +2066,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   S.records_unique = (long long)n_unique_total;
   S.t_discover_ms = disc_ms;
   unsigned long long hctr[16];
-  HIPCHK(hipMemcpyAsync(hctr, d_ctr, sizeof hctr, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(read_back(sc, hctr, d_ctr, sizeof hctr, st));
   S.fwd_nodes = (long long)hctr[0];
   S.fwd_tris = (long long)hctr[1];
   S.fwd_spheres = (long long)hctr[2];
@@ -2166,15 +2177,22 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         k = e;
       }
       int nu = (int)ut.size();
-      int *d_tp = (int *)sc.get("t_path", 4 * nt);
-      unsigned char *d_ok = (unsigned char *)sc.get("t_ok", nt);
-      int *d_u = (int *)sc.get("units", 12 * (size_t)nu);
-      if (!d_tp || !d_ok || !d_u) return AMS_ERR_ALLOC;
-      HIPCHK(hipMemcpyAsync(d_tp, tp.data(), 4 * nt, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(d_u, ut.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(d_u + nu, ub.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(d_u + 2 * nu, ue.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemsetAsync(d_ok, 0, nt, st));
+      // one upload per round from page-locked memory: task paths | units
+      // (tile, begin, end) | success flags (zero); the flags come back into
+      // the bytes after it
+      const size_t up = 4 * nt + 12 * (size_t)nu + nt;
+      char *d_round = (char *)sc.get("t_round", up);
+      char *h_round = (char *)s->stage.get(up + nt);
+      if (!d_round || !h_round) return AMS_ERR_ALLOC;
+      std::memcpy(h_round, tp.data(), 4 * nt);
+      std::memcpy(h_round + 4 * nt, ut.data(), 4 * (size_t)nu);
+      std::memcpy(h_round + 4 * nt + 4 * (size_t)nu, ub.data(), 4 * (size_t)nu);
+      std::memcpy(h_round + 4 * nt + 8 * (size_t)nu, ue.data(), 4 * (size_t)nu);
+      std::memset(h_round + 4 * nt + 12 * (size_t)nu, 0, nt);
+      HIPCHK(hipMemcpyAsync(d_round, h_round, up, hipMemcpyHostToDevice, st));
+      int *d_tp = (int *)d_round;
+      int *d_u = (int *)(d_round + 4 * nt);
+      unsigned char *d_ok = (unsigned char *)(d_round + 4 * nt + 12 * (size_t)nu);
       ea.task_path = d_tp;
       ea.unit_tile = d_u;
       ea.unit_begin = d_u + nu;
@@ -2214,8 +2232,8 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         S.n_evaluate_launches++;
       }
       HIPCHK(hipEventRecord(e1, st));
-      std::vector<unsigned char> ok(nt);
-      HIPCHK(hipMemcpyAsync(ok.data(), d_ok, nt, hipMemcpyDeviceToHost, st));
+      const unsigned char *ok = (const unsigned char *)h_round + up;
+      HIPCHK(hipMemcpyAsync(h_round + up, d_ok, nt, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
@@ -2265,8 +2283,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     k_add_h<<<(unsigned)((nH + 255) / 256), 256, 0, st>>>(d_H, H_out, nH);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipMemcpyAsync(hctr, d_ctr, sizeof hctr, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(read_back(sc, hctr, d_ctr, sizeof hctr, st));
   S.bwd_nodes = (long long)hctr[0];
   S.bwd_tris = (long long)hctr[1];
   S.bwd_queries = (long long)hctr[3];
