@@ -111,6 +111,8 @@ def _bind_contract(L):
         "ams_objective_bwd": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp, vp], C.c_int),
         "ams_pm_operand": ([vp, vp, vp, vp, vp, i32, i32, i32, i32, f32, vp, vp, vp, vp], C.c_int),
         "ams_pm_operand_bwd": ([vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp], C.c_int),
+        "ams_pm_operand_frag": ([vp, vp, vp, vp, vp, i32, i32, i32, i32, f32, vp, vp, vp, vp], C.c_int),
+        "ams_objective_bwd_frag": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp, vp], C.c_int),
         "ams_adam_step": ([vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp], C.c_int),
         "ams_adam_step_dev": ([vp, vp, vp, vp, i64, f32, f32, f32, f32, vp, vp, vp, vp, i64, vp], C.c_int),
         "ams_host_bvh_build": ([vp, vp, i32, i32, vp, i32, vp, vp], C.c_int),
@@ -140,7 +142,8 @@ EXPORTED = [
     "ams_last_error", "ams_version", "ams_mem_outstanding",
     "ams_frag_bytes", "ams_frag_pack", "ams_cgemm_workspace_bytes", "ams_cgemm_frag", "ams_cgemm_f32",
     "ams_objective_workspace_bytes", "ams_objective_fwd", "ams_objective_bwd", "ams_adam_step",
-    "ams_adam_step_dev", "ams_objective_fwd_cov", "ams_pm_operand", "ams_pm_operand_bwd",
+    "ams_adam_step_dev", "ams_objective_fwd_cov", "ams_pm_operand", "ams_pm_operand_bwd", "ams_pm_operand_frag",
+    "ams_objective_bwd_frag",
     "ams_host_bvh_build", "ams_host_bvh4_build", "ams_host_plane_ids",
     "ams_conv3x3_fwd", "ams_conv3x3_bwd_weight", "ams_relu_bwd", "ams_mse_partials", "ams_mse", "ams_to_bf16",
 ]

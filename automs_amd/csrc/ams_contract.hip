@@ -7,10 +7,7 @@
 // the lane order of v_mfma_f32_16x16x32_bf16; each real product is
 // hi*hi + hi*lo + lo*hi with FP32 accumulation (relative error ~1e-5).
 //
-// Fragment layout of a complex matrix X[R][Kd] (row r, reduction index k):
-//   frag[rb][kb][plane][lane][8]  (bf16), rb = r/16, kb = k/32,
-//   lane l holds X[rb*16 + (l&15)][kb*32 + 8*(l>>4) + e], e = 0..7,
-//   planes: 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo.  One (rb, kb) block = 4 KB.
+// Fragment layout of a complex matrix X[R][Kd]: ams_frag.h.
 // The same layout serves the A operand (rows = output rows) and the B
 // operand (rows = output columns), so C = A . B^T with both operands
 // contiguous along the reduction index.
@@ -21,21 +18,13 @@
 #include <string>
 
 #include "../../include/ams.h"
+#include "ams_frag.h"
 #include "ams_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
-
-__device__ __forceinline__ unsigned short bf16_rne(float x) {
-  unsigned int u = __float_as_uint(x);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf / nan
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
 // One thread per (rb, kb, lane): 8 complex values -> 4 planes of 8 bf16.
 __global__ void __launch_bounds__(256) k_frag_pack(const float *__restrict__ X, int R, int Kd, long long sr,
@@ -43,38 +32,18 @@ __global__ void __launch_bounds__(256) k_frag_pack(const float *__restrict__ X, 
                                                    u32x4 *__restrict__ frag) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_items) return;
-  int lane = (int)(t & 63);
-  long long blk = t >> 6;
-  int kb = (int)(blk % n_kb);
-  long long rb = blk / n_kb;
-  long long r = rb * 16 + (lane & 15);
-  int k0 = kb * 32 + 8 * (lane >> 4);
-  unsigned short p[4][8];
+  const ams::FragItem it = ams::frag_item(t, n_kb);
+  float2 v[8];
 #pragma unroll
   for (int e = 0; e < 8; e++) {
-    float re = 0.0f, im = 0.0f;
-    int k = k0 + e;
-    if (r < R && k < Kd) {
-      const float *src = X + 2 * (r * sr + (long long)k * sk);
-      re = src[0];
-      im = src[1];
+    v[e] = make_float2(0.0f, 0.0f);
+    int k = it.k0 + e;
+    if (it.r < R && k < Kd) {
+      const float *src = X + 2 * (it.r * sr + (long long)k * sk);
+      v[e] = make_float2(src[0], src[1]);
     }
-    unsigned short rh = bf16_rne(re), ih = bf16_rne(im);
-    p[0][e] = rh;
-    p[1][e] = bf16_rne(re - bf16_to_f(rh));
-    p[2][e] = ih;
-    p[3][e] = bf16_rne(im - bf16_to_f(ih));
   }
-  u32x4 *dst = frag + blk * 256 + lane;  // 4 planes x 64 lanes of 16 B
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    u32x4 v;
-    v.x = p[q][0] | ((unsigned)p[q][1] << 16);
-    v.y = p[q][2] | ((unsigned)p[q][3] << 16);
-    v.z = p[q][4] | ((unsigned)p[q][5] << 16);
-    v.w = p[q][6] | ((unsigned)p[q][7] << 16);
-    dst[q * 64] = v;
-  }
+  ams::frag_store8(frag, it, v);
 }
 
 __device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }

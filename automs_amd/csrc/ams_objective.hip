@@ -14,6 +14,7 @@
 #include <string>
 
 #include "../../include/ams.h"
+#include "ams_frag.h"
 #include "ams_internal.h"
 
 namespace {
@@ -155,6 +156,35 @@ __global__ void __launch_bounds__(64) k_obj_finish(const double *__restrict__ pa
   if (cov) cov[g] = (float)(s3 / N);
 }
 
+// d L / d y of one element (code c of a group, one Rx), in torch's convention
+// for complex gradients.  bi: the Rx's argmax code; best: its maximum
+// capacity (mode 1); fro: the Frobenius norm of the non-maximum capacities
+// (mode 1); masked: the code's row is masked (mode 1).
+__device__ __forceinline__ float2 obj_grad(float2 y, int c, int bi, float best, float fro, bool masked, int mode,
+                                           float n0, float w, float gl, int N) {
+  float2 out = make_float2(0.0f, 0.0f);
+  if (mode == 0) {
+    if (c == bi) {
+      float s = y.x * y.x + y.y * y.y;
+      // dL/ds = -(1/N) / (ln2 (N0 + s)); g = 2 y dL/ds
+      float dls = -gl * kInvLn2 / ((float)N * (n0 + s));
+      out = make_float2(2.0f * dls * y.x, 2.0f * dls * y.y);
+    }
+    return out;
+  }
+  if (masked) return out;
+  float v = cap_of(y, n0);
+  float dcap = 0.0f;
+  if (c == bi) dcap -= w;
+  if (v != best && fro > 0.0f) dcap += v / fro;
+  float amp = sqrtf(y.x * y.x + y.y * y.y);
+  float a = amp + kEps32;
+  float dca = kInvLn2 * 2.0f * a / n0 / (1.0f + a * a / n0);  // dcap / d|y|
+  float gabs = gl * dcap * dca;
+  if (amp > 0.0f) out = make_float2(gabs * y.x / amp, gabs * y.y / amp);
+  return out;
+}
+
 __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_bwd(const float2 *__restrict__ Y, int C, int N,
                                                                 float n0, int mode, float w,
                                                                 const unsigned char *__restrict__ rowmask,
@@ -173,17 +203,9 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_bwd(const float2 *__
   const int bi = ok ? argmax[(size_t)g * N + r] : 0;
   if (mode == 0) {
     if (!ok) return;
-    for (int c = q; c < C; c += kObjGroups) {
-      float2 out = make_float2(0.0f, 0.0f);
-      if (c == bi) {
-        float2 y = Yg[(size_t)c * N + r];
-        float s = y.x * y.x + y.y * y.y;
-        // dL/ds = -(1/N) / (ln2 (N0 + s)); g = 2 y dL/ds
-        float dls = -gl * kInvLn2 / ((float)N * (n0 + s));
-        out = make_float2(2.0f * dls * y.x, 2.0f * dls * y.y);
-      }
-      Gg[(size_t)c * N + r] = out;
-    }
+    for (int c = q; c < C; c += kObjGroups)
+      Gg[(size_t)c * N + r] =
+          c == bi ? obj_grad(Yg[(size_t)c * N + r], c, bi, 0.0f, 0.0f, false, 0, n0, w, gl, N) : make_float2(0, 0);
     return;
   }
   // mode 1: the maximum capacity of the Rx (over all four groups) first
@@ -198,22 +220,48 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_bwd(const float2 *__
   best = fmaxf(fmaxf(val[0][rl], val[1][rl]), fmaxf(val[2][rl], val[3][rl]));
   if (!ok) return;
   const float fro = (float)sqrt(sums[2 * g + 1]);
-  for (int c = q; c < C; c += kObjGroups) {
-    float2 out = make_float2(0.0f, 0.0f);
-    if (!(rowmask && rowmask[c])) {
-      float2 y = Yg[(size_t)c * N + r];
-      float v = cap_of(y, n0);
-      float dcap = 0.0f;
-      if (c == bi) dcap -= w;
-      if (v != best && fro > 0.0f) dcap += v / fro;
-      float amp = sqrtf(y.x * y.x + y.y * y.y);
-      float a = amp + kEps32;
-      float dca = kInvLn2 * 2.0f * a / n0 / (1.0f + a * a / n0);  // dcap / d|y|
-      float gabs = gl * dcap * dca;
-      if (amp > 0.0f) out = make_float2(gabs * y.x / amp, gabs * y.y / amp);
+  for (int c = q; c < C; c += kObjGroups)
+    Gg[(size_t)c * N + r] =
+        obj_grad(Yg[(size_t)c * N + r], c, bi, best, fro, rowmask && rowmask[c], 1, n0, w, gl, N);
+}
+
+// The same gradient written straight into the split-bf16 fragment layout of
+// the backward contraction (ams_frag.h; rows = the G*C codes, reduction
+// index = Rx): one thread per fragment lane, 8 consecutive Rx of one code.
+// The maximum capacity of an Rx (mode 1) is that of its argmax code.  Equal
+// to ams_frag_pack of k_obj_bwd's output.
+__global__ void __launch_bounds__(256) k_obj_bwd_frag(const float2 *__restrict__ Y, int G, int C, int N, float n0,
+                                                      int mode, float w, const unsigned char *__restrict__ rowmask,
+                                                      const int *__restrict__ argmax,
+                                                      const double *__restrict__ sums,
+                                                      const float *__restrict__ gloss, int n_kb, long long n_items,
+                                                      u32x4 *__restrict__ frag) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_items) return;
+  const ams::FragItem it = ams::frag_item(t, n_kb);
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) v[e] = make_float2(0.0f, 0.0f);
+  if (it.r < (long long)G * C) {
+    const int g = (int)(it.r / C), c = (int)(it.r % C);
+    const float2 *Yg = Y + (size_t)g * C * N;
+    const float gl = gloss[g];
+    const float fro = mode != 0 ? (float)sqrt(sums[2 * g + 1]) : 0.0f;
+    const bool masked = mode != 0 && rowmask && rowmask[c];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int rx = it.k0 + e;
+      if (rx >= N) continue;
+      const int bi = argmax[(size_t)g * N + rx];
+      if (mode == 0) {
+        if (c == bi) v[e] = obj_grad(Yg[(size_t)c * N + rx], c, bi, 0.0f, 0.0f, false, 0, n0, w, gl, N);
+      } else {
+        const float best = rowmask && rowmask[bi] ? 0.0f : cap_of(Yg[(size_t)bi * N + rx], n0);
+        v[e] = obj_grad(Yg[(size_t)c * N + rx], c, bi, best, fro, masked, 1, n0, w, gl, N);
+      }
     }
-    Gg[(size_t)c * N + r] = out;
   }
+  ams::frag_store8(frag, it, v);
 }
 
 // torch.optim.Adam (no weight decay, no amsgrad), single-tensor form:
@@ -353,6 +401,30 @@ int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t m
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ams::set_error(std::string("ams_objective_bwd: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_objective_bwd_frag(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                           const unsigned char *rowmask, const float *gloss, const void *workspace, void *gY_frag,
+                           void *stream) {
+  if (!Y || !gY_frag || !gloss || !workspace || G <= 0 || C <= 0 || N <= 0 || (mode != 0 && mode != 1)) {
+    ams::set_error("ams_objective_bwd_frag: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  int nb = (N + kObjRx - 1) / kObjRx;
+  const double *partial = (const double *)workspace;
+  const double *sums = partial + 3 * (size_t)G * nb;
+  const int *argmax = (const int *)(sums + 2 * (size_t)G);
+  const long long R = (long long)G * C;
+  const int n_kb = (N + 31) / 32;
+  const long long n_items = ((R + 15) / 16) * n_kb * 64;
+  k_obj_bwd_frag<<<(unsigned)((n_items + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      (const float2 *)Y, G, C, N, n0, mode, w, rowmask, argmax, sums, gloss, n_kb, n_items, (u32x4 *)gY_frag);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_objective_bwd_frag: ") + hipGetErrorString(e));
     return AMS_ERR_HIP;
   }
   return AMS_OK;

@@ -24,18 +24,29 @@
 #include <string>
 
 #include "../../include/ams.h"
+#include "ams_frag.h"
 #include "ams_internal.h"
 
 namespace {
 
 constexpr int kPmBlock = 256;
-constexpr int kPmMaxM = 4096;  // AP elements (LDS row of W: 32 KB)
+constexpr int kPmMaxM = 4096;      // AP elements (LDS row of W: 32 KB)
+constexpr int kPmFragMaxM = 256;   // AP elements of the fragment writer (16 rows of W: 32 KB)
 constexpr int kPmApBlock = 256;
 constexpr int kPmMsK = 64;  // metasurface elements per workgroup of k_pm_bwd_ms
 constexpr int kPmMsC = 4;   // code groups (waves) per workgroup of k_pm_bwd_ms
 
+// a * b and acc + a * b with every rounding fixed (explicit fma): the
+// operand kernels below form A the same way, value for value.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cmac(float2 acc, float2 a, float2 b) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
+  return acc;
 }
 // a * conj(b)
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
@@ -84,11 +95,7 @@ __global__ void __launch_bounds__(kPmBlock) k_pm_operand(const float *__restrict
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   float2 acc = make_float2(0.0f, 0.0f);
-  for (int j = 0; j < M; j++) {
-    float2 w = w_row[j], g = G[(size_t)j * K + k];
-    acc.x += w.x * g.x - w.y * g.y;
-    acc.y += w.x * g.y + w.y * g.x;
-  }
+  for (int j = 0; j < M; j++) acc = cmac(acc, w_row[j], G[(size_t)j * K + k]);
   WG[(size_t)c * K + k] = acc;
   const float a = ms_amp[k];
   for (int m = 0; m < nm; m++) {
@@ -96,6 +103,67 @@ __global__ void __launch_bounds__(kPmBlock) k_pm_operand(const float *__restrict
     if (c == 0) U[(size_t)m * K + k] = u;
     A[(size_t)(m * Cn + c) * Kc + k] = cmul(acc, u);
   }
+}
+
+// The same operand written straight into the split-bf16 fragment layout of
+// the contraction (ams_frag.h) instead of a complex matrix.  One workgroup
+// per 16 x 32 fragment block: W of its 16 rows in LDS, two elements per lane
+// (the same products in the same order as k_pm_operand, so the fragments
+// equal ams_frag_pack of that kernel's A), the block's values gathered in
+// LDS and packed by one wave.  Lanes of map 0 also write WG, lanes of code 0
+// write u.  Dynamic LDS: 16 * M float2.
+__global__ void __launch_bounds__(kPmBlock) k_pm_operand_frag(const float *__restrict__ th_ap,
+                                                              const float *__restrict__ amp,
+                                                              const float *__restrict__ th_ms,
+                                                              const float *__restrict__ ms_amp,
+                                                              const float2 *__restrict__ G, int Cn, int M, int K,
+                                                              int nm, float max_phase, int n_kb,
+                                                              u32x4 *__restrict__ frag, float2 *__restrict__ WG,
+                                                              float2 *__restrict__ U) {
+  extern __shared__ float2 w_rows[];  // [16][M]
+  __shared__ float2 tile[16][32];
+  const int R = nm * Cn, Kc = K + M;
+  const int kb = blockIdx.x % n_kb, rb = blockIdx.x / n_kb;
+  for (int idx = threadIdx.x; idx < 16 * M; idx += blockDim.x) {
+    const int rl = idx / M, j = idx % M;
+    const int r = rb * 16 + rl;
+    float2 w = make_float2(0.0f, 0.0f);
+    if (r < R) {
+      const int c = r % Cn;
+      w = ap_weight(amp[(size_t)c * M + j], th_ap[(size_t)c * M + j]);
+    }
+    w_rows[idx] = w;
+  }
+  __syncthreads();
+  const int rl = threadIdx.x / 16, kl0 = 2 * (threadIdx.x % 16);
+  const int r = rb * 16 + rl;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int kl = kl0 + h, k = kb * 32 + kl;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (r < R) {
+      const int c = r % Cn, m = r / Cn;
+      const float2 *wr = w_rows + rl * M;
+      if (k < K) {
+        float2 acc = make_float2(0.0f, 0.0f);
+        for (int j = 0; j < M; j++) acc = cmac(acc, wr[j], G[(size_t)j * K + k]);
+        const float2 u = ms_coef(ms_amp[k], th_ms[(size_t)m * K + k], max_phase);
+        v = cmul(acc, u);
+        if (m == 0) WG[(size_t)c * K + k] = acc;
+        if (c == 0) U[(size_t)m * K + k] = u;
+      } else if (k < Kc) {
+        v = wr[k - K];
+      }
+    }
+    tile[rl][kl] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const ams::FragItem it = ams::frag_item((long long)blockIdx.x * 64 + threadIdx.x, n_kb);
+  float2 v8[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) v8[e] = tile[threadIdx.x & 15][8 * (threadIdx.x >> 4) + e];
+  ams::frag_store8(frag, it, v8);
 }
 
 // Workgroup = 64 metasurface elements k (one per lane) x 4 waves, wave w
@@ -211,6 +279,28 @@ int ams_pm_operand(const float *th_ap, const float *amp, const float *th_ms, con
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ams::set_error(std::string("ams_pm_operand: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_pm_operand_frag(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp,
+                        const float *G, int32_t Cn, int32_t M, int32_t K, int32_t nm, float max_phase, void *frag,
+                        float *WG, float *U, void *stream) {
+  if (!th_ap || !amp || !th_ms || !ms_amp || !G || !frag || !WG || !U || !pm_dims_ok(Cn, M, K, nm) ||
+      M > kPmFragMaxM || !(max_phase > 0.0f)) {
+    ams::set_error("ams_pm_operand_frag: bad arguments (M <= 256)");
+    return AMS_ERR_ARG;
+  }
+  const long long R = (long long)nm * Cn, Kc = (long long)K + M;
+  const int n_kb = (int)((Kc + 31) / 32);
+  const long long n_blocks = ((R + 15) / 16) * n_kb;
+  k_pm_operand_frag<<<(unsigned)n_blocks, kPmBlock, (size_t)16 * M * sizeof(float2), (hipStream_t)stream>>>(
+      th_ap, amp, th_ms, ms_amp, (const float2 *)G, Cn, M, K, nm, max_phase, n_kb, (u32x4 *)frag, (float2 *)WG,
+      (float2 *)U);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_pm_operand_frag: ") + hipGetErrorString(e));
     return AMS_ERR_HIP;
   }
   return AMS_OK;

@@ -156,24 +156,90 @@ class PhaseOperand(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gA: torch.Tensor):
-        L = _lib.lib()
         th_ap, amp, G, WG, U = ctx.saved_tensors
-        Cn, M, K, nm = ctx.dims
-        gA = gA.contiguous()
-        dev = gA.device
-        need_ap = ctx.needs_input_grad[0]
-        # both gradients in one buffer, in FusedAdam's parameter order (AP
-        # phases, then metasurface phases), so its step needs no concatenation
-        n_ap = Cn * M if need_ap else 0
-        gbuf = torch.empty(n_ap + nm * K, dtype=torch.float32, device=dev)
-        g_ap = gbuf[:n_ap].view(Cn, M) if need_ap else None
-        g_ms = gbuf[n_ap:].view(nm, K)
-        gWG = torch.empty(Cn, K, dtype=torch.complex64, device=dev) if need_ap else None
-        _lib.check(L.ams_pm_operand_bwd(_ptr(gA), _ptr(th_ap), _ptr(amp), _ptr(torch.view_as_real(G)), _ptr(WG),
-                                        _ptr(U), Cn, M, K, nm, _ptr(g_ms), _ptr(g_ap) if need_ap else None,
-                                        _ptr(gWG) if need_ap else None, _stream()),
-                   "ams_pm_operand_bwd")
+        g_ap, g_ms = _operand_grads(gA, th_ap, amp, G, WG, U, ctx.dims, ctx.needs_input_grad[0])
         return g_ap, g_ms, None, None, None, None
+
+
+def _operand_grads(gA: torch.Tensor, th_ap, amp, G, WG, U, dims, need_ap: bool):
+    """The phase gradients from gA (ams_pm_operand_bwd).  Both land in one
+    buffer in FusedAdam's parameter order (AP phases, then metasurface
+    phases), so its step needs no concatenation."""
+    L = _lib.lib()
+    Cn, M, K, nm = dims
+    gA = gA.contiguous()
+    dev = gA.device
+    n_ap = Cn * M if need_ap else 0
+    gbuf = torch.empty(n_ap + nm * K, dtype=torch.float32, device=dev)
+    g_ap = gbuf[:n_ap].view(Cn, M) if need_ap else None
+    g_ms = gbuf[n_ap:].view(nm, K)
+    gWG = torch.empty(Cn, K, dtype=torch.complex64, device=dev) if need_ap else None
+    _lib.check(L.ams_pm_operand_bwd(_ptr(gA), _ptr(th_ap), _ptr(amp), _ptr(torch.view_as_real(G)), _ptr(WG),
+                                    _ptr(U), Cn, M, K, nm, _ptr(g_ms), _ptr(g_ap) if need_ap else None,
+                                    _ptr(gWG) if need_ap else None, _stream()),
+               "ams_pm_operand_bwd")
+    return g_ap, g_ms
+
+
+class PhaseStep(torch.autograd.Function):
+    """The whole forward of one metasurface with single-bounce chains (the
+    operand from the phases, the contraction, the objective) and its
+    backward as one Function.  The operand and gY are written straight into
+    the contraction's fragment layout (ams_pm_operand_frag,
+    ams_objective_bwd_frag), so neither A nor gY is formed as a matrix.
+    Returns the per-map loss and coverage (the latter without gradient)."""
+
+    @staticmethod
+    def forward(ctx, th_ap: torch.Tensor, th_ms: torch.Tensor, amp: torch.Tensor, ms_amp: torch.Tensor,
+                G: torch.Tensor, max_phase: float, op: ChannelOperand, mode: int, n0: float, w: float,
+                rowmask: Optional[torch.Tensor]):
+        L = _lib.lib()
+        Cn, M = th_ap.shape
+        nm, K = th_ms.shape
+        R, Kc, N = nm * Cn, op.Kc, op.N
+        dev = th_ap.device
+        th_ap, th_ms = th_ap.contiguous(), th_ms.contiguous()
+        fa = torch.empty(L.ams_frag_bytes(R, Kc), dtype=torch.uint8, device=dev)
+        WG = torch.empty(Cn, K, dtype=torch.complex64, device=dev)
+        U = torch.empty(nm, K, dtype=torch.complex64, device=dev)
+        _lib.check(L.ams_pm_operand_frag(_ptr(th_ap), _ptr(amp), _ptr(th_ms), _ptr(ms_amp),
+                                         _ptr(torch.view_as_real(G)), Cn, M, K, nm, max_phase, _ptr(fa), _ptr(WG),
+                                         _ptr(U), _stream()),
+                   "ams_pm_operand_frag")
+        Y = cgemm_frag(fa, op.frag_fwd, R, N, Kc, conj_b=False, splits=split_count(R, N, Kc))
+        ws = torch.empty(L.ams_objective_workspace_bytes(nm, N), dtype=torch.uint8, device=dev)
+        loss = torch.empty(nm, dtype=torch.float32, device=dev)
+        cov = torch.empty(nm, dtype=torch.float32, device=dev)
+        rm = rowmask.to(device=dev, dtype=torch.uint8).contiguous() if rowmask is not None else None
+        _lib.check(L.ams_objective_fwd_cov(_ptr(Y), nm, Cn, N, mode, n0, w, _ptr(rm) if rm is not None else None,
+                                           _ptr(loss), _ptr(cov), _ptr(ws), _stream()),
+                   "ams_objective_fwd_cov")
+        ctx.save_for_backward(th_ap, amp, G, WG, U, Y)
+        ctx.ws, ctx.rm, ctx.op = ws, rm, op
+        ctx.cfg = (Cn, M, K, nm, mode, n0, w)
+        ctx.mark_non_differentiable(cov)
+        ctx.set_materialize_grads(False)
+        return loss, cov
+
+    @staticmethod
+    def backward(ctx, gloss: torch.Tensor, gcov: torch.Tensor = None):
+        if gloss is None:
+            return (None,) * 11
+        L = _lib.lib()
+        th_ap, amp, G, WG, U, Y = ctx.saved_tensors
+        Cn, M, K, nm, mode, n0, w = ctx.cfg
+        op = ctx.op
+        R, Kc, N = nm * Cn, op.Kc, op.N
+        gl = gloss.to(torch.float32).contiguous()
+        fg = torch.empty(L.ams_frag_bytes(R, N), dtype=torch.uint8, device=Y.device)
+        _lib.check(L.ams_objective_bwd_frag(_ptr(Y), nm, Cn, N, mode, n0, w,
+                                            _ptr(ctx.rm) if ctx.rm is not None else None, _ptr(gl), _ptr(ctx.ws),
+                                            _ptr(fg), _stream()),
+                   "ams_objective_bwd_frag")
+        splits = split_count(R, Kc, N) if R < 128 else op.bwd_splits
+        gA = cgemm_frag(fg, op.frag_bwd, R, Kc, N, conj_b=True, splits=splits)
+        g_ap, g_ms = _operand_grads(gA, th_ap, amp, G, WG, U, (Cn, M, K, nm), ctx.needs_input_grad[0])
+        return (g_ap, g_ms) + (None,) * 9
 
 
 class ContractPlain(torch.autograd.Function):
@@ -308,6 +374,10 @@ class PhaseMapModel(nn.Module):
         # one metasurface: every chain has length 1, and the operand comes from
         # the fused kernels (PhaseOperand); otherwise from torch ops
         self.fused_operand = fused and not plain and len(ms_list) == 1
+        # the whole step as one Function over fragment-writing kernels (up to
+        # 256 AP elements); otherwise the operand, the contraction and the
+        # objective as separate Functions
+        self.fused_step = self.fused_operand and M <= 256
         # metasurface-to-metasurface channels; ms{b}->ms{a} for b > a is the
         # transpose (not the conjugate) of ms{a}->ms{b}, as go.py:147-153 uses it
         self.T = {}
@@ -363,14 +433,19 @@ class PhaseMapModel(nn.Module):
         return fn.apply(A, self.op)
 
     def forward(self) -> torch.Tensor:
-        Y = self.received()
         rowmask = None
         if self.mode == 1:
             # go.py:220: rows of cap zeroed where torch.rand(C) < 0.1 (CPU generator)
             rowmask = (torch.rand(self.code_num * self.n_maps) < 0.1) if self.rowmask_fn is None \
                 else self.rowmask_fn()
         # last_cov: mean over Rx of the best code's power (dB), per map
-        loss, self.last_cov = Objective.apply(Y, self.n_maps, self.mode, self.n0, self.w, rowmask)
+        if self.fused_step:
+            loss, self.last_cov = PhaseStep.apply(self.antenna_theta, self.ms_theta[0], self.antenna_A,
+                                                  self.ms_A[0], self.G_list[0], self.ms_list[0].max_phase_rad,
+                                                  self.op, self.mode, self.n0, self.w, rowmask)
+        else:
+            loss, self.last_cov = Objective.apply(self.received(), self.n_maps, self.mode, self.n0, self.w,
+                                                  rowmask)
         self.loss = loss.sum() if self.n_maps > 1 else loss.reshape(())
         return self.loss
 

@@ -162,6 +162,13 @@ int ams_objective_fwd_cov(const float *Y, int32_t G, int32_t C, int32_t N, int32
 int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
                       const unsigned char *rowmask, const float *gloss, const void *workspace, float *gY,
                       void *stream);
+/* The same gradient written directly in the fragment layout of
+ * ams_frag_pack for the backward contraction: rows the G*C codes, reduction
+ * index the N Rx (gY_frag: ams_frag_bytes(G*C, N) bytes); equal to
+ * ams_frag_pack of ams_objective_bwd's gY. */
+int ams_objective_bwd_frag(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
+                           const unsigned char *rowmask, const float *gloss, const void *workspace, void *gY_frag,
+                           void *stream);
 
 /* Phase-map operand of one metasurface with single-bounce chains, fused
  * (go.py:107-163 for one metasurface and max_ms_reflex = 1, replacing its
@@ -176,6 +183,13 @@ int ams_objective_bwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t m
 int ams_pm_operand(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp, const float *G,
                    int32_t Cn, int32_t M, int32_t K, int32_t nm, float max_phase, float *A, float *WG, float *U,
                    void *stream);
+/* The same operand written directly in the fragment layout of
+ * ams_frag_pack (frag: ams_frag_bytes(nm*Cn, K+M) bytes), for
+ * ams_cgemm_frag; equal to ams_frag_pack of ams_pm_operand's A.  WG and u as
+ * there.  M <= 256. */
+int ams_pm_operand_frag(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp,
+                        const float *G, int32_t Cn, int32_t M, int32_t K, int32_t nm, float max_phase, void *frag,
+                        float *WG, float *U, void *stream);
 /* Its backward: gA complex [nm*Cn][K+M] in torch's convention for complex
  * gradients; g_ms f32 [nm][K] = d loss / d th_ms; when g_ap is not null also
  * g_ap f32 [Cn][M] = d loss / d th_ap, using gWG (complex [Cn][K]) as

@@ -266,6 +266,82 @@ def test_fused_operand_matches_torch_ops(gpu, objective, n_maps, codebook):
         assert out[True][2] is None
 
 
+@pytest.mark.parametrize("Cn,M,K,nm", [(30, 16, 1024, 1), (30, 36, 203, 2), (64, 64, 96, 4)])
+def test_operand_fragments_equal_pack_of_operand(gpu, Cn, M, K, nm):
+    """ams_pm_operand_frag writes exactly ams_frag_pack of ams_pm_operand's
+    A, byte for byte, and the same WG and u (K = 203 puts a fragment lane
+    across the boundary of the metasurface and AP columns; rows and columns
+    past the matrix are zero in both)."""
+    from automs_amd import _lib
+    from automs_amd.phasemap import _ptr, _stream, ap_amplitude, frag_pack
+    L = _lib.lib()
+    _flush(f"case: operand fragments Cn={Cn} M={M} K={K} nm={nm}")
+    g = torch.Generator().manual_seed(K)
+    th_ap = (torch.rand(Cn, M, generator=g) * 7 - 1).to(gpu)
+    amp = ap_amplitude(Cn, M).to(gpu)
+    th_ms = ((torch.rand(nm, K, generator=g) - 0.3) * 20).to(gpu)
+    ms_amp = (torch.rand(K, generator=g) + 0.5).to(gpu)
+    G = _crand(M, K, scale=3e-4, seed=K).to(torch.complex64).to(gpu)
+    R, Kc = nm * Cn, K + M
+    A = torch.empty(R, Kc, dtype=torch.complex64, device=gpu)
+    WG1, WG2 = (torch.empty(Cn, K, dtype=torch.complex64, device=gpu) for _ in range(2))
+    U1, U2 = (torch.empty(nm, K, dtype=torch.complex64, device=gpu) for _ in range(2))
+    Gr = torch.view_as_real(G)
+    _lib.check(L.ams_pm_operand(_ptr(th_ap), _ptr(amp), _ptr(th_ms), _ptr(ms_amp), _ptr(Gr), Cn, M, K, nm, 5.0,
+                                _ptr(A), _ptr(WG1), _ptr(U1), _stream()), "ams_pm_operand")
+    ref = frag_pack(A, R, Kc, Kc, 1)
+    fa = torch.full((L.ams_frag_bytes(R, Kc),), 7, dtype=torch.uint8, device=gpu)
+    _lib.check(L.ams_pm_operand_frag(_ptr(th_ap), _ptr(amp), _ptr(th_ms), _ptr(ms_amp), _ptr(Gr), Cn, M, K, nm,
+                                     5.0, _ptr(fa), _ptr(WG2), _ptr(U2), _stream()), "ams_pm_operand_frag")
+    torch.cuda.synchronize()
+    _flush(f"  fragments equal: {torch.equal(fa, ref)}, WG equal: {torch.equal(WG1, WG2)}, "
+           f"u equal: {torch.equal(U1, U2)}")
+    assert torch.equal(fa, ref) and torch.equal(WG1, WG2) and torch.equal(U1, U2)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_objective_gradient_fragments_equal_pack(gpu, mode):
+    """ams_objective_bwd_frag writes exactly ams_frag_pack of
+    ams_objective_bwd's gY, byte for byte: exact ties (codes 9 and 17 copy
+    code 5), masked rows including some Rx's best code (mode 1), two groups
+    (mode 0), N = 1000 Rx (a partial fragment at the end)."""
+    from automs_amd import _lib
+    from automs_amd.phasemap import N0_90DBM, _ptr, _stream, frag_pack
+    L = _lib.lib()
+    groups = 2 if mode == 0 else 1
+    C, N = 30, 1000
+    _flush(f"case: objective gradient fragments mode={mode}")
+    Y = _crand(groups * C, N, scale=3e-5, seed=71)
+    for gi in range(groups):
+        Y[gi * C + 9] = Y[gi * C + 5]
+        Y[gi * C + 17] = Y[gi * C + 5]
+    Y = Y.to(torch.complex64).to(gpu)
+    rm = None
+    if mode == 1:
+        rm = torch.zeros(C, dtype=torch.uint8)
+        rm[[2, 5, 23]] = 1
+        rm = rm.to(gpu)
+    ws = torch.empty(L.ams_objective_workspace_bytes(groups, N), dtype=torch.uint8, device=gpu)
+    loss = torch.empty(groups, dtype=torch.float32, device=gpu)
+    Yr = torch.view_as_real(Y)
+    _lib.check(L.ams_objective_fwd(_ptr(Yr), groups, C, N, mode, N0_90DBM, 0.5, _ptr(rm) if rm is not None else None,
+                                   _ptr(loss), _ptr(ws), _stream()), "ams_objective_fwd")
+    gl = torch.tensor([1.0, -0.7][:groups], dtype=torch.float32, device=gpu)
+    gY = torch.empty_like(Y)
+    _lib.check(L.ams_objective_bwd(_ptr(Yr), groups, C, N, mode, N0_90DBM, 0.5,
+                                   _ptr(rm) if rm is not None else None, _ptr(gl), _ptr(ws),
+                                   _ptr(torch.view_as_real(gY)), _stream()), "ams_objective_bwd")
+    ref = frag_pack(gY, groups * C, N, N, 1)
+    fg = torch.full((L.ams_frag_bytes(groups * C, N),), 7, dtype=torch.uint8, device=gpu)
+    _lib.check(L.ams_objective_bwd_frag(_ptr(Yr), groups, C, N, mode, N0_90DBM, 0.5,
+                                        _ptr(rm) if rm is not None else None, _ptr(gl), _ptr(ws), _ptr(fg),
+                                        _stream()), "ams_objective_bwd_frag")
+    torch.cuda.synchronize()
+    nz = int((gY.abs() > 0).sum())
+    _flush(f"  fragments equal: {torch.equal(fg, ref)} (nonzero gY entries {nz})")
+    assert nz > 0 and torch.equal(fg, ref)
+
+
 def test_adam_device_count_equals_host_count(gpu):
     """ams_adam_step_dev (step count kept on the device; 7 workgroups, the
     last to finish writing the new state) against
