@@ -2118,9 +2118,25 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         tt[pos] = (int)g;
       }
     }
-    std::unordered_set<uint64_t> traced;
-    traced.reserve(tp.size() * 4 + 16);
-    for (size_t k = 0; k < tp.size(); k++) traced.insert(((uint64_t)tp[k] << 24) | (uint64_t)tt[k]);
+    // (path, tile) pairs already traced: a bitmap over paths x tiles while it
+    // stays within 128 MB (the room: 15 KB, the office: about 1 MB), else a
+    // hash set.  mark() returns true for a pair not traced before.
+    const uint64_t n_pairs = (uint64_t)n_paths * (uint64_t)n_grid;
+    const bool traced_bits_on = n_pairs <= (1ull << 30);
+    std::vector<uint64_t> traced_bits(traced_bits_on ? (size_t)((n_pairs + 63) / 64) : 0, 0);
+    std::unordered_set<uint64_t> traced_set;
+    auto mark = [&](int p, long long g) -> bool {
+      if (traced_bits_on) {
+        const uint64_t b = (uint64_t)p * (uint64_t)n_grid + (uint64_t)g;
+        uint64_t &w = traced_bits[b >> 6];
+        const uint64_t m = 1ull << (b & 63);
+        if (w & m) return false;
+        w |= m;
+        return true;
+      }
+      return traced_set.insert(((uint64_t)p << 24) | (uint64_t)g).second;
+    };
+    for (size_t k = 0; k < tp.size(); k++) mark(tp[k], tt[k]);
 
     EvalArgs ea;
     ea.tnodes = s->d_tnodes;
@@ -2254,7 +2270,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         long long nb[4] = {g - 1, g + 1, g - n1g, g + n1g};
         for (int q = 0; q < 4; q++) {
           if (nb[q] < 0 || nb[q] >= n_grid) continue;
-          if (traced.insert(((uint64_t)tp[k] << 24) | (uint64_t)nb[q]).second) {
+          if (mark(tp[k], nb[q])) {
             np_.push_back(tp[k]);
             nt_.push_back((int)nb[q]);
           }
