@@ -122,7 +122,11 @@ int ams_scene_plane_ids(const ams_scene *scene, int32_t *out);
 
 /* Replaces torch_launch_pathtrace (optixPathTracer.cpp:781-861): traces
  * Tx -> Rx and ACCUMULATES H = sum A e^{-j phi} into H_out, a DEVICE f32
- * buffer [n_tx][n_rx][2] (go.py:671-677, optixPathTracer.cu:259-264). */
+ * buffer [n_tx][n_rx][2] (go.py:671-677, optixPathTracer.cu:259-264).
+ * A scene handle serves one trace at a time (its device scratch and
+ * page-locked staging buffers belong to the handle); traces that run
+ * concurrently, on their own streams and host threads, use one handle
+ * each.  Different handles share no state. */
 int ams_trace_channel(ams_scene *scene, const ams_array *tx, const ams_array *rx,
                       const ams_trace_opts *opts, float *H_out, ams_trace_stats *stats);
 
