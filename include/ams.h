@@ -58,7 +58,9 @@ typedef struct {
                              bits 10, 11: forward queries walk the binary threaded hierarchy
                                      (bit 10) or the four-wide one (bit 11); by default the
                                      binary one for scenes staged in LDS and the four-wide one
-                                     otherwise (same hits either way);
+                                     otherwise (same hits either way).  Bit 11 also makes the
+                                     backward queries of scenes in global memory walk the
+                                     four-wide hierarchy (by default they walk the binary one);
                              bit 7: keep tracing a branch with no reflection left even when it
                                     can no longer reach the Rx array */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */

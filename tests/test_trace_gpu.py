@@ -264,13 +264,14 @@ def test_culling_leaves_results_unchanged(small_room, trace_scene, flags):
     assert sa["fwd_queries"] + sa["bwd_queries"] <= sb["fwd_queries"] + sb["bwd_queries"]
 
 
-@pytest.mark.parametrize("base", [2048, 16])
+@pytest.mark.parametrize("base", [2048, 16, 16 | 2048])
 def test_four_wide_walk_matches_binary(small_room, trace_scene, base):
     """The four-wide hierarchy walk returns the binary walk's hits (flags bit
     10 forces the binary walk), with the scene staged in LDS (bit 11 forces
     the four-wide walk there) and read from global memory (bit 4, where it is
-    the default): identical record sets and successes, the same H, and fewer
-    node boxes tested."""
+    the default for forward queries; with bit 11 the backward queries walk it
+    too): identical record sets and successes, the same H, and fewer node
+    boxes tested forward."""
     import torch
 
     from automs_amd.tracer import trace
