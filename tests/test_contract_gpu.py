@@ -342,6 +342,37 @@ def test_objective_gradient_fragments_equal_pack(gpu, mode):
     assert nz > 0 and torch.equal(fg, ref)
 
 
+@pytest.mark.parametrize("objective,n_maps,codebook", [("logsnr", 1, True), ("capacity", 2, True),
+                                                       ("logsnr", 3, False)])
+def test_phase_step_equals_separate_functions(gpu, objective, n_maps, codebook):
+    """PhaseStep (the whole step over the fragment-writing kernels) against
+    the same model run as PhaseOperand + Contract + Objective: the GEMMs see
+    byte-identical fragments, so the loss, the coverage and the phase
+    gradients are identical bit for bit."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    _flush(f"case: PhaseStep vs separate Functions objective={objective} n_maps={n_maps} codebook={codebook}")
+    Hd, G, F = _channels(M=36, K=200, N=900, seed=81)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (10, 20), None, None, [], max_phase=300.0)]
+    th_ms = (torch.rand(n_maps, 200, generator=torch.Generator().manual_seed(2)) - 0.3) * 6 * math.pi
+    mask = torch.rand(30 * n_maps, generator=torch.Generator().manual_seed(6)) < 0.1
+    out = {}
+    for fused_step in (True, False):
+        model = PhaseMapModel("t", ch, ms, code_num=30, w=0.5, objective=objective, ms_theta=[th_ms],
+                              n_maps=n_maps, optimize_codebook=codebook)
+        assert model.fused_step
+        model.fused_step = fused_step
+        model.rowmask_fn = lambda: mask
+        loss = model()
+        loss.backward()
+        torch.cuda.synchronize()
+        out[fused_step] = [loss.detach().cpu(), model.last_cov.cpu(), model.ms_theta[0].grad.cpu()] + \
+            ([model.antenna_theta.grad.cpu()] if codebook else [])
+    for a, b in zip(out[True], out[False]):
+        assert torch.equal(a, b), (a, b)
+
+
 def test_adam_device_count_equals_host_count(gpu):
     """ams_adam_step_dev (step count kept on the device; 7 workgroups, the
     last to finish writing the new state) against
