@@ -5,7 +5,7 @@ simulated-annealing tuner, every trial retracing and re-optimising from
 scratch (physical_exp_nni_main.py:12-29, physical_exp_nni_model.py:59-70).
 Here an in-process annealing loop searches placements, a small conv net
 predicts each candidate's per-Rx gain map, and the ray tracer re-scores every
-accepted candidate (the tracer stays the judge and the fallback).
+accepted candidate (the tracer's score is the one that decides).
 
 Network: 4 conv3x3 layers, channels 9 -> 32 -> 32 -> 32 -> 1, ReLU between,
 bf16 weights with FP32 master copies, all forward/backward math in
