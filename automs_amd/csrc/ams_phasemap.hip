@@ -31,7 +31,7 @@ namespace {
 
 constexpr int kPmBlock = 256;
 constexpr int kPmMaxM = 4096;      // AP elements (LDS row of W: 32 KB)
-constexpr int kPmFragMaxM = 256;   // AP elements of the fragment writer (16 rows of W: 32 KB)
+constexpr int kPmFragMaxM = 128;   // AP elements of the fragment writer (W rows and G columns: 48 KB)
 constexpr int kPmApBlock = 256;
 constexpr int kPmMsK = 64;  // metasurface elements per workgroup of k_pm_bwd_ms
 constexpr int kPmMsC = 4;   // code groups (waves) per workgroup of k_pm_bwd_ms
@@ -110,8 +110,9 @@ __global__ void __launch_bounds__(kPmBlock) k_pm_operand(const float *__restrict
 // per 16 x 32 fragment block: W of its 16 rows in LDS, two elements per lane
 // (the same products in the same order as k_pm_operand, so the fragments
 // equal ams_frag_pack of that kernel's A), the block's values gathered in
-// LDS and packed by one wave.  Lanes of map 0 also write WG, lanes of code 0
-// write u.  Dynamic LDS: 16 * M float2.
+// LDS and packed by one wave.  The block's 32 columns of G are staged in LDS
+// first, so the M-term products read no global memory.  Lanes of map 0 also
+// write WG, lanes of code 0 write u.  Dynamic LDS: 48 * M float2.
 __global__ void __launch_bounds__(kPmBlock) k_pm_operand_frag(const float *__restrict__ th_ap,
                                                               const float *__restrict__ amp,
                                                               const float *__restrict__ th_ms,
@@ -120,10 +121,16 @@ __global__ void __launch_bounds__(kPmBlock) k_pm_operand_frag(const float *__res
                                                               int nm, float max_phase, int n_kb,
                                                               u32x4 *__restrict__ frag, float2 *__restrict__ WG,
                                                               float2 *__restrict__ U) {
-  extern __shared__ float2 w_rows[];  // [16][M]
+  extern __shared__ float2 pm_lds[];
+  float2 *w_rows = pm_lds;            // [16][M]
+  float2 *g_cols = pm_lds + 16 * M;  // [M][32]
   __shared__ float2 tile[16][32];
   const int R = nm * Cn, Kc = K + M;
   const int kb = blockIdx.x % n_kb, rb = blockIdx.x / n_kb;
+  for (int idx = threadIdx.x; idx < 32 * M; idx += blockDim.x) {
+    const int j = idx >> 5, k = kb * 32 + (idx & 31);
+    g_cols[idx] = k < K ? G[(size_t)j * K + k] : make_float2(0.0f, 0.0f);
+  }
   for (int idx = threadIdx.x; idx < 16 * M; idx += blockDim.x) {
     const int rl = idx / M, j = idx % M;
     const int r = rb * 16 + rl;
@@ -146,7 +153,7 @@ __global__ void __launch_bounds__(kPmBlock) k_pm_operand_frag(const float *__res
       const float2 *wr = w_rows + rl * M;
       if (k < K) {
         float2 acc = make_float2(0.0f, 0.0f);
-        for (int j = 0; j < M; j++) acc = cmac(acc, wr[j], G[(size_t)j * K + k]);
+        for (int j = 0; j < M; j++) acc = cmac(acc, wr[j], g_cols[j * 32 + kl]);
         const float2 u = ms_coef(ms_amp[k], th_ms[(size_t)m * K + k], max_phase);
         v = cmul(acc, u);
         if (m == 0) WG[(size_t)c * K + k] = acc;
@@ -289,13 +296,13 @@ int ams_pm_operand_frag(const float *th_ap, const float *amp, const float *th_ms
                         float *WG, float *U, void *stream) {
   if (!th_ap || !amp || !th_ms || !ms_amp || !G || !frag || !WG || !U || !pm_dims_ok(Cn, M, K, nm) ||
       M > kPmFragMaxM || !(max_phase > 0.0f)) {
-    ams::set_error("ams_pm_operand_frag: bad arguments (M <= 256)");
+    ams::set_error("ams_pm_operand_frag: bad arguments (M <= 128)");
     return AMS_ERR_ARG;
   }
   const long long R = (long long)nm * Cn, Kc = (long long)K + M;
   const int n_kb = (int)((Kc + 31) / 32);
   const long long n_blocks = ((R + 15) / 16) * n_kb;
-  k_pm_operand_frag<<<(unsigned)n_blocks, kPmBlock, (size_t)16 * M * sizeof(float2), (hipStream_t)stream>>>(
+  k_pm_operand_frag<<<(unsigned)n_blocks, kPmBlock, (size_t)48 * M * sizeof(float2), (hipStream_t)stream>>>(
       th_ap, amp, th_ms, ms_amp, (const float2 *)G, Cn, M, K, nm, max_phase, n_kb, (u32x4 *)frag, (float2 *)WG,
       (float2 *)U);
   hipError_t e = hipGetLastError();

@@ -375,9 +375,9 @@ class PhaseMapModel(nn.Module):
         # the fused kernels (PhaseOperand); otherwise from torch ops
         self.fused_operand = fused and not plain and len(ms_list) == 1
         # the whole step as one Function over fragment-writing kernels (up to
-        # 256 AP elements); otherwise the operand, the contraction and the
+        # 128 AP elements); otherwise the operand, the contraction and the
         # objective as separate Functions
-        self.fused_step = self.fused_operand and M <= 256
+        self.fused_step = self.fused_operand and M <= 128
         # metasurface-to-metasurface channels; ms{b}->ms{a} for b > a is the
         # transpose (not the conjugate) of ms{a}->ms{b}, as go.py:147-153 uses it
         self.T = {}

@@ -192,7 +192,7 @@ int ams_pm_operand(const float *th_ap, const float *amp, const float *th_ms, con
 /* The same operand written directly in the fragment layout of
  * ams_frag_pack (frag: ams_frag_bytes(nm*Cn, K+M) bytes), for
  * ams_cgemm_frag; equal to ams_frag_pack of ams_pm_operand's A.  WG and u as
- * there.  M <= 256. */
+ * there.  M <= 128. */
 int ams_pm_operand_frag(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp,
                         const float *G, int32_t Cn, int32_t M, int32_t K, int32_t nm, float max_phase, void *frag,
                         float *WG, float *U, void *stream);
