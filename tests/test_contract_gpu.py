@@ -486,6 +486,37 @@ def test_graphed_optimiser_matches_eager(gpu, tmp_path):
         optimize_model(cap, step=2, graph=True)
 
 
+@pytest.mark.parametrize("steps", [0, 1, 2, 3])
+def test_graphed_optimiser_short_runs(gpu, steps):
+    """optimize_model(graph=True) for 0 to 3 steps (no capture, the eager
+    first step only, one replay, two replays) equals the eager loop: loss
+    history and final parameters within 1e-6; with 0 steps the parameters
+    are untouched."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel, optimize_model
+    _flush(f"case: graphed optimiser, {steps} steps")
+    Hd, G, F = _channels(M=16, K=128, N=512, seed=91)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (8, 16), None, None, [], max_phase=360.0)]
+    th_ms = torch.rand(1, 128, generator=torch.Generator().manual_seed(3)) * 6.0
+    res = {}
+    for graph in (False, True):
+        model = PhaseMapModel("t", ch, ms, code_num=8, ms_theta=[th_ms])
+        p0 = [p.detach().cpu().clone() for p in model.parameters()]
+        optimize_model(model, lr=0.3, step=steps, graph=graph)
+        torch.cuda.synchronize()
+        ps = [p.detach().cpu() for p in model.parameters()]
+        if steps == 0:
+            assert all(torch.equal(a, b) for a, b in zip(p0, ps)) and model.loss_list == []
+        res[graph] = (torch.tensor(model.loss_list, dtype=torch.float64), ps)
+    (la, pa), (lb, pb) = res[False], res[True]
+    assert la.shape == lb.shape == (steps,)
+    if steps:
+        assert ((la - lb).abs() <= 1e-6 * la.abs()).all()
+    for a, b in zip(pa, pb):
+        assert ((a.double() - b.double()).norm() <= 1e-6 * a.double().norm()).item()
+
+
 def test_graphed_optimiser_two_metasurfaces(gpu, tmp_path):
     """optimize_model(graph=True) on a model with two metasurfaces, whose
     operand comes from torch ops (a library complex GEMM among them): 8

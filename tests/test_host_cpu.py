@@ -434,3 +434,17 @@ def test_contraction_oracle_fp32_mode_and_start_helpers():
     a2, b2 = start_phases(K, M, Cc, seed=4)
     assert torch.equal(a1, a2) and torch.equal(b1, b2) and a1.shape == (1, K) and b1.shape == (Cc, M)
     assert float(a1.min()) >= 0 and float(a1.max()) < 2 * math.pi
+
+
+def test_fused_adam_rejects_a_wrong_best_buffer():
+    """FusedAdam.step checks the best-parameter buffer (size, dtype,
+    layout) before it launches anything."""
+    import torch
+    from automs_amd.phasemap import FusedAdam
+    a = torch.nn.Parameter(torch.zeros(5))
+    opt = FusedAdam([a])
+    a.grad = torch.ones(5)
+    loss = torch.tensor(1.0)
+    for bad in (torch.zeros(4), torch.zeros(5, dtype=torch.float64), torch.zeros(10)[::2]):
+        with pytest.raises(ValueError):
+            opt.step(loss=loss, best=bad)
