@@ -302,6 +302,10 @@ int ams_pm_operand_frag(const float *th_ap, const float *amp, const float *th_ms
   const long long R = (long long)nm * Cn, Kc = (long long)K + M;
   const int n_kb = (int)((Kc + 31) / 32);
   const long long n_blocks = ((R + 15) / 16) * n_kb;
+  if (n_blocks > 0x7fffffffLL) {
+    ams::set_error("ams_pm_operand_frag: operand too large for one launch");
+    return AMS_ERR_ARG;
+  }
   k_pm_operand_frag<<<(unsigned)n_blocks, kPmBlock, (size_t)48 * M * sizeof(float2), (hipStream_t)stream>>>(
       th_ap, amp, th_ms, ms_amp, (const float2 *)G, Cn, M, K, nm, max_phase, n_kb, (u32x4 *)frag, (float2 *)WG,
       (float2 *)U);
@@ -324,8 +328,8 @@ int ams_pm_operand_bwd(const float *gA, const float *th_ap, const float *amp, co
   k_pm_bwd_ms<<<(unsigned)((K + kPmMsK - 1) / kPmMsK), kPmMsK * kPmMsC, 0, st>>>(
       (const float2 *)gA, (const float2 *)WG, (const float2 *)U, Cn, M, K, nm, g_ms, g_ap ? (float2 *)gWG : nullptr);
   if (g_ap)
-    k_pm_bwd_ap<<<dim3((unsigned)M, (unsigned)Cn), kPmApBlock, 0, st>>>((const float2 *)gA, (const float2 *)gWG, (const float2 *)G,
-                                                    th_ap, amp, Cn, M, K, nm, g_ap);
+    k_pm_bwd_ap<<<dim3((unsigned)M, (unsigned)Cn), kPmApBlock, 0, st>>>(
+        (const float2 *)gA, (const float2 *)gWG, (const float2 *)G, th_ap, amp, Cn, M, K, nm, g_ap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ams::set_error(std::string("ams_pm_operand_bwd: ") + hipGetErrorString(e));
