@@ -33,8 +33,8 @@ constexpr int kPmBlock = 256;
 constexpr int kPmMaxM = 4096;      // AP elements (LDS row of W: 32 KB)
 constexpr int kPmFragMaxM = 128;   // AP elements of the fragment writer (W rows and G columns: 48 KB)
 constexpr int kPmApBlock = 256;
-constexpr int kPmMsK = 64;  // metasurface elements per workgroup of k_pm_bwd_ms
-constexpr int kPmMsC = 4;   // code groups (waves) per workgroup of k_pm_bwd_ms
+constexpr int kPmMsK = 32;  // metasurface elements per workgroup of k_pm_bwd_ms
+constexpr int kPmMsC = 8;   // code groups per workgroup of k_pm_bwd_ms
 
 // a * b and acc + a * b with every rounding fixed (explicit fma): the
 // operand kernels below form A the same way, value for value.
@@ -173,8 +173,8 @@ __global__ void __launch_bounds__(kPmBlock) k_pm_operand_frag(const float *__res
   ams::frag_store8(frag, it, v8);
 }
 
-// Workgroup = 64 metasurface elements k (one per lane) x 4 waves, wave w
-// summing codes c = w, w+4, ...; the four partial sums are added in wave
+// Workgroup = kPmMsK metasurface elements k x kPmMsC code groups, group q
+// summing codes c = q, q + kPmMsC, ...; the group sums are added in group
 // order.  g_ms for every map, then gWG when the AP phases need a gradient.
 __global__ void __launch_bounds__(kPmMsK *kPmMsC) k_pm_bwd_ms(const float2 *__restrict__ gA,
                                                               const float2 *__restrict__ WG,
