@@ -248,16 +248,26 @@ __global__ void __launch_bounds__(256) k_obj_bwd_frag(const float2 *__restrict__
     const float gl = gloss[g];
     const float fro = mode != 0 ? (float)sqrt(sums[2 * g + 1]) : 0.0f;
     const bool masked = mode != 0 && rowmask && rowmask[c];
+    // the lane's 8 values of y and their argmax codes are loaded together (one
+    // round of memory latency), then combined
+    float2 yc[8];
+    int bis[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int rx = it.k0 + e;
+      yc[e] = rx < N ? Yg[(size_t)c * N + rx] : make_float2(0.0f, 0.0f);
+      bis[e] = rx < N ? argmax[(size_t)g * N + rx] : -1;
+    }
 #pragma unroll
     for (int e = 0; e < 8; e++) {
       const int rx = it.k0 + e;
       if (rx >= N) continue;
-      const int bi = argmax[(size_t)g * N + rx];
+      const int bi = bis[e];
       if (mode == 0) {
-        if (c == bi) v[e] = obj_grad(Yg[(size_t)c * N + rx], c, bi, 0.0f, 0.0f, false, 0, n0, w, gl, N);
+        if (c == bi) v[e] = obj_grad(yc[e], c, bi, 0.0f, 0.0f, false, 0, n0, w, gl, N);
       } else {
         const float best = rowmask && rowmask[bi] ? 0.0f : cap_of(Yg[(size_t)bi * N + rx], n0);
-        v[e] = obj_grad(Yg[(size_t)c * N + rx], c, bi, best, fro, masked, 1, n0, w, gl, N);
+        v[e] = obj_grad(yc[e], c, bi, best, fro, masked, 1, n0, w, gl, N);
       }
     }
   }
