@@ -256,8 +256,10 @@ def contraction_office(iters: int = 20) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # 10 timed steps (about 0.35 s of tracing): the three traces of a step run
+    # concurrently, and their interleaving varies from step to step
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="room_boxes_ms", choices=["room_boxes", "room_boxes_ms", "office"])
     ap.add_argument("--opt-steps", type=int, default=500)
     ap.add_argument("--no-opt", action="store_true")
