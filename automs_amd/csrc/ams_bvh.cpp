@@ -184,7 +184,7 @@ void ams_bvh_build(int n, const double *lo, const double *hi, int leaf_max, AmsB
   out->order = std::move(B.idx);
 }
 
-void ams_plane_ids(int n_tri, const float *tri, int *plane_out, double *pl_n_out, double *pl_d_out) {
+void ams_plane_ids(int n_tri, const float *tri, int *plane_out, double *pl_n_out, double *pl_d_out, double *own_out) {
   // Quantised (normal, offset) -> smallest triangle index.  Same definition
   // as the one the path keys are compared under: unit normal from
   // cross(v1-v0, v2-v0) in FP64, sign made positive on its first component
@@ -216,6 +216,12 @@ void ams_plane_ids(int n_tri, const float *tri, int *plane_out, double *pl_n_out
       }
     }
     double d = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];
+    if (own_out) {
+      own_out[4 * (size_t)i] = n[0];
+      own_out[4 * (size_t)i + 1] = n[1];
+      own_out[4 * (size_t)i + 2] = n[2];
+      own_out[4 * (size_t)i + 3] = d;
+    }
     auto key = std::make_tuple(std::llround(n[0] * 1e6), std::llround(n[1] * 1e6),
                                std::llround(n[2] * 1e6), std::llround(d * 1e5));
     auto it = first.find(key);

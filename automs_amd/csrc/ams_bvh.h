@@ -20,4 +20,7 @@ int ams_bvh_collapse4(const AmsBvhBuild &b, std::vector<AmsNode4> *out);
 
 // Plane ids: for each triangle, the smallest triangle index lying on the same
 // plane (unit normal and offset equal after canonical sign, to 1e-6 / 1e-5 m).
-void ams_plane_ids(int n_tri, const float *tri, int *plane_out, double *pl_n_out, double *pl_d_out);
+// own_out (may be null): each triangle's own plane (canonical unit normal,
+// offset), 4 doubles per triangle, before grouping
+void ams_plane_ids(int n_tri, const float *tri, int *plane_out, double *pl_n_out, double *pl_d_out,
+                   double *own_out = nullptr);

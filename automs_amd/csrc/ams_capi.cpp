@@ -102,6 +102,7 @@ int ams_scene_destroy(ams_scene *s) {
   ams::dfree(s->d_planes);
   ams::dfree(s->d_planes_f);
   ams::dfree(s->d_pbox);
+  ams::dfree(s->d_tplane);
   ams::dfree(s->d_tnodes4);
   ams::dfree(s->d_mat_prop);
   ams::dfree(s->d_refl_phase);

@@ -103,6 +103,8 @@ struct ams_scene {
   AmsPlane *d_planes = nullptr;
   float4 *d_planes_f = nullptr;  // the same planes rounded to f32: (n, d)
   AmsPlaneBox *d_pbox = nullptr;
+  AmsPlane *d_tplane = nullptr;  // each triangle's own plane (FP64), by original index
+  int n_inexact = 0;             // triangles whose own plane is not bitwise their plane id's
   float *d_mat_prop = nullptr;
   float *d_refl_phase = nullptr;
   ams::Scratch scratch;

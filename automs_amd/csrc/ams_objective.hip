@@ -82,6 +82,7 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_fwd(const float2 *__
   const int r = blockIdx.x * kObjRx + rl;
   const bool ok = r < N;
   const float2 *Yg = Y + (size_t)g * C * N;
+  const unsigned char *rm = rowmask ? rowmask + (size_t)g * C : nullptr;  // the group's row mask
   double s1 = 0.0, s2 = 0.0, s3 = 0.0;
   // pass 1: each group's first maximum (power in mode 0, capacity in mode 1)
   float best = -1.0f, pmax = 0.0f;
@@ -91,7 +92,7 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_fwd(const float2 *__
       float2 y = Yg[(size_t)c * N + r];
       float p = y.x * y.x + y.y * y.y;
       pmax = fmaxf(pmax, p);
-      float v = mode == 0 ? p : (rowmask && rowmask[c] ? 0.0f : cap_of(y, n0));
+      float v = mode == 0 ? p : (rm && rm[c] ? 0.0f : cap_of(y, n0));
       if (v > best) {
         best = v;
         bi = c;
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_fwd(const float2 *__
   // pass 2 (mode 1): squares of the codes whose capacity is not the maximum
   if (mode != 0 && ok)
     for (int c = q; c < C; c += kObjGroups) {
-      float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
+      float v = rm && rm[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
       if (v != best) s2 += (double)v * v;
     }
   double t1 = block_sum(s1, sh);
@@ -199,6 +200,7 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_bwd(const float2 *__
   const bool ok = r < N;
   const float2 *Yg = Y + (size_t)g * C * N;
   float2 *Gg = gY + (size_t)g * C * N;
+  const unsigned char *rm = rowmask ? rowmask + (size_t)g * C : nullptr;
   const float gl = gloss[g];
   const int bi = ok ? argmax[(size_t)g * N + r] : 0;
   if (mode == 0) {
@@ -212,7 +214,7 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_bwd(const float2 *__
   float best = -1.0f;
   if (ok)
     for (int c = q; c < C; c += kObjGroups) {
-      float v = rowmask && rowmask[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
+      float v = rm && rm[c] ? 0.0f : cap_of(Yg[(size_t)c * N + r], n0);
       if (v > best) best = v;
     }
   val[q][rl] = best;
@@ -222,7 +224,7 @@ __global__ void __launch_bounds__(kObjRx *kObjGroups) k_obj_bwd(const float2 *__
   const float fro = (float)sqrt(sums[2 * g + 1]);
   for (int c = q; c < C; c += kObjGroups)
     Gg[(size_t)c * N + r] =
-        obj_grad(Yg[(size_t)c * N + r], c, bi, best, fro, rowmask && rowmask[c], 1, n0, w, gl, N);
+        obj_grad(Yg[(size_t)c * N + r], c, bi, best, fro, rm && rm[c], 1, n0, w, gl, N);
 }
 
 // The same gradient written straight into the split-bf16 fragment layout of
@@ -247,7 +249,8 @@ __global__ void __launch_bounds__(256) k_obj_bwd_frag(const float2 *__restrict__
     const float2 *Yg = Y + (size_t)g * C * N;
     const float gl = gloss[g];
     const float fro = mode != 0 ? (float)sqrt(sums[2 * g + 1]) : 0.0f;
-    const bool masked = mode != 0 && rowmask && rowmask[c];
+    const unsigned char *rm = rowmask ? rowmask + (size_t)g * C : nullptr;
+    const bool masked = mode != 0 && rm && rm[c];
     // the lane's 8 values of y and their argmax codes are loaded together (one
     // round of memory latency), then combined
     float2 yc[8];
@@ -266,7 +269,7 @@ __global__ void __launch_bounds__(256) k_obj_bwd_frag(const float2 *__restrict__
       if (mode == 0) {
         if (c == bi) v[e] = obj_grad(yc[e], c, bi, 0.0f, 0.0f, false, 0, n0, w, gl, N);
       } else {
-        const float best = rowmask && rowmask[bi] ? 0.0f : cap_of(Yg[(size_t)bi * N + rx], n0);
+        const float best = rm && rm[bi] ? 0.0f : cap_of(Yg[(size_t)bi * N + rx], n0);
         v[e] = obj_grad(yc[e], c, bi, best, fro, masked, 1, n0, w, gl, N);
       }
     }

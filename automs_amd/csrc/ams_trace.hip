@@ -1144,6 +1144,7 @@ struct EvalArgs {
   const AmsPlane *planes;
   const float4 *planes_f;  // the planes in f32, (n, d)
   const AmsPlaneBox *pbox;
+  const AmsPlane *tplane;  // each triangle's own plane, by original index
   const float *src;
   int n_src;
   const float *dst;
@@ -1167,12 +1168,54 @@ struct EvalArgs {
   int gs2;                       // Rx per tile
 };
 
+// Path length of a successful chain along its reflection triangles' own
+// planes, in FP64: the length the chain traced (optixPathTracer.cu:344 sums
+// the traced segments), for a chain whose reflection triangles are only
+// nearly on the planes of their plane ids.  The image of the Tx element
+// across the key's planes gives the first leg's direction, as in the walk;
+// rt holds the chain's reflection triangles (original indices) in the order
+// the walk meets them from the Rx, stride kEvalBlock.
+__device__ __forceinline__ double own_plane_length(const EvalArgs &a, uint64_t khi, uint64_t klo, int nref, float3 s3,
+                                                float3 r3, const int *rt) {
+  double Ix = s3.x, Iy = s3.y, Iz = s3.z;
+  for (int k = 0; k < nref; k++) {
+    const AmsPlane P = a.planes[key_get_plane(khi, klo, k)];
+    double f = 2.0 * (Ix * P.n[0] + Iy * P.n[1] + Iz * P.n[2] - P.d);
+    Ix -= f * P.n[0];
+    Iy -= f * P.n[1];
+    Iz -= f * P.n[2];
+  }
+  double ox = r3.x, oy = r3.y, oz = r3.z;
+  double dx = Ix - ox, dy = Iy - oy, dz = Iz - oz;
+  const double il = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  dx *= il;
+  dy *= il;
+  dz *= il;
+  double L = 0.0;
+  for (int k = 0; k < nref; k++) {
+    const AmsPlane P = a.tplane[rt[k * kEvalBlock]];
+    const double nd = P.n[0] * dx + P.n[1] * dy + P.n[2] * dz;
+    const double t = (P.d - (P.n[0] * ox + P.n[1] * oy + P.n[2] * oz)) / nd;
+    ox += dx * t;
+    oy += dy * t;
+    oz += dz * t;
+    L += t;
+    dx -= 2.0 * nd * P.n[0];
+    dy -= 2.0 * nd * P.n[1];
+    dz -= 2.0 * nd * P.n[2];
+  }
+  // the closest approach to the Tx element on the last leg
+  return L + ((double)s3.x - ox) * dx + ((double)s3.y - oy) * dy + ((double)s3.z - oz) * dz;
+}
+
 // One lane per (Rx in the tile, Tx element), looping over the tasks of its
 // work unit.  WIDE: the scene query walks the four-wide hierarchy (global
 // memory scenes: fewer dependent L2 loads per query).
 template <bool LDS, bool EXPORT, bool WIDE>
 __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
   extern __shared__ float4 ams_smem[];
+  // reflection triangles of each lane's current chain
+  __shared__ int refl_tri[AMS_MAX_HIT * kEvalBlock];
   const AmsNode *tnodes = a.tnodes;
   const AmsTri *tris = a.tris;
   if (LDS) {
@@ -1229,6 +1272,7 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
       // task export (optixPathTracer.cu:243-250): cosines as the reference forms them
       if (EXPORT) x_recv = dot3(d, a.dst_dir) / (sqrtf(dot3(d, d)) * sqrtf(dot3(a.dst_dir, a.dst_dir)));
       int ttl = h + 1, ri = 0, last_pl = -1;
+      bool inexact = false;  // a reflection on a triangle off its plane id's plane
       nchain++;
       // Unfold the path analytically before any query: from the Rx towards
       // the image, every reflection point must lie ahead and inside the
@@ -1296,21 +1340,14 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
         if (slot < 0) tt = INFINITY;
         if (sh && bs < tt && bs <= tlim) {
           if (ttl == 1) {
-            // optixPathTracer.cu:398-406
+            // optixPathTracer.cu:398-406; the phase is added after the walk
             D += bs;
             A *= beam_pattern(d * -1.0f, a.src_dir, a.src_bptype) * a.wave_len /
                  (4.0f * (float)M_PI * (Dres + D));
-            double cyc = L * a.inv_wave_len_d;
-            float ang = (float)(2.0 * M_PI * (cyc - floor(cyc))) - psi;
             if (EXPORT) {  // optixPathTracer.cu:407-412
               x_loss *= a.wave_len / (4.0f * (float)M_PI * (Dres + D));
               x_launch = -dot3(d, a.src_dir) / (sqrtf(dot3(d, d)) * sqrtf(dot3(a.src_dir, a.src_dir)));
-              x_phase = (float)(2.0 * M_PI * cyc) - psi;
             }
-            float sn, cs;
-            sincosf(ang, &sn, &cs);
-            acc_re += (double)(A * cs);
-            acc_im += (double)(-A * sn);
             ok = true;
           }
           break;
@@ -1334,6 +1371,8 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
         bool refl = (bm >> (k - 1)) & 1;
         if (refl) {
           if (ri >= nref || (uint32_t)pl != key_get_plane(khi, klo, nref - 1 - ri)) break;
+          refl_tri[ri * kEvalBlock + threadIdx.x] = torig;  // in walk order (Rx side first)
+          inexact |= nm.w < 0.0f;
           ri++;
         }
         // optixPathTracer.cu:369-391
@@ -1352,6 +1391,17 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
           d = norm3(d - N * (2.0f * dot3(d, N)));
           psi += a.refl_phase[m];
         }
+      }
+      if (ok) {
+        // the path length in FP64, range-reduced in cycles before the FP32 sincos
+        if (inexact) L = own_plane_length(a, khi, klo, nref, s3, r3, refl_tri + threadIdx.x);
+        double cyc = L * a.inv_wave_len_d;
+        float ang = (float)(2.0 * M_PI * (cyc - floor(cyc))) - psi;
+        if (EXPORT) x_phase = (float)(2.0 * M_PI * cyc) - psi;
+        float sn, cs;
+        sincosf(ang, &sn, &cs);
+        acc_re += (double)(A * cs);
+        acc_im += (double)(-A * sn);
       }
     }
     if (EXPORT && active) {
@@ -1482,7 +1532,8 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
   s->n_mat = n_mat;
   std::vector<int> plane(n_tri > 0 ? n_tri : 1);
   std::vector<double> pl_n(3 * (size_t)(n_tri > 0 ? n_tri : 1)), pl_d(n_tri > 0 ? n_tri : 1);
-  if (n_tri > 0) ams_plane_ids(n_tri, tri, plane.data(), pl_n.data(), pl_d.data());
+  std::vector<double> own(4 * (size_t)(n_tri > 0 ? n_tri : 1));
+  if (n_tri > 0) ams_plane_ids(n_tri, tri, plane.data(), pl_n.data(), pl_d.data(), own.data());
   s->plane.assign(plane.begin(), plane.begin() + n_tri);
   std::vector<double> lo(3 * (size_t)n_tri), hi(3 * (size_t)n_tri);
   std::vector<AmsTriAux> aux(n_tri > 0 ? n_tri : 1);
@@ -1493,26 +1544,30 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
       lo[3 * i + k] = std::min((double)t[k], std::min((double)t[3 + k], (double)t[6 + k]));
       hi[3 * i + k] = std::max((double)t[k], std::max((double)t[3 + k], (double)t[6 + k]));
     }
-    // FP32 normal and longest edge as the kernels of the reference compute them
-    float e1[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
-    float e2[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
-    float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-    float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    if (l > 0) {
-      c[0] /= l;
-      c[1] /= l;
-      c[2] /= l;
-    } else {
-      c[0] = 0;
-      c[1] = 0;
-      c[2] = 1;
-    }
+    // Unit normal normalize(cross(v1 - v0, v2 - v0)) (optixPathTracer.cu:299-300),
+    // formed in FP64 from the f32 vertices and rounded to f32: the cross
+    // product of a sliver triangle's nearly parallel edges loses most of its
+    // digits in FP32, and its reflections would leave in another direction.
+    double e1[3] = {(double)t[3] - t[0], (double)t[4] - t[1], (double)t[5] - t[2]};
+    double e2[3] = {(double)t[6] - t[0], (double)t[7] - t[1], (double)t[8] - t[2]};
+    double cd[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    double l = std::sqrt(cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2]);
+    float c[3] = {0.0f, 0.0f, 1.0f};
+    if (l > 0)
+      for (int k = 0; k < 3; k++) c[k] = (float)(cd[k] / l);
     auto el = [&](int a0, int b0) {
       float dx = t[3 * a0] - t[3 * b0], dy = t[3 * a0 + 1] - t[3 * b0 + 1], dz = t[3 * a0 + 2] - t[3 * b0 + 2];
       return std::sqrt(dx * dx + dy * dy + dz * dz);
     };
     float me = std::max(std::max(el(0, 1), el(1, 2)), el(2, 0));
-    aux[i].n_maxedge = make_float4(c[0], c[1], c[2], me);
+    // A triangle whose own plane is not bitwise the plane of its plane id (a
+    // nearly coplanar triangle grouped with others) carries a negative
+    // longest edge (the kernels use its square): a chain reflecting on it
+    // takes its phase length from its own plane.
+    bool inexact = own[4 * (size_t)i] != pl_n[3 * i] || own[4 * (size_t)i + 1] != pl_n[3 * i + 1] ||
+                   own[4 * (size_t)i + 2] != pl_n[3 * i + 2] || own[4 * (size_t)i + 3] != pl_d[i];
+    s->n_inexact += inexact;
+    aux[i].n_maxedge = make_float4(c[0], c[1], c[2], inexact ? -me : me);
     planes[i].n[0] = pl_n[3 * i];
     planes[i].n[1] = pl_n[3 * i + 1];
     planes[i].n[2] = pl_n[3 * i + 2];
@@ -1571,6 +1626,11 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
   std::vector<AmsNode4> N4;
   s->depth4 = ams_bvh_collapse4(B, &N4);
   s->n_tnodes4 = (int)N4.size();
+  std::vector<AmsPlane> tplane(std::max(n_tri, 1));
+  for (int i = 0; i < n_tri; i++) {
+    for (int k = 0; k < 3; k++) tplane[i].n[k] = own[4 * (size_t)i + k];
+    tplane[i].d = own[4 * (size_t)i + 3];
+  }
   std::vector<float4> planes_f(planes.size());
   for (size_t k = 0; k < planes.size(); k++)
     planes_f[k] = make_float4((float)planes[k].n[0], (float)planes[k].n[1], (float)planes[k].n[2], (float)planes[k].d);
@@ -1584,6 +1644,7 @@ int upload_scene(ams_scene *s, int n_tri, const float *tri, const int32_t *mat, 
       dev_upload(&s->d_planes, planes.data(), sizeof(AmsPlane) * planes.size()) ||
       dev_upload(&s->d_planes_f, planes_f.data(), sizeof(float4) * planes_f.size()) ||
       dev_upload(&s->d_pbox, pbox.data(), sizeof(AmsPlaneBox) * pbox.size()) ||
+      dev_upload(&s->d_tplane, tplane.data(), sizeof(AmsPlane) * tplane.size()) ||
       dev_upload(&s->d_mat_prop, mp.data(), sizeof(float) * mp.size()) ||
       dev_upload(&s->d_refl_phase, rp.data(), sizeof(float) * rp.size()))
     return AMS_ERR_HIP;
@@ -1876,9 +1937,17 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   }
   da.perm = (const unsigned *)sc.get("perm", 4 * (size_t)o->fib_n);
   da.dirs = (const float4 *)sc.get("dirs", sizeof(float4) * (size_t)std::max<long long>(1, o->fib_n));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  // the timing events are destroyed on every return path
+  struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  HIPCHK(hipEventCreate(&ev.a));
+  HIPCHK(hipEventCreate(&ev.b));
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   float disc_ms = 0.0f;
   size_t n_unique_total = 0;
   std::vector<uint64_t> uh, ul;
@@ -2171,6 +2240,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ea.planes = s->d_planes;
     ea.planes_f = s->d_planes_f;
     ea.pbox = s->d_pbox;
+    ea.tplane = s->d_tplane;
     ea.src = d_src;
     ea.n_src = n_src;
     ea.dst = d_dst;
@@ -2330,8 +2400,6 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   S.bwd_queries = (long long)hctr[3];
   S.chains = (long long)hctr[4];
   S.successes = (long long)hctr[5];
-  HIPCHK(hipEventDestroy(e0));
-  HIPCHK(hipEventDestroy(e1));
   S.t_total_ms = ms_since(t_start);
   if (stats) *stats = S;
   if (o->keys_out && o->keys_cap > 0) {

@@ -181,6 +181,17 @@ def _operand_grads(gA: torch.Tensor, th_ap, amp, G, WG, U, dims, need_ap: bool):
     return g_ap, g_ms
 
 
+def _row_mask(rowmask: Optional[torch.Tensor], groups: int, C: int, dev) -> Optional[torch.Tensor]:
+    """The capacity objective's row masks as u8 [groups * C] on the device:
+    group g masks its rows rowmask[g * C + c]."""
+    if rowmask is None:
+        return None
+    if rowmask.numel() != groups * C:
+        raise ValueError(f"rowmask has {rowmask.numel()} entries; the objective needs one per row "
+                         f"({groups} groups x {C} codes)")
+    return rowmask.reshape(-1).to(device=dev, dtype=torch.uint8).contiguous()
+
+
 class PhaseStep(torch.autograd.Function):
     """The whole forward of one metasurface with single-bounce chains (the
     operand from the phases, the contraction, the objective) and its
@@ -210,7 +221,7 @@ class PhaseStep(torch.autograd.Function):
         ws = torch.empty(L.ams_objective_workspace_bytes(nm, N), dtype=torch.uint8, device=dev)
         loss = torch.empty(nm, dtype=torch.float32, device=dev)
         cov = torch.empty(nm, dtype=torch.float32, device=dev)
-        rm = rowmask.to(device=dev, dtype=torch.uint8).contiguous() if rowmask is not None else None
+        rm = _row_mask(rowmask, nm, Cn, dev)
         _lib.check(L.ams_objective_fwd_cov(_ptr(Y), nm, Cn, N, mode, n0, w, _ptr(rm) if rm is not None else None,
                                            _ptr(loss), _ptr(cov), _ptr(ws), _stream()),
                    "ams_objective_fwd_cov")
@@ -273,7 +284,7 @@ class Objective(torch.autograd.Function):
         ws = torch.empty(L.ams_objective_workspace_bytes(groups, N), dtype=torch.uint8, device=Y.device)
         loss = torch.empty(groups, dtype=torch.float32, device=Y.device)
         cov = torch.empty(groups, dtype=torch.float32, device=Y.device)
-        rm = rowmask.to(device=Y.device, dtype=torch.uint8).contiguous() if rowmask is not None else None
+        rm = _row_mask(rowmask, groups, Cc, Y.device)
         _lib.check(L.ams_objective_fwd_cov(_ptr(Y), groups, Cc, N, mode, n0, w,
                                            _ptr(rm) if rm is not None else None, _ptr(loss), _ptr(cov), _ptr(ws),
                                            _stream()),

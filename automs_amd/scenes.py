@@ -53,6 +53,7 @@ class Scene:
     mat_prop: np.ndarray
     refl_phase: Optional[np.ndarray] = None
     names: List[str] = field(default_factory=list)
+    info: Dict = field(default_factory=dict)  # generator facts (e.g. the office's door gaps)
 
     @property
     def n_tri(self) -> int:
@@ -72,6 +73,7 @@ class Scene:
             mat_prop=np.concatenate([self.mat_prop, np.full((1, 19), value, np.float32)]),
             refl_phase=np.concatenate([rp, np.asarray([phase], np.float32)]),
             names=self.names + [name],
+            info=self.info,
         )
 
 
@@ -195,6 +197,7 @@ def office_scene(seed: int = 0, size=(24.0, 16.0, 3.0), cell: float = 0.15, refl
     rng = SplitMix64(seed)
     X, Y, Z = size
     groups = {}  # material name -> list of triangle arrays
+    gaps = []  # (corridor wall y, gap x0, gap x1)
 
     def add(name, t):
         groups.setdefault(name, []).append(t)
@@ -216,6 +219,7 @@ def office_scene(seed: int = 0, size=(24.0, 16.0, 3.0), cell: float = 0.15, refl
             x0 = r * rw
             # corridor wall of this room with a 1 m door gap at a seeded spot
             gx = x0 + rng.uniform(0.5, rw - 1.5)
+            gaps.append((wall_y, gx, gx + 1.0))
             for a, b in ((x0, gx), (gx + 1.0, x0 + rw)):
                 if b - a > 0.05:
                     add("inner_wall", _box((a + b) / 2, wall_y, Z / 2, b - a, th, Z))
@@ -250,13 +254,26 @@ def office_scene(seed: int = 0, size=(24.0, 16.0, 3.0), cell: float = 0.15, refl
     gam = np.asarray([rng.uniform(0.3, 0.7) for _ in names], np.float32)
     mat_prop = np.repeat(gam[:, None], 19, axis=1).astype(np.float32)
     return Scene(tri=tri, mat=mat, mat_prop=mat_prop, refl_phase=np.full(len(names), refl_phase, np.float32),
-                 names=names)
+                 names=names, info=dict(door_gaps=gaps, corridor=(cy0, cy1), wall_thickness=th))
+
+
+def _on_wall(x: float, wall_y: float, gaps, clear: float = 0.4) -> float:
+    """x moved along a corridor wall until it is at least `clear` from every
+    door gap of that wall (the nearest such position)."""
+    spans = sorted((a, b) for (y, a, b) in gaps if abs(y - wall_y) < 1e-6)
+    for a, b in spans:
+        if a - clear < x < b + clear:
+            x = a - clear if x - (a - clear) < (b + clear) - x else b + clear
+    return x
 
 
 def office(seed: int = 0, fib_n: int = 1 << 24, ms_fib_n: int = 1 << 24, n_rx=(256, 256), ap=(8, 8),
            ms_size=(48, 48), max_reflection: int = 5, cell: float = 0.15) -> Dict[str, Workload]:
     """Office workload: 8x8 AP (beam pattern 2) on the corridor's west end,
-    three 48 x 48 metasurfaces on room walls, 256 x 256 Rx at z = 1.2 m.
+    three 48 x 48 metasurfaces that the AP sees along the corridor (on the
+    south and north corridor walls, clear of their door gaps, facing the
+    corridor, and on the east wall at the corridor's end), 256 x 256 Rx at
+    z = 1.2 m.
     fib_n rays in total per AP trace (distributed over the elements) and
     ms_fib_n per metasurface trace.  Returns H_d and G{i}, F{i} traces."""
     wl = WAVE_LEN_60GHZ
@@ -266,10 +283,12 @@ def office(seed: int = 0, fib_n: int = 1 << 24, ms_fib_n: int = 1 << 24, n_rx=(2
     rx = gen_pos_from_endpoint([0.1, X - 0.1, n_rx[0]], [0.1, Y - 0.1, n_rx[1]], 1.2).numpy()
     tx_spec = ArraySpec(tx.astype(np.float32), np.array([1, 0, 0], np.float32), 2, 0.5e-5)
     rx_spec = ArraySpec(rx.astype(np.float32), np.array([0, 0, 0], np.float32), 0, 0.06)
-    # metasurfaces: on the corridor-facing wall of two rooms and on the east wall
-    cy0 = Y / 2 - 1.0
-    placements = [([6.0, cy0 - 0.06, 2.0], [0.0, -1.0, 0.0]),
-                  ([14.0, Y / 2 + 1.0 + 0.06, 2.0], [0.0, 1.0, 0.0]),
+    # metasurfaces: on the corridor faces of its two walls and on the east wall
+    cy0, cy1 = scene.info["corridor"]
+    half = scene.info["wall_thickness"] / 2
+    gaps = scene.info["door_gaps"]
+    placements = [([_on_wall(10.0, cy0, gaps), cy0 + half, 2.0], [0.0, 1.0, 0.0]),
+                  ([_on_wall(16.0, cy1, gaps), cy1 - half, 2.0], [0.0, -1.0, 0.0]),
                   ([X, Y / 2, 1.8], [-1.0, 0.0, 0.0])]
     ms_list = []
     for c, k in placements:

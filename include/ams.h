@@ -154,8 +154,9 @@ int ams_cgemm_f32(const float *A, int64_t lda, const float *B, int64_t ldb, int3
 /* ---- coverage objective (go.py:159-245) and Adam (go.py:296-336) -------
  * Y complex f32 [G][C][N]: G groups (phase maps) x C codes x N Rx.
  * mode 0: L_g = -(1/N) sum_r log2(1 + max_c |y|^2 / n0)
- * mode 1: the reference capacity loss with row mask rowmask[C] (u8, device)
- *         and weight w (go.py:214-225). */
+ * mode 1: the reference capacity loss with row masks rowmask[G][C] (u8,
+ *         device; group g masks rows rowmask[g * C + c]) and weight w
+ *         (go.py:214-225). */
 int64_t ams_objective_workspace_bytes(int32_t G, int32_t N);
 int ams_objective_fwd(const float *Y, int32_t G, int32_t C, int32_t N, int32_t mode, float n0, float w,
                       const unsigned char *rowmask, float *loss, void *workspace, void *stream);

@@ -311,21 +311,25 @@ static void spheres_destroy(or_spheres *sp) {
   free(sp);
 }
 
-/* Moller-Trumbore in FP64 with closed edges.  Returns t or -1. margin = the
- * smallest barycentric coordinate times the longest edge (metres from the
- * nearest edge, approximately). */
+/* tie tolerance in metres for counting near-boundary events */
+static double g_tie_eps = 1e-6;
+void or_set_tie_eps(double e) { g_tie_eps = e; }
+
+/* Moller-Trumbore in FP64 with closed edges: the plane distance t (-1 for a
+ * ray parallel to the plane) and the signed margin, the smallest
+ * barycentric coordinate times the longest edge (metres inside the nearest
+ * edge, approximately; negative outside).  A hit needs margin >= 0. */
 static inline double tri_hit(const or_scene *s, int i, v3 o, v3 d, double *margin) {
   v3 e1 = s->e1[i], e2 = s->e2[i];
   v3 pv = cross(d, e2);
   double det = dot(e1, pv);
+  *margin = -INFINITY;
   if (det == 0.0) return -1.0;
   double inv = 1.0 / det;
   v3 tv = sub(o, s->v0[i]);
   double u = dot(tv, pv) * inv;
-  if (u < 0.0 || u > 1.0) return -1.0;
   v3 qv = cross(tv, e1);
   double v = dot(d, qv) * inv;
-  if (v < 0.0 || u + v > 1.0) return -1.0;
   double t = dot(e2, qv) * inv;
   double w = 1.0 - u - v;
   double m = fmin(u, fmin(v, w));
@@ -336,13 +340,15 @@ static inline double tri_hit(const or_scene *s, int i, v3 o, v3 d, double *margi
 typedef struct {
   double t;
   int tri;
-  double margin;   /* edge distance of the accepted hit */
-  double near_miss; /* distance of the closest rejected (near-edge) triangle at smaller t */
+  double margin;    /* edge distance of the accepted hit */
+  double near_miss; /* smallest t of a triangle missed by less than the tie tolerance */
+  int edge_tie;     /* the accepted hit lies within the tie tolerance of an edge */
+  int tmin_tie;     /* a triangle meets the ray within the tie tolerance of tmin */
 } tri_q;
 
 /* closest triangle with t in [tmin, tmax], skipping triangles of plane skip_pl */
 static tri_q closest_tri(const or_scene *s, v3 o, v3 d, double tmin, double tmax, int skip_pl) {
-  tri_q q = {INFINITY, -1, INFINITY, INFINITY};
+  tri_q q = {INFINITY, -1, INFINITY, INFINITY, 0, 0};
   if (s->n_tri == 0) return q;
   v3 inv = mk(1.0 / d.x, 1.0 / d.y, 1.0 / d.z);
   int stack[128], sp = 0;
@@ -357,12 +363,22 @@ static tri_q closest_tri(const or_scene *s, v3 o, v3 d, double tmin, double tmax
         if (s->plane[i] == skip_pl) continue;
         double m;
         double t = tri_hit(s, i, o, d, &m);
+        /* A perturbation of the ray by g_tie_eps moves its meeting point with
+         * the plane by g_tie_eps / |cos| within the plane and along the ray:
+         * the tie band of a grazing leg is wider. */
+        double band = g_tie_eps / fmax(fabs(dot(s->n0[i], d)), 1e-3);
+        if (m > -band && fabs(t - tmin) < band) q.tmin_tie = 1;
+        if (m < 0.0) {
+          if (m > -band && t >= tmin && t < q.near_miss) q.near_miss = t;
+          continue;
+        }
         if (t >= tmin && t <= best) {
           if (t < best || (t == best && i < q.tri)) {
             best = t;
             q.t = t;
             q.tri = i;
             q.margin = m;
+            q.edge_tie = m < band;
           }
         }
       }
@@ -374,20 +390,17 @@ static tri_q closest_tri(const or_scene *s, v3 o, v3 d, double tmin, double tmax
   return q;
 }
 
-/* tie margin of a ray against near-miss triangles closer than t: brute force
- * distance of the ray to the edges is expensive; the margin of the accepted
- * hit plus the Rx boundary margin is what the tests use. */
-
 typedef struct {
   double t;
   int idx;
-  double margin; /* |acc - perp| of the accepted sphere */
+  double margin; /* acc - perp of the accepted sphere */
+  double near_t; /* smallest b of a sphere missed by less than the tie tolerance */
 } sph_q;
 
 /* closest reception sphere: optixPathTracer.cu:436-444 with the acceptance
  * radius above; reported at t = b (closest approach). */
 static sph_q closest_sphere(const or_spheres *sp, v3 o, v3 d, double tmin, double tmax) {
-  sph_q q = {INFINITY, -1, INFINITY};
+  sph_q q = {INFINITY, -1, INFINITY, INFINITY};
   if (!sp || sp->n == 0) return q;
   v3 inv = mk(1.0 / d.x, 1.0 / d.y, 1.0 / d.z);
   int stack[128], top = 0;
@@ -403,6 +416,8 @@ static sph_q closest_sphere(const or_spheres *sp, v3 o, v3 d, double tmin, doubl
         double b = dot(O, d);
         double c = dot(O, O);
         double perp2 = c - b * b;
+        if (perp2 >= acc2 && b >= tmin && c > r2 && sqrt(perp2) - sp->acc < g_tie_eps && b < q.near_t)
+          q.near_t = b;
         if (perp2 < acc2 && b >= tmin && b <= best && c > r2) {
           if (b < best || (b == best && i < q.idx)) {
             best = b;
@@ -451,6 +466,22 @@ static inline int theta_bin(v3 d, v3 N) {
   int id = (int)((th / (M_PI / 2) * 36 + 1) / 2);
   if (id > 18) id = 18;
   return id;
+}
+
+/* The two bins either side of the nearest bin edge when the incidence angle
+ * lies within g_tie_ang radians of that edge (an FP32 angle may round to the
+ * other bin); returns 0 otherwise.  Edges are at (k - 1/2) 5 degrees. */
+static double g_tie_ang = 1e-5;
+void or_set_tie_ang(double a) { g_tie_ang = a; }
+static inline int theta_edge(v3 d, v3 N, int *lo, int *hi) {
+  double c = dot(d, N) / (len(N) * len(d));
+  double th = acos(fmin(1.0, fabs(c)));
+  double x = th / (M_PI / 36) + 0.5; /* bin edges at integer x */
+  double k = floor(x + 0.5);
+  if (fabs(x - k) * (M_PI / 36) >= g_tie_ang || k < 1 || k > 18) return 0;
+  *lo = (int)k - 1;
+  *hi = (int)k;
+  return 1;
 }
 
 /* -------------------------------------------------------------- keys ---- */
@@ -536,9 +567,26 @@ static void kbuf_push(kbuf *kb, uint64_t hi, uint64_t lo) {
   kb->n++;
 }
 
-/* tie tolerance in metres for counting near-boundary events */
-static double g_tie_eps = 1e-6;
-void or_set_tie_eps(double e) { g_tie_eps = e; }
+/* 1: or_discover drops the records of rays with a near tie, and
+ * or_discover_ref those of launch threads with one (test bookkeeping) */
+static int g_skip_ties = 0;
+void or_set_skip_ties(int on) { g_skip_ties = on; }
+
+/* A query's outcome is a near tie when a sphere or triangle decision or the
+ * order of the two hits could go the other way under a perturbation of
+ * g_tie_eps metres: the accepted hits' margins, near misses up to the event
+ * that ends the query, and a sphere hit just before the triangle hit. */
+static inline int query_tie(const tri_q *tq, const sph_q *sq) {
+  double eps = g_tie_eps;
+  int sph = sq->idx >= 0 && sq->t < tq->t;
+  double t_end = sph ? sq->t : tq->t;
+  if (sph && (sq->margin < eps || tq->t - sq->t < eps)) return 1;
+  if (isfinite(sq->near_t) && sq->near_t <= t_end + eps) return 1;
+  if (tq->tri >= 0 && tq->edge_tie) return 1;
+  if (tq->tmin_tie) return 1;
+  if (isfinite(tq->near_miss) && tq->near_miss <= tq->t + eps) return 1;
+  return 0;
+}
 
 /* One primary ray: optixPathTracer.cu:150-189 with closest-hit forward
  * :285-331.  Appends (key, tile) records; optional per-event log. */
@@ -562,8 +610,8 @@ static void forward_ray(const or_scene *s, const or_spheres *rx, v3 o, v3 d, int
       tri_q tq = closest_tri(s, cur.o, cur.d, OR_TMIN, INFINITY, cur.last_pl);
       sph_q sq = closest_sphere(rx, cur.o, cur.d, OR_TMIN, tq.t);
       (*nq)++;
+      if (query_tie(&tq, &sq)) tie = 1;
       if (sq.idx >= 0 && sq.t < tq.t) {
-        if (sq.margin < g_tie_eps || tq.t - sq.t < g_tie_eps) tie = 1;
         int h = K + 1 - cur.ttl;
         uint64_t hi, lo;
         key_pack(h, cur.b, cur.pl, tile_of(sq.idx, n1, gs), &hi, &lo);
@@ -574,7 +622,6 @@ static void forward_ray(const or_scene *s, const or_spheres *rx, v3 o, v3 d, int
         continue;
       }
       if (tq.tri < 0) break; /* miss: optixPathTracer.cu:276-279 */
-      if (tq.margin < g_tie_eps) tie = 1;
       int tri = tq.tri;
       if (log && *log_n + 2 <= log_cap) { log[(*log_n)++] = 0; log[(*log_n)++] = tri; }
       cur.o = add(cur.o, scl(cur.d, tq.t));
@@ -585,6 +632,9 @@ static void forward_ray(const or_scene *s, const or_spheres *rx, v3 o, v3 d, int
         int m = s->mat[tri];
         int th = theta_bin(cur.d, N);
         int pen = s->mat_prop[th + 19 * m] < 0;
+        int blo, bhi;
+        if (theta_edge(cur.d, N, &blo, &bhi) && (s->mat_prop[blo + 19 * m] < 0) != (s->mat_prop[bhi + 19 * m] < 0))
+          tie = 1; /* penetrability of an angle at a bin edge */
         cur.ttl--;
         int h = K + 1 - cur.ttl;
         fbranch ch = cur;
@@ -631,9 +681,10 @@ long or_discover(void *scene, const float *src, int n_src, const float *dst, int
       double dd[3];
       or_fib_dir(k, fib_n, dd);
       v3 o = mk(src[3 * e], src[3 * e + 1], src[3 * e + 2]);
-      long before = kb->n;
+      long before = kb->n, ties_before = nties;
       forward_ray(s, rx, o, mk(dd[0], dd[1], dd[2]), max_reflection, n1_dst, gridsize, kb, &nq, &nties,
                   NULL, 0, NULL);
+      if (g_skip_ties && nties > ties_before) kb->n = before;
       /* drop consecutive duplicates of this ray */
       long m = before;
       for (long i = before; i < kb->n; i++) {
@@ -705,7 +756,7 @@ int or_closest_hit(void *scene, const double *o, const double *d, double tmin, i
   for (int i = 0; i < s->n_tri; i++) {
     double m;
     double t = tri_hit(s, i, O, D, &m);
-    if (t >= tmin && (t < best || (t == best && i < bi))) { best = t; bi = i; }
+    if (m >= 0.0 && t >= tmin && (t < best || (t == best && i < bi))) { best = t; bi = i; }
   }
   *t_out = best;
   return bi;
@@ -734,6 +785,16 @@ static inline double bp(v3 a, v3 b, int type) {
   return or_beam_pattern(x, y, type);
 }
 
+/* 1 when the angle of a to b lies within g_tie_ang of a jump of the beam
+ * pattern: type 2 at 2 pi / 3 (0.24495 -> 0.1), type 3 at pi / 2 */
+static inline int bp_tie(v3 a, v3 b, int type) {
+  if (type != 2 && type != 3) return 0;
+  double c = dot(a, b) / (len(a) * len(b));
+  double th = acos(fmax(-1.0, fmin(1.0, c)));
+  double jump = type == 2 ? M_PI * 2 / 3 : M_PI / 2;
+  return fabs(th - jump) < g_tie_ang;
+}
+
 /* One backward chain: optixPathTracer.cu:223-271 with :332-417.  Returns 1
  * on success and the complex contribution A e^{-j phi} e^{j psi}. */
 static int backward_chain(const or_scene *s, const bw_par *P, int h, int bmask, const unsigned *pl, v3 si,
@@ -744,6 +805,7 @@ static int backward_chain(const or_scene *s, const bw_par *P, int h, int bmask, 
   v3 o = rj;
   v3 d = nrm(sub(I, rj));
   double A = bp(d, P->dst_dir, P->dst_bptype);
+  if (bp_tie(d, P->dst_dir, P->dst_bptype)) *tie = 1;
   double L = 0.0, D = 0.0, Dres = 0.0, far = 1e6, psi = 0.0;
   int ttl = h + 1, ri = 0, last_pl = -1;
   double acc = fmax(P->src_r, OR_ACC_MIN), acc2 = acc * acc, r2 = P->src_r * P->src_r;
@@ -754,13 +816,18 @@ static int backward_chain(const or_scene *s, const bw_par *P, int h, int bmask, 
     v3 O = sub(si, o);
     double bs = dot(O, d), c = dot(O, O), perp2 = c - bs * bs;
     int sh = perp2 < acc2 && bs >= OR_TMIN && c > r2;
-    if (sh && fabs(sqrt(fmax(perp2, 0)) - acc) < g_tie_eps) *tie = 1;
+    /* near ties: the own Tx sphere's boundary, its order against the
+     * triangle hit, the triangle hit's edge and near-missed triangles */
+    if (bs >= OR_TMIN && c > r2 && fabs(sqrt(fmax(perp2, 0)) - acc) < g_tie_eps && bs <= tq.t + g_tie_eps)
+      *tie = 1;
+    if (sh && fabs(tq.t - bs) < g_tie_eps) *tie = 1;
+    if (isfinite(tq.near_miss) && tq.near_miss <= ((sh && bs < tq.t) ? bs : tq.t) + g_tie_eps) *tie = 1;
     if (sh && bs < tq.t) {
-      if (tq.t - bs < g_tie_eps) *tie = 1;
       if (ttl != 1) return 0; /* own Tx reached early: optixPathTracer.cu:396-415 */
       L += bs;
       D += bs;
       double f = bp(scl(d, -1.0), P->src_dir, P->src_bptype) * P->wave_len / (4 * M_PI * (Dres + D));
+      if (bp_tie(scl(d, -1.0), P->src_dir, P->src_bptype)) *tie = 1;
       A *= f;
       double ph = 2 * M_PI * fmod(L / P->wave_len, 1.0) - psi;
       *re = A * cos(ph);
@@ -768,8 +835,9 @@ static int backward_chain(const or_scene *s, const bw_par *P, int h, int bmask, 
       *L_out = L;
       return 1;
     }
+    if (tq.tmin_tie) *tie = 1;
     if (tq.tri < 0) return 0; /* miss */
-    if (tq.margin < g_tie_eps) *tie = 1;
+    if (tq.edge_tie) *tie = 1;
     int tri = tq.tri;
     o = add(o, scl(d, tq.t));
     L += tq.t;
@@ -781,6 +849,9 @@ static int backward_chain(const or_scene *s, const bw_par *P, int h, int bmask, 
     int m = s->mat[tri];
     int th = theta_bin(d, N);
     double loss = fabs(s->mat_prop[th + 19 * m]);
+    int blo, bhi;
+    if (theta_edge(d, N, &blo, &bhi) && fabs(s->mat_prop[blo + 19 * m]) != fabs(s->mat_prop[bhi + 19 * m]))
+      *tie = 1; /* the loss of an angle at a bin edge */
     ttl--;
     int k = h + 1 - ttl;
     int refl = (bmask >> (k - 1)) & 1;
@@ -790,6 +861,9 @@ static int backward_chain(const or_scene *s, const bw_par *P, int h, int bmask, 
       ri++;
     }
     double fspl = 1.0;
+    double dtol = g_tie_eps + 1e-6 * D; /* FP32 path lengths */
+    if (far < 1e6 && fabs(D - far) < dtol) *tie = 1;
+    if (fabs(D - 2.0 * s->maxedge[tri] * s->maxedge[tri] / P->wave_len) < dtol) *tie = 1;
     if (D > far) { /* Fraunhofer split, optixPathTracer.cu:369-391 */
       fspl = P->wave_len / (4 * M_PI * Dres);
       Dres = 0.0;
@@ -810,6 +884,15 @@ static int backward_chain(const or_scene *s, const bw_par *P, int h, int bmask, 
 typedef struct {
   long n_chains, n_success, n_queries, n_ties;
 } or_eval_stats;
+
+/* when set, or_evaluate marks tie_mask[i * n_dst + j] for every (Tx i, Rx j)
+ * with a near-tie chain (test bookkeeping) */
+static unsigned char *g_tie_mask = NULL;
+static long g_tie_mask_n = 0;
+void or_set_tie_mask(unsigned char *mask, long n) {
+  g_tie_mask = mask;
+  g_tie_mask_n = n;
+}
 
 /* Evaluate tasks.  paths: [n_paths][2] u64 keys (tile field ignored);
  * task_path/task_tile: [n_tasks]; H: [n_src][n_dst][2] FP64, accumulated;
@@ -862,6 +945,7 @@ void or_evaluate(void *scene, const float *src, int n_src, const float *dst, int
           hp[1] += im;
         }
         nties += tie;
+        if (tie && g_tie_mask && (long)i * n_dst + j < g_tie_mask_n) g_tie_mask[(long)i * n_dst + j] = 1;
       }
     }
     if (task_ok) task_ok[t] = (unsigned char)ok;
@@ -1125,6 +1209,98 @@ void or_ref_rays(int launch_x, int launch_y, int samples, int n_src, int *elem, 
   }
 }
 
+/* One launch thread t of reference-mode discovery: its samples, each a
+ * 32-slot branch tree walked in slot order, with per-thread min_hc gating and
+ * the reserve cap (optixPathTracer.cu:143-189).  Keys go to kb; when log is
+ * given, every Rx event appends (t, sample, h, tile, recorded) to it. */
+static long ref_thread(const or_scene *s, const or_spheres *rx, const float *src, int n_src, int n1_dst,
+                       int gridsize, int n_grid, int K, int launch_x, int launch_y, int samples, int reserve,
+                       int gating, long t, int *min_hc, fbranch *slot, kbuf *kb, int *log, long log_cap,
+                       long *log_n, long *n_tie_threads) {
+  long nq = 0, kb_before = kb->n;
+  int tie = 0;
+  int ix = (int)(t / launch_y), iy = (int)(t % launch_y);
+  unsigned seed = or_tea4((unsigned)t, 0u);
+  memset(min_hc, 0, sizeof(int) * n_grid);
+  int res_cnt = 0;
+  for (int smp = 0; smp < samples; smp++) {
+    float dd[3];
+    int e = or_ref_sample(&seed, ix, iy, launch_x, launch_y, n_src, dd);
+    for (int k = 0; k < (1 << OR_MAX_HIT); k++) slot[k].ttl = 0;
+    slot[0].o = mk(src[3 * e], src[3 * e + 1], src[3 * e + 2]);
+    slot[0].d = nrm(mk(dd[0], dd[1], dd[2]));
+    slot[0].ttl = K + 1;
+    slot[0].b = 0;
+    slot[0].nref = 0;
+    slot[0].last_pl = -1;
+    for (int k = 0; k < OR_MAX_HIT; k++) slot[0].pl[k] = OR_NO_PLANE;
+    for (int i = 0; i < (1 << OR_MAX_HIT); i++) {
+      fbranch cur = slot[i];
+      while (cur.ttl > 0) {
+        tri_q tq = closest_tri(s, cur.o, cur.d, OR_TMIN, INFINITY, cur.last_pl);
+        sph_q sq = closest_sphere(rx, cur.o, cur.d, OR_TMIN, tq.t);
+        nq++;
+        if (query_tie(&tq, &sq)) tie = 1;
+        if (sq.idx >= 0 && sq.t < tq.t) {
+          int grid = (int)tile_of(sq.idx, n1_dst, gridsize);
+          int h = K + 1 - cur.ttl;
+          int rec = 0;
+          if (!gating || res_cnt < reserve) {
+            if (!gating || min_hc[grid] <= h) {
+              min_hc[grid] = h + 1;
+              uint64_t hi, lo;
+              key_pack(h, cur.b, cur.pl, (unsigned)grid, &hi, &lo);
+              kbuf_push(kb, hi, lo);
+              res_cnt++;
+              rec = 1;
+            }
+          }
+          if (log && *log_n + 5 <= log_cap) {
+            int *ev = log + *log_n;
+            ev[0] = (int)t;
+            ev[1] = smp;
+            ev[2] = h;
+            ev[3] = grid;
+            ev[4] = rec;
+            *log_n += 5;
+          }
+          cur.o = add(cur.o, scl(cur.d, sq.t));
+          cur.last_pl = -1;
+          continue;
+        }
+        if (tq.tri < 0) break;
+        int tri = tq.tri;
+        cur.o = add(cur.o, scl(cur.d, tq.t));
+        cur.last_pl = s->plane[tri];
+        if (cur.ttl > 1) {
+          v3 N0 = s->n0[tri];
+          v3 N = dot(N0, scl(cur.d, -1.0)) < 0 ? scl(N0, -1.0) : N0;
+          int m = s->mat[tri];
+          int thb = theta_bin(cur.d, N);
+          int pen = s->mat_prop[thb + 19 * m] < 0;
+          int blo, bhi;
+          if (theta_edge(cur.d, N, &blo, &bhi) &&
+              (s->mat_prop[blo + 19 * m] < 0) != (s->mat_prop[bhi + 19 * m] < 0))
+            tie = 1;
+          cur.ttl--;
+          int h = K + 1 - cur.ttl;
+          int cs = cur.b | (1 << (h - 1));
+          fbranch ch = cur;
+          ch.b = cs;
+          ch.d = sub(cur.d, scl(N, 2.0 * dot(cur.d, N)));
+          if (ch.nref < OR_MAX_HIT) ch.pl[ch.nref] = (unsigned)s->plane[tri];
+          ch.nref++;
+          slot[cs] = ch;
+          if (!pen) break;
+        }
+      }
+    }
+  }
+  if (tie && g_skip_ties) kb->n = kb_before;
+  if (tie && n_tie_threads) (*n_tie_threads)++;
+  return nq;
+}
+
 long or_discover_ref(void *scene, const float *src, int n_src, const float *dst, int n0_dst, int n1_dst,
                      double dst_r, int max_reflection, int gridsize, int launch_x, int launch_y, int samples,
                      int reserve, int gating, uint64_t *out, long cap, long *stats) {
@@ -1132,14 +1308,13 @@ long or_discover_ref(void *scene, const float *src, int n_src, const float *dst,
   or_spheres *rx = spheres_create(n0_dst * n1_dst, dst, dst_r);
   int n1g = (n1_dst - 1) / gridsize + 1, n0g = (n0_dst - 1) / gridsize + 1, n_grid = n0g * n1g;
   long nthr = (long)launch_x * launch_y;
-  int K = max_reflection;
   int nt = 1;
 #ifdef _OPENMP
   nt = omp_get_max_threads();
 #endif
   kbuf *bufs = calloc(nt, sizeof(kbuf));
-  long nq = 0;
-#pragma omp parallel reduction(+ : nq)
+  long nq = 0, ntie = 0;
+#pragma omp parallel reduction(+ : nq, ntie)
   {
     int tid = 0;
 #ifdef _OPENMP
@@ -1149,69 +1324,9 @@ long or_discover_ref(void *scene, const float *src, int n_src, const float *dst,
     int *min_hc = malloc(sizeof(int) * n_grid);
     fbranch slot[1 << OR_MAX_HIT];
 #pragma omp for schedule(dynamic, 16)
-    for (long t = 0; t < nthr; t++) {
-      int ix = (int)(t / launch_y), iy = (int)(t % launch_y);
-      unsigned seed = or_tea4((unsigned)t, 0u);
-      memset(min_hc, 0, sizeof(int) * n_grid);
-      int res_cnt = 0;
-      for (int smp = 0; smp < samples; smp++) {
-        float dd[3];
-        int e = or_ref_sample(&seed, ix, iy, launch_x, launch_y, n_src, dd);
-        for (int k = 0; k < (1 << OR_MAX_HIT); k++) slot[k].ttl = 0;
-        slot[0].o = mk(src[3 * e], src[3 * e + 1], src[3 * e + 2]);
-        slot[0].d = nrm(mk(dd[0], dd[1], dd[2]));
-        slot[0].ttl = K + 1;
-        slot[0].b = 0;
-        slot[0].nref = 0;
-        slot[0].last_pl = -1;
-        for (int k = 0; k < OR_MAX_HIT; k++) slot[0].pl[k] = OR_NO_PLANE;
-        for (int i = 0; i < (1 << OR_MAX_HIT); i++) {
-          fbranch cur = slot[i];
-          while (cur.ttl > 0) {
-            tri_q tq = closest_tri(s, cur.o, cur.d, OR_TMIN, INFINITY, cur.last_pl);
-            sph_q sq = closest_sphere(rx, cur.o, cur.d, OR_TMIN, tq.t);
-            nq++;
-            if (sq.idx >= 0 && sq.t < tq.t) {
-              if (!gating || res_cnt < reserve) {
-                int grid = (int)tile_of(sq.idx, n1_dst, gridsize);
-                int h = K + 1 - cur.ttl;
-                if (!gating || min_hc[grid] <= h) {
-                  min_hc[grid] = h + 1;
-                  uint64_t hi, lo;
-                  key_pack(h, cur.b, cur.pl, (unsigned)grid, &hi, &lo);
-                  kbuf_push(kb, hi, lo);
-                  res_cnt++;
-                }
-              }
-              cur.o = add(cur.o, scl(cur.d, sq.t));
-              cur.last_pl = -1;
-              continue;
-            }
-            if (tq.tri < 0) break;
-            int tri = tq.tri;
-            cur.o = add(cur.o, scl(cur.d, tq.t));
-            cur.last_pl = s->plane[tri];
-            if (cur.ttl > 1) {
-              v3 N0 = s->n0[tri];
-              v3 N = dot(N0, scl(cur.d, -1.0)) < 0 ? scl(N0, -1.0) : N0;
-              int m = s->mat[tri];
-              int thb = theta_bin(cur.d, N);
-              int pen = s->mat_prop[thb + 19 * m] < 0;
-              cur.ttl--;
-              int h = K + 1 - cur.ttl;
-              int cs = cur.b | (1 << (h - 1));
-              fbranch ch = cur;
-              ch.b = cs;
-              ch.d = sub(cur.d, scl(N, 2.0 * dot(cur.d, N)));
-              if (ch.nref < OR_MAX_HIT) ch.pl[ch.nref] = (unsigned)s->plane[tri];
-              ch.nref++;
-              slot[cs] = ch;
-              if (!pen) break;
-            }
-          }
-        }
-      }
-    }
+    for (long t = 0; t < nthr; t++)
+      nq += ref_thread(s, rx, src, n_src, n1_dst, gridsize, n_grid, max_reflection, launch_x, launch_y, samples,
+                       reserve, gating, t, min_hc, slot, kb, NULL, 0, NULL, &ntie);
     free(min_hc);
   }
   long total = 0;
@@ -1230,6 +1345,7 @@ long or_discover_ref(void *scene, const float *src, int n_src, const float *dst,
     stats[1] = nq;
     stats[2] = m;
     stats[3] = total;
+    stats[4] = ntie;
   }
   long ret = m <= cap ? m : -1;
   if (ret >= 0)
@@ -1237,4 +1353,26 @@ long or_discover_ref(void *scene, const float *src, int n_src, const float *dst,
   free(all);
   spheres_destroy(rx);
   return ret;
+}
+
+/* The Rx events of launch threads [t_begin, t_end) of or_discover_ref, in
+ * the order each thread meets them: 5 ints per event (t, sample, h, tile,
+ * recorded).  Returns the number of ints written (at most log_cap). */
+long or_discover_ref_log(void *scene, const float *src, int n_src, const float *dst, int n0_dst, int n1_dst,
+                         double dst_r, int max_reflection, int gridsize, int launch_x, int launch_y, int samples,
+                         int reserve, int gating, long t_begin, long t_end, int *log, long log_cap) {
+  or_scene *s = (or_scene *)scene;
+  or_spheres *rx = spheres_create(n0_dst * n1_dst, dst, dst_r);
+  int n1g = (n1_dst - 1) / gridsize + 1, n0g = (n0_dst - 1) / gridsize + 1, n_grid = n0g * n1g;
+  int *min_hc = malloc(sizeof(int) * n_grid);
+  fbranch slot[1 << OR_MAX_HIT];
+  kbuf kb = {0};
+  long n = 0;
+  for (long t = t_begin; t < t_end; t++)
+    ref_thread(s, rx, src, n_src, n1_dst, gridsize, n_grid, max_reflection, launch_x, launch_y, samples, reserve,
+               gating, t, min_hc, slot, &kb, log, log_cap, &n, NULL);
+  free(kb.k);
+  free(min_hc);
+  spheres_destroy(rx);
+  return n;
 }

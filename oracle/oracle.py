@@ -42,6 +42,8 @@ def lib():
         L.or_beam_pattern.restype = C.c_double
         L.or_fib_dir.argtypes = [C.c_long, C.c_long, dp]
         L.or_set_tie_eps.argtypes = [C.c_double]
+        L.or_set_skip_ties.argtypes = [C.c_int]
+        L.or_set_tie_mask.argtypes = [C.POINTER(C.c_ubyte), C.c_long]
         L.or_discover.argtypes = [vp, fp, C.c_int, fp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_long,
                                   C.c_int, C.c_long, C.c_long, u64p, C.c_long, lp]
         L.or_discover.restype = C.c_long
@@ -67,6 +69,10 @@ def lib():
         L.or_ref_rays.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), fp]
         L.or_ref_rays.restype = None
         L.or_discover_ref.restype = C.c_long
+        L.or_discover_ref_log.argtypes = [vp, fp, C.c_int, fp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, ip,
+                                          C.c_long]
+        L.or_discover_ref_log.restype = C.c_long
         _lib = L
     return _lib
 
@@ -143,6 +149,37 @@ def set_tie_eps(e):
     lib().or_set_tie_eps(float(e))
 
 
+class tie_bookkeeping:
+    """Context for tie-attributed parity checks: inside it, discovery drops
+    the records of rays (Fibonacci mode) or launch threads (reference mode)
+    that meet a near tie when skip=True, and evaluation marks every (Tx, Rx)
+    entry with a near-tie chain in .mask ([n_src, n_dst] bool) when n_src,
+    n_dst are given.  eps: the tie tolerance in metres."""
+
+    def __init__(self, eps=1e-6, skip=False, n_src=0, n_dst=0):
+        self.eps, self.skip = eps, skip
+        self._m = np.zeros(max(1, n_src * n_dst), np.uint8)
+        self.shape = (n_src, n_dst)
+        self.mask = None
+
+    def __enter__(self):
+        L = lib()
+        L.or_set_tie_eps(float(self.eps))
+        L.or_set_skip_ties(1 if self.skip else 0)
+        if self.shape[0] * self.shape[1]:
+            L.or_set_tie_mask(self._m.ctypes.data_as(C.POINTER(C.c_ubyte)), self._m.size)
+        return self
+
+    def __exit__(self, *exc):
+        L = lib()
+        L.or_set_tie_eps(1e-6)
+        L.or_set_skip_ties(0)
+        L.or_set_tie_mask(None, 0)
+        if self.shape[0] * self.shape[1]:
+            self.mask = self._m.reshape(self.shape).astype(bool)
+        return False
+
+
 def discover(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, fib_n, per_element, ray_begin=0,
              ray_end=None, cap=1 << 24):
     src_a, sp = _f(np.asarray(src).reshape(-1, 3))
@@ -193,14 +230,33 @@ def discover_ref(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, lau
     n0, n1 = dst.shape[0], dst.shape[1]
     dst_a, dp = _f(dst.reshape(-1, 3))
     out = np.zeros((cap, 2), np.uint64)
-    st = np.zeros(4, np.int64)
+    st = np.zeros(5, np.int64)
     m = lib().or_discover_ref(sc.h, sp, src_a.shape[0], dp, n0, n1, dst_r, max_reflection, gridsize, launch_x,
                               launch_y, samples, reserve, 1 if gating else 0,
                               out.ctypes.data_as(C.POINTER(C.c_uint64)), cap,
                               st.ctypes.data_as(C.POINTER(C.c_long)))
     if m < 0:
         raise RuntimeError("oracle discovery cap exceeded")
-    return out[:m].copy(), dict(rays=int(st[0]), queries=int(st[1]), unique=int(st[2]), raw=int(st[3]))
+    return out[:m].copy(), dict(rays=int(st[0]), queries=int(st[1]), unique=int(st[2]), raw=int(st[3]),
+                                tie_threads=int(st[4]))
+
+
+def discover_ref_log(sc: OracleScene, src, dst, dst_r, max_reflection, gridsize, launch_x, launch_y, samples,
+                     reserve, t_begin, t_end, gating: bool = True, cap=1 << 22):
+    """The Rx events of launch threads [t_begin, t_end) of discover_ref in the
+    order each thread meets them: int array [n, 5] of (thread, sample, h,
+    tile, recorded)."""
+    src_a, sp = _f(np.asarray(src).reshape(-1, 3))
+    dst = np.asarray(dst)
+    n0, n1 = dst.shape[0], dst.shape[1]
+    dst_a, dp = _f(dst.reshape(-1, 3))
+    log = np.zeros(cap, np.int32)
+    n = lib().or_discover_ref_log(sc.h, sp, src_a.shape[0], dp, n0, n1, dst_r, max_reflection, gridsize, launch_x,
+                                  launch_y, samples, reserve, 1 if gating else 0, t_begin, t_end,
+                                  log.ctypes.data_as(C.POINTER(C.c_int)), cap)
+    if n + 5 > cap:
+        raise RuntimeError("oracle event log cap reached")
+    return log[:n].reshape(-1, 5).copy()
 
 
 def trace_ray_log(sc: OracleScene, o, d, dst, dst_r, max_reflection, gridsize, cap=256):

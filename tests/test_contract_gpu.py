@@ -119,6 +119,51 @@ def test_model_loss_and_gradient_vs_oracle(gpu, objective):
     assert rel_gm < 2e-3
 
 
+@pytest.mark.parametrize("fused_step", [True, False])
+def test_capacity_row_masks_per_map(gpu, fused_step):
+    """The capacity objective with two phase maps and a different row mask
+    per map (rows g * C + c): loss and phase gradients against the
+    complex128 oracle evaluated map by map with that map's mask, both
+    through PhaseStep and through Contract + Objective; a mask of the wrong
+    length is refused before any launch."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    from oracle import contract_ref as R
+    _flush(f"case: capacity row masks per map fused_step={fused_step}")
+    Hd, G, F = _channels(M=16, K=128, N=700, seed=21)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (8, 16), None, None, [], max_phase=360.0)]
+    th_ms = torch.rand(2, 128, generator=torch.Generator().manual_seed(4)) * 2 * math.pi
+    C = 12
+    mask = torch.zeros(2 * C, dtype=torch.bool)
+    mask[[1, 4, 7]] = True          # map 0
+    mask[C + np.array([0, 2, 9, 11])] = True  # map 1: other rows
+    model = PhaseMapModel("t", ch, ms, code_num=C, w=0.5, objective="capacity", ms_theta=[th_ms], n_maps=2)
+    model.fused_step = fused_step
+    model.rowmask_fn = lambda: mask
+    loss = model()
+    loss.backward()
+    g_ms = model.ms_theta[0].grad.cpu().double()
+    g_ap = model.antenna_theta.grad.cpu().double()
+    ta = model.antenna_theta.detach().cpu().double().requires_grad_(True)
+    tm = th_ms.double().clone().requires_grad_(True)
+    Y = R.received(model.antenna_A.cpu(), ta, Hd, [G], [F], [tm], [2 * math.pi])
+    lo = sum(R.loss_capacity(Y[g * C:(g + 1) * C], model.n0, 0.5, mask[g * C:(g + 1) * C]) for g in range(2))
+    lo.backward()
+    rel_l = abs(loss.item() - lo.item()) / abs(lo.item())
+    rel_m = ((g_ms - tm.grad).norm() / tm.grad.norm()).item()
+    rel_a = ((g_ap - ta.grad).norm() / ta.grad.norm()).item()
+    # the same model with map 0's mask on both maps gives another loss
+    lo0 = sum(R.loss_capacity(Y[g * C:(g + 1) * C], model.n0, 0.5, mask[:C]) for g in range(2))
+    _flush(f"  loss gpu {loss.item():.5f} oracle {lo.item():.5f} (map 0's mask on both: {lo0.item():.5f}); "
+           f"rel {rel_l:.1e}, g_ms rel {rel_m:.1e}, g_ap rel {rel_a:.1e}")
+    assert rel_l < 1e-4 and rel_m < 2e-3 and rel_a < 2e-3
+    assert abs(lo0.item() - lo.item()) > 1e-3 * abs(lo.item())
+    model.rowmask_fn = lambda: mask[:C]
+    with pytest.raises(ValueError):
+        model()
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_objective_ties_mask_and_coverage(gpu, mode):
     """The objective kernels on Y with exact ties (codes 9 and 17 copy code
