@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/ams.h"
@@ -23,6 +24,12 @@
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Split partials of ams_cgemm_frag_noalloc: a file-scope device array, so
+// that call allocates nothing.  Each GPU holds its own copy until the process
+// ends; kernels get its device address (scratch_dev), never the symbol.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -405,6 +412,32 @@ void launch_cgemm_wg(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, i
   k_cgemm_wg<CJ><<<dim3(blocks, 1, splits), 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, C, ldc, part);
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+int scratch_dev(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    ams::set_error(std::string("ams_cgemm_frag_noalloc: no current device: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      ams::set_error(std::string("ams_cgemm_frag_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      return AMS_ERR_HIP;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return AMS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -480,6 +513,31 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
     return AMS_ERR_HIP;
   }
   return AMS_OK;
+}
+
+int64_t ams_cgemm_noalloc_max_bytes(void) { return (int64_t)kScratchBytes; }
+
+int ams_cgemm_frag_noalloc(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b,
+                           float *C, int64_t ldc, int32_t splits, void *stream) {
+  if (splits < 1 || Kd <= 0) {
+    ams::set_error("ams_cgemm_frag_noalloc: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  const int n_kb = (Kd + 31) / 32;
+  if (splits > n_kb) splits = n_kb;  // as ams_cgemm_frag does
+  void *ws = nullptr;
+  if (splits > 1) {
+    const int64_t need = ams_cgemm_workspace_bytes(Ra, Rb, splits);
+    if (need > (int64_t)kScratchBytes) {
+      ams::set_error("ams_cgemm_frag_noalloc: " + std::to_string(splits) + " split partials of " +
+                     std::to_string(Ra) + " x " + std::to_string(Rb) + " need " + std::to_string(need) +
+                     " bytes, beyond the " + std::to_string(kScratchBytes) + "-byte scratch");
+      return AMS_ERR_ARG;
+    }
+    int rc = scratch_dev(&ws);
+    if (rc) return rc;
+  }
+  return ams_cgemm_frag(A, B, Ra, Rb, Kd, conj_b, C, ldc, splits, ws, stream);
 }
 
 int ams_cgemm_f32(const float *A, int64_t lda, const float *B, int64_t ldb, int32_t Ra, int32_t Rb, int32_t Kd,

@@ -147,6 +147,18 @@ int ams_frag_pack(const float *X, int32_t R, int32_t Kd, int64_t sr, int64_t sk,
 int64_t ams_cgemm_workspace_bytes(int32_t Ra, int32_t Rb, int32_t splits);
 int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b, float *C,
                    int64_t ldc, int32_t splits, void *workspace, void *stream);
+/* ams_cgemm_frag without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  The
+ * split partials live in a library-owned device array of
+ * ams_cgemm_noalloc_max_bytes() (32 MiB) per GPU, held until the process
+ * ends; a call whose ams_cgemm_workspace_bytes(Ra, Rb, splits) exceeds it
+ * returns AMS_ERR_ARG.  The call allocates nothing.  Calls on one GPU share
+ * that array: calls that overlap on one GPU (other streams or threads) race
+ * and must be ordered by the caller. */
+int64_t ams_cgemm_noalloc_max_bytes(void);
+int ams_cgemm_frag_noalloc(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b,
+                           float *C, int64_t ldc, int32_t splits, void *stream);
 /* the same product in plain FP32 on the vector ALU (row strides in elements) */
 int ams_cgemm_f32(const float *A, int64_t lda, const float *B, int64_t ldb, int32_t Ra, int32_t Rb, int32_t Kd,
                   int32_t conj_b, float *C, int64_t ldc, void *stream);
