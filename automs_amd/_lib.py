@@ -110,6 +110,7 @@ def _bind_contract(L):
         "ams_cgemm_frag": ([vp, vp, i32, i32, i32, i32, vp, i64, i32, vp, vp], C.c_int),
         "ams_cgemm_f32": ([vp, i64, vp, i64, i32, i32, i32, i32, vp, i64, vp], C.c_int),
         "ams_cgemm_noalloc_max_bytes": ([], i64),
+        "ams_valu_probe": ([vp, i32, i32, i32, vp], C.c_int),
         "ams_cgemm_frag_noalloc": ([vp, vp, i32, i32, i32, i32, vp, i64, i32, vp], C.c_int),
         "ams_objective_workspace_bytes": ([i32, i32], i64),
         "ams_objective_fwd": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp], C.c_int),
@@ -152,5 +153,5 @@ EXPORTED = [
     "ams_objective_bwd_frag",
     "ams_host_bvh_build", "ams_host_bvh4_build", "ams_host_plane_ids",
     "ams_conv3x3_fwd", "ams_conv3x3_bwd_weight", "ams_relu_bwd", "ams_mse_partials", "ams_mse", "ams_to_bf16",
-    "ams_cgemm_noalloc_max_bytes", "ams_cgemm_frag_noalloc",
+    "ams_cgemm_noalloc_max_bytes", "ams_cgemm_frag_noalloc", "ams_valu_probe",
 ]

@@ -16,7 +16,7 @@ LIB = PKG / "libams.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["ams_capi.cpp", "ams_bvh.cpp", "ams_trace.hip", "ams_contract.hip", "ams_objective.hip", "ams_phasemap.hip",
-           "ams_surrogate.hip"]
+           "ams_surrogate.hip", "ams_probe.hip"]
 
 # -packed-fp32-ops: keep the compiler from emitting v_pk_{add,mul,fma}_f32
 HIP_FLAGS = [

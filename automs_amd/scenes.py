@@ -273,7 +273,9 @@ def office(seed: int = 0, fib_n: int = 1 << 24, ms_fib_n: int = 1 << 24, n_rx=(2
     three 48 x 48 metasurfaces that the AP sees along the corridor (on the
     south and north corridor walls, clear of their door gaps, facing the
     corridor, and on the east wall at the corridor's end), 256 x 256 Rx at
-    z = 1.2 m.
+    z = 1.2 m.  Traces: H_d, G{i} (AP -> metasurface i), F{i} (metasurface i
+    -> Rx) and T{i}{j} (metasurface i -> j, i < j), as build_channels forms
+    them; OFFICE_CHANNEL_KEYS gives their go.py channel names.
     fib_n rays in total per AP trace (distributed over the elements) and
     ms_fib_n per metasurface trace.  Returns H_d and G{i}, F{i} traces."""
     wl = WAVE_LEN_60GHZ
@@ -306,4 +308,18 @@ def office(seed: int = 0, fib_n: int = 1 << 24, ms_fib_n: int = 1 << 24, n_rx=(2
                                 max_reflection, fib_n, False, ms=ms_list)
         out[f"F{i}"] = Workload(f"office/F{i}", sc, ArraySpec(m["pos"], m["normal"], 1, 0.5e-5), rx_spec, wl,
                                 max_reflection, ms_fib_n, False, ms=ms_list)
+    # metasurface i -> j (i < j), the others absorbing (go.py:1226-1245)
+    for i in range(len(ms_list)):
+        for j in range(i + 1, len(ms_list)):
+            others = [t for k, mm in enumerate(ms_list) if k not in (i, j) for t in mm["blackhole"]]
+            sc = scene.with_extra(others, 0.0, "blackhole")
+            out[f"T{i}{j}"] = Workload(f"office/T{i}{j}", sc, ArraySpec(ms_list[i]["pos"], ms_list[i]["normal"], 1,
+                                                                       0.5e-5),
+                                       ArraySpec(ms_list[j]["pos"], ms_list[j]["normal"], 1, 0.06), wl,
+                                       max_reflection, ms_fib_n, False, ms=ms_list)
     return out
+
+
+OFFICE_CHANNEL_KEYS = {"H_d": "src_to_dst", "G0": "src_to_ms{0}", "G1": "src_to_ms{1}", "G2": "src_to_ms{2}",
+                       "F0": "ms{0}_to_dst", "F1": "ms{1}_to_dst", "F2": "ms{2}_to_dst",
+                       "T01": "ms{0}_to_ms{1}", "T02": "ms{0}_to_ms{2}", "T12": "ms{1}_to_ms{2}"}

@@ -9,8 +9,17 @@ Tx/Rx arrays) are resident before the timed region.
 Prints one JSON line:
   value        closest-hit queries per second (Mrays/s), whole job
   rx_channels_per_s   n_tx * n_rx / step seconds, summed over the traces
-  roofline     the dominant kernel's algorithmic bytes / HIP-event time
+  roofline     the dominant kernel against its bound, VALU instruction
+               issue (wave64 VALU instructions per query from rocprofv3
+               PMC x live queries / HIP-event time), with the LDS bytes of
+               the staged scene and the HBM traffic beside it
   cpu_baseline the FP64 oracle on a bounded sample of the same workload
+  office_full  the office workload at its stated size: 2^24 rays per trace,
+               H_d, G_i, F_i of three 48 x 48 metasurfaces and the
+               metasurface-to-metasurface traces T_ij, traced concurrently
+
+`python bench.py --pmc-summary DIR` prints the per-kernel table of the
+rocprofv3 counter CSV files under DIR (the profiles/ summaries).
 """
 from __future__ import annotations
 
@@ -26,17 +35,112 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# VALU issue: two wave64 VALU instructions per CU per cycle (FP32 and 32-bit
+# integer adds issue at 1.86 per CU per cycle in the ams_valu_probe counter
+# run, profiles/valu_issue_ceiling.txt; FMA at 0.97, half rate), 256 CUs at
+# the 2.4 GHz peak engine clock.  LDS: 128 bytes per clock per CU.
+N_CU = 256
+PEAK_CLOCK_HZ = 2.4e9
+VALU_ISSUE_PER_CU_CYCLE = 2.0
+VALU_PEAK_GINST = VALU_ISSUE_PER_CU_CYCLE * N_CU * PEAK_CLOCK_HZ / 1e9
+LDS_PEAK_GBS = N_CU * 128 * PEAK_CLOCK_HZ / 1e9
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (calibrated against a 1 GiB device copy); the table and the commands that
-# produced it are in profiles/hbm_traffic.md.
+# in KiB per dispatch, scaled by 2 and 1 (a 1 GiB device copy reads back as
+# 0.5 GiB fetched and 1 GiB written), and the L2 hit rate TCC_HIT / (TCC_HIT
+# + TCC_MISS); profiles/hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("room_boxes_ms", "k_discover"): {"bytes": 213.9e6 + 178.4e6, "l2_hit": 0.945},
-    ("room_boxes_ms", "k_evaluate"): {"bytes": 2.2e6 + 43.0e6, "l2_hit": 0.862},
-    ("office", "k_discover"): {"bytes": 523.8e6 + 283.4e6, "l2_hit": 0.926},
-    ("office", "k_evaluate"): {"bytes": 8.9e6 + 255.2e6, "l2_hit": 0.953},
+    ("room_boxes_ms", "k_discover"): {"bytes": 1.049e5 * 2048 + 1.771e5 * 1024, "l2_hit": 0.945},
+    ("room_boxes_ms", "k_evaluate"): {"bytes": 850.8 * 2048 + 4.155e4 * 1024, "l2_hit": 0.582},
+    ("office", "k_discover"): {"bytes": 2.424e5 * 2048 + 1.976e5 * 1024, "l2_hit": 0.922},
+    ("office", "k_evaluate"): {"bytes": 4585 * 2048 + 3.583e5 * 1024, "l2_hit": 0.864},
     ("office_batched", "k_cgemm"): {"bytes": 1547.4e6 + 134.2e6, "l2_hit": 0.782, "mfma_util": 0.592},
 }
+
+
+# VALU wave64 instructions per closest-hit query, from rocprofv3 --pmc
+# SQ_INSTS_VALU over a profiled run divided by that run's queries (the
+# tracer's counters of the same run); profiles/valu_tracer_*.txt.
+MEASURED_VALU = {
+    ("room_boxes_ms", "k_evaluate"): 1.096e9 * 26 / 1517460974,     # 18.78, k_evaluate<true, false>
+    ("room_boxes_ms", "k_discover"): 2.284e9 * 6 / 500351026,       # 27.39, k_discover<true>
+    ("office", "k_evaluate"): 3.218e10 * 157 / 113655616856,        # 44.45, k_evaluate<false, false>
+    ("office", "k_discover"): 7.414e9 * 10 / 1387450191,            # 53.43, k_discover_wide<false>
+}
+
+
+def pmc_summary(path: str) -> str:
+    """Per-kernel table of the rocprofv3 counter-collection CSV files under
+    path: dispatches, mean duration and the mean of every counter per
+    dispatch, with the VALU-issue fraction (SQ_INSTS_VALU per cycle per CU
+    at the peak clock), VALUBusy, lane utilisation and wait fraction when
+    those counters are present."""
+    import csv
+    import glob
+    import re
+    rows = {}
+    for fn in sorted(glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True)):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                name = re.sub(r"\(anonymous namespace\)::", "", r["Kernel_Name"])
+                name = re.sub(r"\((?!anonymous).*$", "", name).strip()
+                name = re.sub(r"^void ", "", name)
+                k = rows.setdefault(name, {"disp": set(), "dur": {}, "c": {}})
+                did = r.get("Dispatch_Id") or r.get("Correlation_Id")
+                k["disp"].add(did)
+                if r.get("Start_Timestamp") and r.get("End_Timestamp"):
+                    k["dur"][did] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+                k["c"][r["Counter_Name"]] = k["c"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    out = []
+    for name, k in sorted(rows.items(), key=lambda kv: -sum(kv[1]["dur"].values())):
+        n = len(k["disp"])
+        dur = sum(k["dur"].values()) / max(1, len(k["dur"]))
+        c = {a: v / n for a, v in k["c"].items()}
+        line = f"{name[:70]:70s} disp {n:5d} avg {dur * 1e3:9.3f} ms"
+        for a in sorted(c):
+            line += f"  {a} {c[a]:.4g}"
+        if "SQ_INSTS_VALU" in c and dur > 0:
+            line += (f"  | VALU/CU/cycle {c['SQ_INSTS_VALU'] / (dur * PEAK_CLOCK_HZ * N_CU):.3f} at 2.4 GHz, "
+                     f"VALU issue frac {c['SQ_INSTS_VALU'] / (dur * PEAK_CLOCK_HZ * N_CU * VALU_ISSUE_PER_CU_CYCLE):.3f}")
+            if c.get("GRBM_GUI_ACTIVE"):
+                line += f", {c['SQ_INSTS_VALU'] / N_CU / (c['GRBM_GUI_ACTIVE'] / 8):.3f} per GUI-active cycle"
+        if "SQ_ACTIVE_INST_VALU" in c and c.get("GRBM_GUI_ACTIVE"):
+            line += f"  VALUBusy {100 * c['SQ_ACTIVE_INST_VALU'] / N_CU / (c['GRBM_GUI_ACTIVE'] / 8):.1f}%"
+        if "SQ_THREAD_CYCLES_VALU" in c and c.get("SQ_ACTIVE_INST_VALU"):
+            line += f"  lanes {c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.3f}"
+        if "SQ_WAIT_INST_ANY" in c and c.get("SQ_WAVE_CYCLES"):
+            line += f"  wait {c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:.3f}"
+        out.append(line)
+    return "\n".join(out)
+
+
+def valu_roofline(workload: str, kernel: str, queries: float, launches: int, kernel_ms: float,
+                  alg_bytes: float, lds_staged: bool) -> dict:
+    """The roofline object of a tracer kernel: VALU instruction issue (the
+    bound: about 30 instructions per node visit and lanes 96% active), with
+    the LDS bytes of the staged scene and the measured HBM traffic beside
+    it.  Live: launches, queries and HIP-event time of the timed steps."""
+    per_launch_s = kernel_ms * 1e-3 / max(1, launches)
+    ipq = MEASURED_VALU.get((workload, kernel))
+    tr = MEASURED_TRAFFIC.get((workload, kernel), {})
+    out = {"kernel": kernel, "bound": "valu", "unit": "Ginst/s", "peak": VALU_PEAK_GINST,
+           "avg_launch_ms": per_launch_s * 1e3, "queries_per_launch": queries / max(1, launches),
+           "valu_inst_per_query": ipq,
+           "achieved": None, "frac": None, "traffic": tr.get("bytes"), "l2_hit_rate": tr.get("l2_hit")}
+    if ipq is not None:
+        out["achieved"] = ipq * queries / max(1, launches) / per_launch_s / 1e9
+        out["frac"] = out["achieved"] / VALU_PEAK_GINST
+    if tr.get("bytes"):
+        out["hbm"] = {"gb_per_s": tr["bytes"] / per_launch_s / 1e9, "frac": tr["bytes"] / per_launch_s / 1e9 /
+                      HBM_PEAK_GBS}
+    abps = alg_bytes / max(1, launches) / per_launch_s / 1e9
+    if lds_staged:
+        out["lds"] = {"algorithmic_gb_per_s": abps, "peak": LDS_PEAK_GBS, "frac": abps / LDS_PEAK_GBS,
+                      "what": "32 B per node visit and 48 B per triangle test, read from the LDS-staged scene"}
+    else:
+        out["scene_fetch"] = {"algorithmic_gb_per_s": abps,
+                              "what": "32 B per node visit and 48 B per triangle test, served by L2 and MALL"}
+    return out
 
 
 def kernel_bytes(stats: dict) -> dict:
@@ -59,7 +163,7 @@ def workload_traces(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int, office: bool = False) -> dict:
+def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int) -> dict:
     """FP64 oracle (OpenMP) on the same scene with fewer rays per trace."""
     from automs_amd import scenes
     from oracle import oracle as O
@@ -89,16 +193,6 @@ def cpu_baseline(name: str, sample_fib: int, sample_ms_fib: int, office: bool = 
         what += f", a 16 x 16 metasurface and {sample_ms_fib} rays in total from it"
     out = {"value": q / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
            "sample": f"{what} ({q} closest-hit queries in {dt:.1f} s; FP64 oracle, OpenMP)"}
-    if office:
-        w = scenes.office(fib_n=1 << 10, ms_fib_n=1 << 12)["H_d"]
-        t0 = time.perf_counter()
-        _, st, _ = O.trace_workload(w)
-        dto = time.perf_counter() - t0
-        qo = st["fwd_queries"] + st["bwd_queries"]
-        out["office_H_d"] = {"mrays_per_s": qo / dto / 1e6, "seconds": dto, "queries": qo,
-                             "rx_channels_per_s": w.tx.n * w.rx.n / dto,
-                             "sample": f"office H_d ({w.scene.n_tri} triangles, 64 x 65,536) with {w.fib_n} "
-                                       "discovery rays; the neighbour expansion then traces every reachable tile"}
     return out
 
 
@@ -253,6 +347,105 @@ def contraction_office(iters: int = 20) -> dict:
                               "traces": traced}}
 
 
+def office_full(steps: int, warmup: int, cpu: bool, profile: bool = False) -> dict:
+    """The office workload at its stated size (100,046 triangles, 5
+    reflections, 2^24 rays per trace): H_d [64 x 65,536], G_i [64 x 2304],
+    F_i [2304 x 65,536] of three 48 x 48 metasurfaces and T_ij [2304 x 2304],
+    the ten traces of build_channels (go.py:1135-1247), traced concurrently
+    by tracer.trace_many.  A step is all ten traces; then one untimed step
+    with the traces in turn gives each kernel's launch durations alone on the
+    GPU."""
+    import torch
+
+    from automs_amd import scenes
+    from automs_amd.tracer import TraceScene, trace, trace_many
+    dev = torch.device("cuda:0")
+    tr = scenes.office()
+    handles = {k: TraceScene(w.scene) for k, w in tr.items()}
+    outs = {k: torch.zeros(w.tx.n, w.rx.n, 2, dtype=torch.float32, device=dev) for k, w in tr.items()}
+
+    def job(k):
+        w = tr[k]
+        return dict(ts=handles[k], tx=w.tx, rx=w.rx, wave_len=w.wave_len, max_reflection=w.max_reflection,
+                    fib_n=w.fib_n, per_element=w.per_element, gridsize=w.gridsize, result=outs[k])
+
+    def step():
+        for k in tr:
+            outs[k].zero_()
+        return {k: r[1] for k, r in zip(tr, trace_many([job(k) for k in tr]))}
+
+    for i in range(warmup):
+        step()
+        print(f"office warmup {i + 1}/{warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = {k: {} for k in tr}
+    for i in range(steps):
+        sts = step()
+        for k, st in sts.items():
+            for f, v in st.items():
+                per[k][f] = per[k].get(f, 0) + v
+        print(f"office step {i + 1}/{steps}: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tot = {}
+    for k in tr:
+        for f, v in per[k].items():
+            tot[f] = tot.get(f, 0) + v
+    queries = tot["fwd_queries"] + tot["bwd_queries"]
+    iso = {}
+    if not profile:
+        for k in tr:
+            outs[k].zero_()
+            _, st, _ = trace(**job(k))
+            for f, v in st.items():
+                iso[f] = iso.get(f, 0) + v
+        torch.cuda.synchronize()
+    ref = iso or tot
+    ikt = {"k_discover": ref["t_discover_ms"], "k_evaluate": ref["t_evaluate_ms"]}
+    dom = max(ikt, key=ikt.get)
+    q_of = {"k_discover": "fwd_queries", "k_evaluate": "bwd_queries"}
+    l_of = {"k_discover": "n_discover_launches", "k_evaluate": "n_evaluate_launches"}
+    t_of = {"k_discover": "t_discover_ms", "k_evaluate": "t_evaluate_ms"}
+    kb = kernel_bytes(tot)
+    roof = valu_roofline("office", dom, tot[q_of[dom]], tot[l_of[dom]], tot[t_of[dom]], kb[dom], False)
+    if iso:
+        ikb = kernel_bytes(iso)
+        r_iso = valu_roofline("office", dom, iso[q_of[dom]], iso[l_of[dom]], iso[t_of[dom]], ikb[dom], False)
+        roof["isolated"] = {k: r_iso[k] for k in ("avg_launch_ms", "achieved", "frac")}
+        roof["isolated"]["step_seconds"] = (iso["t_total_ms"]) * 1e-3
+    n_channels = sum(w.tx.n * w.rx.n for w in tr.values())
+    out = {
+        "workload": "office, 100,046 triangles, <= 5 reflections, 8 x 8 AP, 3 metasurfaces of 48 x 48, "
+                    "256 x 256 Rx, 2^24 rays per trace; traces H_d, G0-2, F0-2, T01, T02, T12 concurrently",
+        "steps": steps, "warmup": warmup, "seconds_per_step": dt / steps,
+        "mrays_per_s": queries / dt / 1e6, "rx_channels_per_s": n_channels * steps / dt,
+        "primary_rays_per_s": tot["rays"] / dt,
+        "traces": {k: {"n_tx": tr[k].tx.n, "n_rx": tr[k].rx.n, "rays": per[k]["rays"] // steps,
+                       "paths": per[k]["paths"] // steps, "successes": per[k]["successes"] // steps,
+                       "queries": (per[k]["fwd_queries"] + per[k]["bwd_queries"]) // steps,
+                       "wall_ms": per[k]["t_total_ms"] / steps} for k in tr},
+        "roofline": roof,
+        "totals": {f: tot[f] for f in ("fwd_queries", "bwd_queries", "fwd_nodes", "bwd_nodes",
+                                       "n_discover_launches", "n_evaluate_launches")},
+    }
+    if cpu:
+        from oracle import oracle as O
+        w = scenes.office(fib_n=1 << 10, ms_fib_n=1 << 12)["H_d"]
+        t1 = time.perf_counter()
+        _, st, _ = O.trace_workload(w)
+        dto = time.perf_counter() - t1
+        qo = st["fwd_queries"] + st["bwd_queries"]
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": qo / dto / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+                               "seconds": dto, "rx_channels_per_s": w.tx.n * w.rx.n / dto,
+                               "sample": f"office H_d ({w.scene.n_tri} triangles, 64 x 65,536) with {w.fib_n} "
+                                         f"discovery rays ({qo} closest-hit queries in {dto:.1f} s; FP64 oracle, "
+                                         "OpenMP); the neighbour expansion then traces every reachable tile"}
+        out["gpu_over_cpu_mrays"] = out["mrays_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,9 +464,20 @@ def main():
                     help="short run for rocprof counter passes: 1 traced step, 5 contraction GEMMs and a "
                          "1 GiB device copy that calibrates FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-office", action="store_true", help="skip the office workload at its stated size")
+    ap.add_argument("--office-steps", type=int, default=2)
+    ap.add_argument("--office-warmup", type=int, default=1)
+    ap.add_argument("--pmc-summary", metavar="DIR", help="print the per-kernel table of rocprofv3 counter CSVs")
+    ap.add_argument("--only-office", action="store_true",
+                    help="run only the office workload at its stated size and print its object")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="run the VALU issue-rate probe kernels (for a counter run) and exit")
     ap.add_argument("--cpu-sample-fib", type=int, default=1 << 18)
     ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the path shards by Rx tiles "
               "across GPUs by design only)", file=sys.stderr)
@@ -284,6 +488,28 @@ def main():
     from automs_amd.tracer import TraceScene, trace, trace_many
 
     dev = torch.device("cuda:0")
+    if args.valu_probe:
+        import ctypes as C
+
+        from automs_amd import _lib
+        blocks = N_CU * 32
+        buf = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for kind in (0, 1, 2, 3):
+            for _ in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                _lib.check(_lib.lib().ams_valu_probe(C.c_void_p(buf.data_ptr()), blocks, 20000, kind, st),
+                           "ams_valu_probe")
+                e1.record()
+                torch.cuda.synchronize()
+                print(f"valu probe kind {kind}: {e0.elapsed_time(e1):.3f} ms", flush=True)
+        return
+    if args.only_office:
+        r = office_full(1 if args.profile_pass else args.office_steps, 0 if args.profile_pass else args.office_warmup,
+                        cpu=False, profile=args.profile_pass)
+        print(json.dumps(r), flush=True)
+        return
     if args.profile_pass:
         args.steps, args.warmup, args.no_opt, args.no_cpu_baseline = 1, 0, True, True
         x = torch.ones(1 << 28, dtype=torch.float32, device=dev)
@@ -354,8 +580,14 @@ def main():
     ikl = {"k_discover": iso["n_discover_launches"], "k_evaluate": iso["n_evaluate_launches"]}
     ikb = kernel_bytes(iso)
     dom = max(ikt, key=ikt.get)
-    achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
-    iso_achieved = ikb[dom] / (ikt[dom] * 1e-3) / 1e9
+    q_of = {"k_discover": "fwd_queries", "k_evaluate": "bwd_queries"}
+    lds = args.workload != "office"  # the room scenes are staged in LDS
+    roof = valu_roofline(args.workload, dom, tot[q_of[dom]], kl[dom], kt[dom], kb[dom], lds)
+    r_iso = valu_roofline(args.workload, dom, iso[q_of[dom]], ikl[dom], ikt[dom], ikb[dom], lds)
+    roof["isolated"] = {k: r_iso.get(k) for k in ("avg_launch_ms", "achieved", "frac")}
+    roof["streams"] = 1 if args.sequential else len(traces)
+    roof["note"] = ("live durations are those of the timed steps, whose traces run concurrently on separate "
+                    "streams and share the GPU; isolated = one untimed step with the traces in turn")
     n_channels = sum(w.tx.n * w.rx.n for w in traces.values())
     line = {
         "metric": "Mrays/s & Rx-channels/s; phase-map opt sec to <=0.1 dB of plain optimum",
@@ -382,25 +614,7 @@ def main():
                                                       "fwd_queries", "bwd_queries", "fwd_nodes", "fwd_tris",
                                                       "fwd_spheres", "bwd_nodes", "bwd_tris", "t_discover_ms",
                                                       "t_evaluate_ms", "t_total_ms")},
-        "roofline": {
-            "kernel": dom,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC.get((args.workload, dom), {}).get("bytes"),
-            "l2_hit_rate": MEASURED_TRAFFIC.get((args.workload, dom), {}).get("l2_hit"),
-            "avg_launch_ms": kt[dom] / max(1, kl[dom]),
-            "algorithmic_bytes_per_launch": kb[dom] / max(1, kl[dom]),
-            "streams": 1 if args.sequential else len(traces),
-            "isolated": {"avg_launch_ms": ikt[dom] / max(1, ikl[dom]), "achieved": iso_achieved,
-                         "frac": iso_achieved / HBM_PEAK_GBS},
-            "note": "algorithmic bytes count every node, triangle and sphere fetch; the scene is staged in LDS "
-                    "per workgroup, so they are served on chip and frac can exceed 1; traffic is the measured "
-                    "HBM bytes per launch. Live durations are those of the timed steps, whose traces run "
-                    "concurrently on separate streams; isolated = one untimed step with the traces in turn",
-        },
+        "roofline": roof,
     }
     if args.workload == "room_boxes_ms" and not args.no_opt:
         from automs_amd.phasemap import Metasurface, channels_from_traces, time_to_target
@@ -430,8 +644,20 @@ def main():
         print(f"contraction: {r}", file=sys.stderr, flush=True)
         line["contraction_office_batched"] = r
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_fib, args.cpu_sample_ms_fib,
-                                            office=not args.no_contract)
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_fib, args.cpu_sample_ms_fib)
+    if not args.no_office and args.workload != "office":
+        r = office_full(1 if args.profile_pass else args.office_steps, 0 if args.profile_pass else args.office_warmup,
+                        cpu=not args.no_cpu_baseline, profile=args.profile_pass)
+        print(f"office: {r}", file=sys.stderr, flush=True)
+        line["office_full"] = r
+    if args.profile_pass:
+        # every query traced in this process (timed and isolated steps), for
+        # the per-query instruction counts of a counter run
+        room = {f: tot.get(f, 0) + iso.get(f, 0) for f in ("fwd_queries", "bwd_queries", "fwd_nodes", "bwd_nodes",
+                                                         "n_discover_launches", "n_evaluate_launches")}
+        line["profile_totals"] = {args.workload: room}
+        if "office_full" in line:
+            line["profile_totals"]["office"] = line["office_full"]["totals"]
     print(json.dumps(line), flush=True)
 
 

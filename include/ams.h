@@ -268,6 +268,14 @@ int ams_host_bvh4_build(const double *lo, const double *hi, int32_t n, int32_t l
 /* plane id of every triangle as ams_scene_plane_ids reports it */
 int ams_host_plane_ids(const float *tri, int32_t n_tri, int32_t *plane_out);
 
+/* ---- measurement tooling --------------------------------------------------
+ * VALU issue-rate probe, no counterpart in the reference: independent VALU
+ * instruction streams at full occupancy (kind 0: non-packed FP32 FMA, 1:
+ * 32-bit integer shift / xor / add, 2: FP32 add, 3: 32-bit integer add) for a
+ * counter run that calibrates the VALU-issue ceiling of the tracer kernels'
+ * roofline.  out: device, blocks * 256 floats (one per lane). */
+int ams_valu_probe(float *out, int32_t blocks, int32_t iters, int32_t kind, void *stream);
+
 const char *ams_last_error(void);
 int ams_version(void);
 /* device bytes currently held by the library (cudaMemManager.h:75-81) */
