@@ -12,11 +12,20 @@ bf16 weights with FP32 master copies, all forward/backward math in
 libams.so (ams_conv3x3_fwd / ams_conv3x3_bwd_weight / ams_relu_bwd /
 ams_mse), Adam via ams_adam_step.
 
-Target of a placement: the per-Rx gain (dB) a phase-aligned metasurface can
-add to the AP's broadside beam,
+Score of a placement (PlacementTracer.trial_score): what a trial of
+physical_exp_nni_model.py:96-149 reports for it, the coverage after optimising
+the AP codebook and the metasurface phases together (30 codes, Adam lr 0.3,
+the mean-log-SNR objective, best parameters kept): the mean over the Rx grid
+of the best code's power in dB, minus the same figure for the optimised
+codebook without a metasurface.  The network's label is that difference per
+Rx (trial_gain_map, a 64 x 64 map whose mean is the score).
+
+A cheaper proxy, the per-Rx gain a metasurface with phases matched to every Rx
+at once could add to the AP's broadside beam,
     h_d = sum_tx H_d[tx, rx],  g_k = sum_tx G[tx, k],  m = sum_k |g_k| |F[k, rx]|,
-    gain = 20 log10((|h_d| + m + eps) / (|h_d| + eps)), clamped to [0, 40] dB,
-and the placement score is its mean over the Rx grid (coverage gain, dB).
+    gain = 20 log10((|h_d| + m + eps) / (|h_d| + eps)), clamped to [0, 40] dB
+(gain_map), is an upper bound no single phase map reaches and does not rank
+placements like the trial (rank_check measures both against it).
 """
 from __future__ import annotations
 
@@ -32,11 +41,13 @@ import torch
 
 from . import _lib
 from .geometry import gen_ms
+from .phasemap import Metasurface, PhaseMapModel, optimize_model
 from .scenes import ArraySpec, SplitMix64, Workload, room_boxes, room_boxes_scene, WAVE_LEN_60GHZ
 from .tracer import TraceScene, trace
 
 ROOM = (10.0, 8.0, 3.0)
 GRID = 64
+LABEL_SCALE = 1.0  # the network predicts gain / LABEL_SCALE (dB)
 EPS = 1e-7
 MS_SIZE = (32, 32)
 
@@ -246,29 +257,40 @@ def features(occ: np.ndarray, p: Placement) -> np.ndarray:
 # ------------------------------------------------------------------ tracing --
 
 class PlacementTracer:
-    """Traces the gain map of placements in one scene: H_d once, then G and
-    F per placement (reduced ray counts; the scene handle is reused)."""
+    """Traces placements in one scene: H_d once, then G and F per placement
+    (reduced ray counts; the scene handle is reused).  gain_map gives the
+    network's training target, trial_score the optimised coverage gain that
+    a trial reports."""
 
-    def __init__(self, scene_seed: int, fib_ap: int = 1 << 18, fib_ms: int = 1 << 18, max_reflection: int = 3):
+    def __init__(self, scene_seed: int, fib_ap: int = 1 << 18, fib_ms: int = 1 << 18, max_reflection: int = 3,
+                 trial_steps: int = 40, code_num: int = 30):
         self.base = room_boxes(scene_seed, fib_n=fib_ap, max_reflection=max_reflection)
         self.ts = TraceScene(self.base.scene)
         self.fib_ap, self.fib_ms = fib_ap, fib_ms
         w = self.base
         res, _, _ = trace(self.ts, w.tx, w.rx, w.wave_len, w.max_reflection, fib_ap, True, w.gridsize)
-        self.hd = torch.complex(res[..., 0], res[..., 1]).sum(0)  # broadside beam
+        self.Hd = torch.complex(res[..., 0], res[..., 1])
+        self.hd = self.Hd.sum(0)  # broadside beam
+        self.trial_steps, self.code_num = trial_steps, code_num
+        self._base_cov = None
+        self._trials = 0
         self.n_traces = 0
         self.gpu_seconds = 0.0
 
-    def gain_map(self, p: Placement) -> torch.Tensor:
+    def channels(self, p: Placement):
+        """G [n_ap, K] and F [K, n_rx] of a placement, complex."""
         w = self.base
         pos, _, nrm = p.ms()
-        t0 = time.perf_counter()
         resG, _, _ = trace(self.ts, w.tx, ArraySpec(pos, nrm, 1, 0.06), w.wave_len, w.max_reflection, self.fib_ap,
                            True, w.gridsize)
         resF, _, _ = trace(self.ts, ArraySpec(pos, nrm, 1, 0.5e-5), w.rx, w.wave_len, w.max_reflection, self.fib_ms,
                            False, w.gridsize)
-        g = torch.complex(resG[..., 0], resG[..., 1]).sum(0)
-        F = torch.complex(resF[..., 0], resF[..., 1])
+        return torch.complex(resG[..., 0], resG[..., 1]), torch.complex(resF[..., 0], resF[..., 1])
+
+    def gain_map(self, p: Placement) -> torch.Tensor:
+        t0 = time.perf_counter()
+        G, F = self.channels(p)
+        g = G.sum(0)
         m = (g.abs()[:, None] * F.abs()).sum(0)
         a = self.hd.abs()
         gain = (20 * torch.log10((a + m + EPS) / (a + EPS))).clamp(0, 40).reshape(GRID, GRID)
@@ -277,11 +299,59 @@ class PlacementTracer:
         self.n_traces += 1
         return gain
 
+    @torch.no_grad()
+    def _power_db(self, model: PhaseMapModel) -> torch.Tensor:
+        Y = model.received()
+        return 10 * torch.log10((Y.real ** 2 + Y.imag ** 2).amax(0) + 1e-30)
+
+    def _optimised_power_db(self, ch: dict, ms_list) -> torch.Tensor:
+        """Per-Rx best-code power (dB) after optimising the codebook (and the
+        metasurface phases) for trial_steps Adam steps."""
+        model = PhaseMapModel("trial", ch, ms_list, code_num=self.code_num, objective="logsnr")
+        optimize_model(model, lr=0.3, step=self.trial_steps, graph=True, record=False)
+        pw = self._power_db(model)
+        del model
+        # every trial's graph leaves its private memory pool in torch's cache
+        # once freed; hand those pools back to the device now and then, so the
+        # tracer's own allocations (outside torch) keep finding memory
+        self._trials += 1
+        if self._trials % 16 == 0:
+            torch.cuda.empty_cache()
+        return pw
+
+    def base_power_db(self) -> torch.Tensor:
+        """Per-Rx power of the optimised codebook without a metasurface."""
+        if self._base_cov is None:
+            self._base_cov = self._optimised_power_db({"src_to_dst": self.Hd}, [])
+        return self._base_cov
+
+    def trial_gain_map(self, p: Placement) -> torch.Tensor:
+        """Per-Rx coverage gain (dB, [64, 64]) of the placement after its
+        trial: the AP codebook and the metasurface phases optimised together
+        (physical_exp_nni_model.py:96-149 with the log-SNR objective) against
+        the optimised codebook alone.  H_d is traced once per scene, without
+        the metasurface's footprint (a 16 cm square of wall)."""
+        t0 = time.perf_counter()
+        base = self.base_power_db()
+        G, F = self.channels(p)
+        ms = Metasurface(None, MS_SIZE, None, None, [], max_phase=360.0)
+        pw = self._optimised_power_db({"src_to_dst": self.Hd, "src_to_ms{0}": G, "ms{0}_to_dst": F}, [ms])
+        gain = (pw - base).reshape(GRID, GRID)
+        torch.cuda.synchronize()
+        self.gpu_seconds += time.perf_counter() - t0
+        self.n_traces += 1
+        return gain
+
+    def trial_score(self, p: Placement) -> float:
+        """The trial's coverage gain (dB): the mean of trial_gain_map."""
+        return float(self.trial_gain_map(p).mean())
+
 
 def generate_dataset(n_scenes: int, per_scene: int, seed: int = 0, fib_ap: int = 1 << 18, fib_ms: int = 1 << 18,
-                     log=None, first_scene: int = 0):
-    """Traced gain maps of random placements: returns (placements [N, 3]
-    (wall, s, z), scene seeds [N], gains [N, 64, 64] f32 on the GPU)."""
+                     log=None, first_scene: int = 0, label: str = "trial"):
+    """Gain maps of random placements (label "trial": trial_gain_map;
+    "proxy": gain_map): returns (placements [N, 3] (wall, s, z), scene seeds
+    [N], gains [N, 64, 64] f32 on the GPU)."""
     rng = SplitMix64(seed * 1000003 + first_scene)
     P, S_, Ys = [], [], []
     for sidx in range(first_scene, first_scene + n_scenes):
@@ -290,7 +360,7 @@ def generate_dataset(n_scenes: int, per_scene: int, seed: int = 0, fib_ap: int =
             p = random_placement(rng)
             P.append([p.wall, p.s, p.z])
             S_.append(sidx)
-            Ys.append(pt.gain_map(p))
+            Ys.append(pt.trial_gain_map(p) if label == "trial" else pt.gain_map(p))
         if log:
             log(f"scene {sidx + 1 - first_scene}/{n_scenes}: {pt.n_traces} placements traced in {pt.gpu_seconds:.1f} s")
         del pt
@@ -306,9 +376,9 @@ def build_inputs(placements: np.ndarray, scene_ids: np.ndarray) -> torch.Tensor:
 
 
 def save_dataset(path: str, placements, scene_ids, gains: torch.Tensor) -> None:
-    """Gains quantised to uint8 in 40/255 dB steps (the clamp range)."""
-    q = torch.round(gains.clamp(0, 40) * (255 / 40)).to(torch.uint8).cpu().numpy()
-    np.savez_compressed(path, placements=placements, scene_ids=scene_ids, gains_u8=q)
+    """Gains as float16 (the trial gains are small and signed)."""
+    np.savez_compressed(path, placements=placements, scene_ids=scene_ids,
+                        gains_f16=gains.cpu().numpy().astype(np.float16))
 
 
 def load_dataset(paths: List[str]):
@@ -317,14 +387,13 @@ def load_dataset(paths: List[str]):
         d = np.load(p)
         P.append(d["placements"])
         S_.append(d["scene_ids"])
-        G.append(d["gains_u8"])
-    g = torch.from_numpy(np.concatenate(G)).float() * (40 / 255)
-    return np.concatenate(P), np.concatenate(S_), g.cuda()
+        G.append(d["gains_f16"].astype(np.float32) if "gains_f16" in d else d["gains_u8"] * np.float32(40 / 255))
+    return np.concatenate(P), np.concatenate(S_), torch.from_numpy(np.concatenate(G)).cuda()
 
 
 def train(net: SurrogateNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20, batch: int = 64, lr: float = 2e-3,
-          seed: int = 0, log=None) -> List[float]:
-    """Minibatch Adam on the MSE of the gain map (scaled by 1/10)."""
+          seed: int = 0, log=None, scale: float = LABEL_SCALE) -> List[float]:
+    """Minibatch Adam on the MSE of the gain map (divided by scale)."""
     g = torch.Generator().manual_seed(seed)
     n = X.shape[0]
     hist = []
@@ -336,7 +405,7 @@ def train(net: SurrogateNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20,
         for i in range(0, n - batch + 1, batch):
             idx = perm[i:i + batch].to(X.device)
             pred = net.forward(X[idx], keep=True)
-            loss, gpred = net.mse(pred, Y[idx] / 10)
+            loss, gpred = net.mse(pred, Y[idx] / scale)
             net.backward(gpred)
             net.adam_step(lr_ep)
             tot += loss.detach()
@@ -349,7 +418,7 @@ def train(net: SurrogateNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20,
 
 def predict_score(net: SurrogateNet, occ: np.ndarray, ps: List[Placement]) -> torch.Tensor:
     X = torch.from_numpy(np.stack([features(occ, p) for p in ps])).cuda()
-    return net.forward(X).mean((1, 2, 3)) * 10
+    return net.forward(X).mean((1, 2, 3)) * LABEL_SCALE
 
 
 # ---------------------------------------------------------------- annealing --
@@ -398,19 +467,43 @@ def calibrate_temperature(batch_score, seed: int, pairs: int = 32, chi0: float =
     return t0, t0 / ratio
 
 
+def rank_check(pt: PlacementTracer, net: "SurrogateNet", occ: np.ndarray, n: int = 64, seed: int = 77) -> dict:
+    """On n random placements of the search scene: Spearman rank correlation
+    of the proxy score (traced matched-phase gain) and of the network's
+    prediction with the trial score, and the trial score of the placement
+    each of them ranks first against the best trial score."""
+    from scipy.stats import spearmanr
+    rng = SplitMix64(seed)
+    ps = [random_placement(rng) for _ in range(n)]
+    trial = np.asarray([pt.trial_score(p) for p in ps])
+    proxy = np.asarray([float(pt.gain_map(p).mean()) for p in ps])
+    pred = predict_score(net, occ, ps).cpu().numpy() if net is not None else None
+    out = {"placements": n, "spearman_proxy_vs_trial": float(spearmanr(proxy, trial)[0]),
+           "trial_db_best": float(trial.max()), "trial_db_of_proxy_best": float(trial[int(np.argmax(proxy))]),
+           "trial_db_mean": float(trial.mean())}
+    if pred is not None:
+        out["spearman_prediction_vs_trial"] = float(spearmanr(pred, trial)[0])
+        out["trial_db_of_prediction_best"] = float(trial[int(np.argmax(pred))])
+    return out
+
+
 def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 2000, search_scene: int = 1000,
                    epochs: int = 20, fib: int = 1 << 18, seed: int = 0, log=print, datasets=None,
                    t0: Optional[float] = None, t1: Optional[float] = None, chi0: float = 0.2,
-                   t_ratio: float = 1000.0, net_path: Optional[str] = None, searches=("plain", "surrogate")) -> dict:
+                   t_ratio: float = 1000.0, net_path: Optional[str] = None, searches=("plain", "surrogate"),
+                   judge: str = "trial", rank_n: int = 64) -> dict:
     """Config-4 experiment: traced dataset -> trained surrogate -> placement
     search three ways on an unseen scene: plain annealing (every candidate
-    traced), surrogate-guided annealing (accepted candidates traced), and
-    plain annealing cut to the surrogate search's number of traces.
+    scored by the judge), surrogate-guided annealing (accepted candidates
+    scored by the judge), and plain annealing cut to the surrogate search's
+    number of judge calls.  judge "trial": PlacementTracer.trial_score (the
+    optimised coverage gain a trial reports); "proxy": the mean traced
+    matched-phase gain.
 
     With t0 / t1 None each search calibrates its own schedule with
-    calibrate_temperature on its own score function (plain: traced pairs,
-    counted in its time and traces; surrogate: predicted pairs)."""
-    out: Dict = {"sa_steps": sa_steps, "search_scene": search_scene}
+    calibrate_temperature on its own score function (plain: judged pairs,
+    counted in its time and calls; surrogate: predicted pairs)."""
+    out: Dict = {}
     t_start = time.perf_counter()
     if datasets:
         P, S_, Y = load_dataset(datasets)
@@ -437,19 +530,36 @@ def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 20
     torch.cuda.synchronize()
     out["train_seconds"] = time.perf_counter() - tt
     with torch.no_grad():
-        pv = torch.cat([net.forward(Xva[i:i + 64]) for i in range(0, n_val, 64)]) * 10
+        pv = torch.cat([net.forward(Xva[i:i + 64]) for i in range(0, n_val, 64)]) * LABEL_SCALE
         mse_val = float(((pv - Yva) ** 2).mean())
         base = float(((Ytr.mean() - Yva) ** 2).mean())
         sv, tv = pv.mean((1, 2, 3)), Yva.mean((1, 2, 3))
         r2_score = 1 - float(((sv - tv) ** 2).sum() / ((tv - tv.mean()) ** 2).sum())
-    out.update(train_mse_final_db2=hist[-1] * 100, val_mse_db2=mse_val, val_mse_mean_predictor_db2=base,
+    out.update(train_mse_final_db2=hist[-1] * LABEL_SCALE ** 2, val_mse_db2=mse_val, val_mse_mean_predictor_db2=base,
                val_score_r2=r2_score)
     log(f"validation: map mse {mse_val:.2f} dB^2 (mean predictor {base:.2f}); placement-score R^2 {r2_score:.3f}")
+    out.update(search_experiment(net, sa_steps, search_scene, fib, seed, log, chi0, t_ratio, searches, judge, rank_n,
+                                 t0, t1))
+    return out
 
+
+def search_experiment(net: "SurrogateNet", sa_steps: int = 2000, search_scene: int = 1000, fib: int = 1 << 18,
+                      seed: int = 0, log=print, chi0: float = 0.2, t_ratio: float = 1000.0,
+                      searches=("plain", "surrogate"), judge: str = "trial", rank_n: int = 64,
+                      t0: Optional[float] = None, t1: Optional[float] = None) -> dict:
+    """The searches of run_experiment with a trained network, on the unseen
+    scene search_scene: the rank check, plain annealing, surrogate-guided
+    annealing and plain annealing cut to the surrogate search's judge calls."""
+    out: Dict = {"sa_steps": sa_steps, "search_scene": search_scene, "judge": judge}
     occ = occupancy(search_scene)
     pt = PlacementTracer(search_scene, fib, fib)
+    if rank_n:
+        out["rank_check"] = rank_check(pt, net, occ, rank_n)
+        log(f"rank check on {rank_n} held-out placements: {out['rank_check']}")
 
     def traced(p):
+        if judge == "trial":
+            return pt.trial_score(p)
         return float(pt.gain_map(p).mean())
 
     def timed(fn):
@@ -493,6 +603,7 @@ def run_experiment(n_scenes: int = 20, per_scene: int = 1000, sa_steps: int = 20
         out["gain_gap_db"] = v1 - v2
         out["time_ratio"] = s2 / s1
         out["surrogate_gain_over_same_budget_db"] = v2 - v3
+    pt.ts.close()
     return out
 
 
@@ -515,6 +626,8 @@ if __name__ == "__main__":
     ap.add_argument("--chi0", type=float, default=0.2, help="start acceptance of a median downhill move")
     ap.add_argument("--t-ratio", type=float, default=1000.0, help="start / end temperature")
     ap.add_argument("--searches", default="plain,surrogate")
+    ap.add_argument("--judge", default="trial", choices=["trial", "proxy"])
+    ap.add_argument("--rank-n", type=int, default=64)
     a = ap.parse_args()
     logf = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if a.make_dataset:
@@ -526,7 +639,7 @@ if __name__ == "__main__":
         sys.exit(0)
     res = run_experiment(a.scenes, a.per_scene, a.sa_steps, epochs=a.epochs, fib=a.fib, log=logf,
                          datasets=a.datasets.split(",") if a.datasets else None, chi0=a.chi0, t_ratio=a.t_ratio,
-                         net_path=a.net, searches=tuple(a.searches.split(",")))
+                         net_path=a.net, searches=tuple(a.searches.split(",")), judge=a.judge, rank_n=a.rank_n)
     txt = json.dumps(res)
     print(txt, flush=True)
     if a.out:

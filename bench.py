@@ -347,6 +347,62 @@ def contraction_office(iters: int = 20) -> dict:
                               "traces": traced}}
 
 
+def reference_run() -> dict:
+    """The reference's own phase-map run (physical_exp_nni_model.py on the
+    4321 scene: 6 x 6 AP, 3 x 84 x 75 Rx, one 160 x 200 metasurface, 4
+    reflections, the reference's rays; four traces, then three models of
+    200 Adam steps), end to end, against its published time (about 140 s on
+    an A100 including its plots, README.md:140) and, loosely, its mean
+    coverage figures (tests/golden/reference_run, written from its
+    output/0_STANDALONE files)."""
+    import numpy as np
+
+    from automs_amd import experiment as E
+    folder = str(ROOT / "tests" / "golden" / "scene_4321")
+    r = E.run_trial(folder)
+    ref = json.load(open(ROOT / "tests" / "golden" / "reference_run" / "stats.json"))
+    p = np.load(ROOT / "tests" / "golden" / "reference_run" / "params.npz")
+    ch = r["_channels"]
+    _, _, _, ms = E.trial_geometry(folder)
+    with_ref = {E.MODEL_NAMES[1]: E.coverage_with_phases(ch["noms"], [], p["ap_noms"]),
+                E.MODEL_NAMES[3]: E.coverage_with_phases(ch["ms"], [ms], p["ap_ms"], p["ms_theta0"].reshape(1, -1))}
+    return {
+        "workload": "physical_exp_nni_model.py: 4321 scene (2689 triangles), 6 x 6 AP, 3 x 84 x 75 Rx, a 160 x 200 "
+                    "metasurface on wall 1 at ms_j 0.1194, 4 reflections, 100 x 100 reference rays per trace; "
+                    "models: AP codebook, + MS fixed, + MS optimised, 200 Adam steps each (capacity loss, w 0.5)",
+        "seconds_total": r["seconds_total"], "seconds_traces": r["seconds_traces"],
+        "seconds_optimise": r["seconds_optimise"], "published_seconds": 140.0,
+        "published": "about 140 s on an NVIDIA A100, plots included (README.md:140)",
+        "speedup_vs_published": 140.0 / r["seconds_total"],
+        "mean_coverage_db": {k: v["mean_db"] for k, v in r["models"].items()},
+        "reference_mean_coverage_db": {k: ref[k]["mean_db"] for k in E.MODEL_NAMES},
+        "frac_rx_above_minus_70_db": {k: v["frac_above_floor"] for k, v in r["models"].items()},
+        "reference_frac_rx_above_minus_70_db": {k: ref[k]["frac_above_floor"] for k in E.MODEL_NAMES},
+        "reference_phases_on_these_channels_mean_db": {k: v["mean_db"] for k, v in with_ref.items()},
+        "final_result": r["final_result"],
+        "shapes": r["shapes"],
+    }
+
+
+def placement_search(sa_steps: int) -> dict:
+    """The placement search (hyper-config tuner of physical_exp_nni_main.py
+    with its trial objective): plain simulated annealing with every candidate
+    judged by a traced and optimised trial, against annealing steered by the
+    conv-net surrogate with the tracer re-scoring every accepted candidate,
+    on a room scene outside the training set (seed 1000).  The network is
+    the committed one (automs_amd/data/surrogate_room_trial.*: its dataset
+    size, training and validation are in the .json next to it)."""
+    from automs_amd import surrogate as S
+    data = ROOT / "automs_amd" / "data"
+    net = S.SurrogateNet()
+    net.load(str(data / "surrogate_room_trial.npy"))
+    r = S.search_experiment(net, sa_steps=sa_steps, search_scene=1000, rank_n=64,
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
+    meta = json.load(open(data / "surrogate_room_trial.json"))
+    r["network"] = {k: meta[k] for k in ("dataset", "training", "validation", "command")}
+    return r
+
+
 def office_full(steps: int, warmup: int, cpu: bool, profile: bool = False) -> dict:
     """The office workload at its stated size (100,046 triangles, 5
     reflections, 2^24 rays per trace): H_d [64 x 65,536], G_i [64 x 2304],
@@ -465,6 +521,9 @@ def main():
                          "1 GiB device copy that calibrates FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-office", action="store_true", help="skip the office workload at its stated size")
+    ap.add_argument("--no-reference-run", action="store_true", help="skip the reference's own phase-map run")
+    ap.add_argument("--no-placement-search", action="store_true", help="skip the surrogate placement search")
+    ap.add_argument("--sa-steps", type=int, default=2000, help="annealing steps of each placement search")
     ap.add_argument("--office-steps", type=int, default=2)
     ap.add_argument("--office-warmup", type=int, default=1)
     ap.add_argument("--pmc-summary", metavar="DIR", help="print the per-kernel table of rocprofv3 counter CSVs")
@@ -650,6 +709,14 @@ def main():
                         cpu=not args.no_cpu_baseline, profile=args.profile_pass)
         print(f"office: {r}", file=sys.stderr, flush=True)
         line["office_full"] = r
+    if not args.no_reference_run and not args.profile_pass:
+        r = reference_run()
+        print(f"reference run: {r}", file=sys.stderr, flush=True)
+        line["reference_run"] = r
+    if not args.no_placement_search and not args.profile_pass:
+        r = placement_search(args.sa_steps)
+        print(f"placement search: {r}", file=sys.stderr, flush=True)
+        line["placement_search"] = r
     if args.profile_pass:
         # every query traced in this process (timed and isolated steps), for
         # the per-query instruction counts of a counter run

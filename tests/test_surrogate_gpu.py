@@ -49,12 +49,12 @@ def test_conv_net_forward_backward_vs_oracle(gpu):
 def test_surrogate_learns_traced_gain_maps(gpu):
     from automs_amd import surrogate as S
     _flush("case: surrogate training on a small traced dataset")
-    P, Sid, Y = S.generate_dataset(2, 96, seed=5, fib_ap=1 << 16, fib_ms=1 << 16, log=_flush)
+    P, Sid, Y = S.generate_dataset(2, 96, seed=5, fib_ap=1 << 16, fib_ms=1 << 16, log=_flush, label="proxy")
     X = S.build_inputs(P, Sid)
     Y = Y.reshape(-1, 1, 64, 64)
     Xtr, Ytr, Xva, Yva = X[:160], Y[:160], X[160:], Y[160:]
     net = S.SurrogateNet(seed=0)
-    hist = S.train(net, Xtr, Ytr, epochs=40, batch=32, lr=3e-3)
+    hist = S.train(net, Xtr, Ytr, epochs=40, batch=32, lr=3e-3, scale=10.0)
     with torch.no_grad():
         pv = net.forward(Xva) * 10
     mse = float(((pv - Yva) ** 2).mean())

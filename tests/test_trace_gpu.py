@@ -127,6 +127,17 @@ def test_torch_op_scene_4321_matches_oracle(gpu):
     torch.cuda.synchronize()
     Hg = result.double().cpu().numpy()
     Hg = Hg[..., 0] + 1j * Hg[..., 1]
+    # a second call on the same scene reuses the cached scene handle
+    n_cached = len(channel._SCENE_CACHE)
+    result2 = torch.zeros_like(result)
+    torch.ops.pathtrace.torch_launch_pathtrace(
+        result2, src.to(gpu), dst.to(gpu), torch.cat([src - 0.5e-5, src + 0.5e-5], -1).to(gpu),
+        torch.cat([dst - 0.06, dst + 0.06], -1).to(gpu), tri, torch.tensor(sc.mat, device=gpu), 1, 100, 100, wl, 4,
+        100, torch.tensor(sc.mat_prop, device=gpu), torch.tensor([-1.0, 0, 0]), torch.tensor([0.0, 0, 0]), 0, 2, 0, 5,
+        0.5e-5, 0.06)
+    torch.cuda.synchronize()
+    assert len(channel._SCENE_CACHE) == n_cached
+    assert torch.allclose(result2, result, rtol=1e-6, atol=1e-12)
     w = Workload("4321", sc, ArraySpec(src.numpy(), np.array([-1, 0, 0], np.float32), 2, 0.5e-5),
                  ArraySpec(dst.numpy(), np.zeros(3, np.float32), 0, 0.06), wl, 4, 100 * 100, False)
     osc = O.OracleScene.from_scene(sc)
