@@ -120,6 +120,11 @@ def _bind_contract(L):
         "ams_pm_operand_bwd": ([vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp], C.c_int),
         "ams_pm_operand_frag": ([vp, vp, vp, vp, vp, i32, i32, i32, i32, f32, vp, vp, vp, vp], C.c_int),
         "ams_objective_bwd_frag": ([vp, i32, i32, i32, i32, f32, f32, vp, vp, vp, vp, vp], C.c_int),
+        "ams_chain_first": ([vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp], C.c_int),
+        "ams_chain_step": ([vp, vp, i32, i32, i32, vp, i64, vp, vp], C.c_int),
+        "ams_chain_bwd_scale": ([vp, i64, vp, vp, i32, i32, i32, vp, vp], C.c_int),
+        "ams_chain_bwd_ms": ([vp, i64, vp, vp, i32, vp, i32, i32, i32, vp, i32, vp, vp, vp], C.c_int),
+        "ams_chain_bwd_ap": ([vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp], C.c_int),
         "ams_adam_step": ([vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp], C.c_int),
         "ams_adam_step_dev": ([vp, vp, vp, vp, i64, f32, f32, f32, f32, vp, vp, vp, vp, i64, vp], C.c_int),
         "ams_host_bvh_build": ([vp, vp, i32, i32, vp, i32, vp, vp], C.c_int),
@@ -154,4 +159,5 @@ EXPORTED = [
     "ams_host_bvh_build", "ams_host_bvh4_build", "ams_host_plane_ids",
     "ams_conv3x3_fwd", "ams_conv3x3_bwd_weight", "ams_relu_bwd", "ams_mse_partials", "ams_mse", "ams_to_bf16",
     "ams_cgemm_noalloc_max_bytes", "ams_cgemm_frag_noalloc", "ams_valu_probe",
+    "ams_chain_first", "ams_chain_step", "ams_chain_bwd_scale", "ams_chain_bwd_ms", "ams_chain_bwd_ap",
 ]

@@ -17,6 +17,18 @@
 //   g_ap      = -Im(conj(W) gW)
 // (d u / d th = i u and d W / d th = -i W; the remainder passes the gradient
 // through unchanged, as torch.remainder's backward does).
+//
+// Chains over several metasurfaces (go.py:107-157, max_ms_reflex >= 1): the
+// surfaces' elements are concatenated (K = sum K_b, period[k] the max phase of
+// element k's surface) and T is the [K][K] block matrix of the ms-to-ms
+// channels (block (a, b) = ms{a}->ms{b}, zero on the diagonal; symmetric,
+// since ms{b}->ms{a} is the transpose of ms{a}->ms{b}).  Then
+//   P_1 = WG o u,   P_r = (P_{r-1} . T) o u,   A[:, :K] = sum_r P_r
+// (o: per row of map m, u[m]), the products P_{r-1} . T on the MFMA
+// contraction (ams_cgemm_frag), the elementwise steps and their backward here:
+//   gP_R = gA,  gQ_r = gP_r o conj(u),  gP_{r-1} = gA + gQ_r . conj(T)^T,
+//   gu[m][k] = sum_r sum_c gP_r[r][k] conj(Q_r[r][k])  (Q_1 = WG),
+//   gWG[c][k] = sum_m gP_1[r][k] conj(u[m][k]).
 
 #include <hip/hip_runtime.h>
 
@@ -75,12 +87,14 @@ __device__ __forceinline__ float2 ms_coef(float a, float th, float max_phase) {
 // grid (ceil(K / 256), Cn): a workgroup holds W[c][:] in LDS and forms
 // WG[c][k] for 256 elements k, then A for every map.  Workgroups of column
 // block 0 also write the W columns of A.
+// period (may be null): the max phase of each element k, else max_phase.
 __global__ void __launch_bounds__(kPmBlock) k_pm_operand(const float *__restrict__ th_ap,
                                                          const float *__restrict__ amp,
                                                          const float *__restrict__ th_ms,
                                                          const float *__restrict__ ms_amp,
                                                          const float2 *__restrict__ G, int Cn, int M, int K, int nm,
-                                                         float max_phase, float2 *__restrict__ A,
+                                                         float max_phase, const float *__restrict__ period,
+                                                         float2 *__restrict__ A,
                                                          float2 *__restrict__ WG, float2 *__restrict__ U) {
   extern __shared__ float2 w_row[];
   const int c = blockIdx.y;
@@ -98,8 +112,9 @@ __global__ void __launch_bounds__(kPmBlock) k_pm_operand(const float *__restrict
   for (int j = 0; j < M; j++) acc = cmac(acc, w_row[j], G[(size_t)j * K + k]);
   WG[(size_t)c * K + k] = acc;
   const float a = ms_amp[k];
+  const float pk = period ? period[k] : max_phase;
   for (int m = 0; m < nm; m++) {
-    float2 u = ms_coef(a, th_ms[(size_t)m * K + k], max_phase);
+    float2 u = ms_coef(a, th_ms[(size_t)m * K + k], pk);
     if (c == 0) U[(size_t)m * K + k] = u;
     A[(size_t)(m * Cn + c) * Kc + k] = cmul(acc, u);
   }
@@ -262,6 +277,107 @@ __global__ void __launch_bounds__(kPmApBlock) k_pm_bwd_ap(const float2 *__restri
   g_ap[(size_t)c * M + j] = -(w.x * s.y - w.y * s.x);
 }
 
+// One chain round: P[r][k] = Q[r][k] u[m][k] (r = m Cn + c), A[r][k] += P.
+// P may be null (the last round).  Thread per element.
+__global__ void __launch_bounds__(kPmBlock) k_chain_step(const float2 *__restrict__ Q, const float2 *__restrict__ U,
+                                                         int Cn, int nm, int K, float2 *__restrict__ A,
+                                                         long long lda, float2 *__restrict__ P) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)nm * Cn * K;
+  if (t >= n) return;
+  const long long r = t / K;
+  const int k = (int)(t - r * K);
+  const int m = (int)(r / Cn);
+  const float2 p = cmul(Q[t], U[(size_t)m * K + k]);
+  float2 *a = A + r * lda + k;
+  *a = make_float2(a->x + p.x, a->y + p.y);
+  if (P) P[t] = p;
+}
+
+// gP[r][k] = gA[r][k] (+ Gacc[r][k]): the gradient reaching chain round r's
+// output, gA at row stride lda, Gacc (the later rounds' part) dense.
+__device__ __forceinline__ float2 chain_gp(const float2 *__restrict__ gA, long long lda,
+                                           const float2 *__restrict__ Gacc, long long r, int K, int k) {
+  float2 g = gA[r * lda + k];
+  if (Gacc) {
+    const float2 h = Gacc[r * K + k];
+    g.x += h.x;
+    g.y += h.y;
+  }
+  return g;
+}
+
+// gQ[r][k] = gP[r][k] conj(u[m][k]), dense [R][K].
+__global__ void __launch_bounds__(kPmBlock) k_chain_bwd_scale(const float2 *__restrict__ gA, long long lda,
+                                                              const float2 *__restrict__ Gacc,
+                                                              const float2 *__restrict__ U, int Cn, int nm, int K,
+                                                              float2 *__restrict__ gQ) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)nm * Cn * K;
+  if (t >= n) return;
+  const long long r = t / K;
+  const int k = (int)(t - r * K);
+  const int m = (int)(r / Cn);
+  gQ[t] = cmulc(chain_gp(gA, lda, Gacc, r, K, k), U[(size_t)m * K + k]);
+}
+
+// Round r's share of the metasurface gradient: gu[m][k] (+)= sum_c gP[r][k]
+// conj(X[.][k]), X = Q_r (row r) or, for the first round, WG (row c).  With
+// g_ms: g_ms = Im(conj(u) gu) after the update; with gWG: gWG[c][k] = sum_m
+// gP[r][k] conj(u[m][k]).  Workgroups as k_pm_bwd_ms (group sums in order).
+__global__ void __launch_bounds__(kPmMsK *kPmMsC) k_chain_bwd_ms(const float2 *__restrict__ gA, long long lda,
+                                                                 const float2 *__restrict__ Gacc,
+                                                                 const float2 *__restrict__ X, int x_by_code,
+                                                                 const float2 *__restrict__ U, int Cn, int nm, int K,
+                                                                 float2 *__restrict__ gu, int accumulate,
+                                                                 float *__restrict__ g_ms,
+                                                                 float2 *__restrict__ gWG) {
+  __shared__ float2 red[kPmMsC][kPmMsK];
+  const int kl = threadIdx.x % kPmMsK, cg = threadIdx.x / kPmMsK;
+  const int k = blockIdx.x * kPmMsK + kl;
+  const bool ok = k < K;
+  for (int m = 0; m < nm; m++) {
+    float2 s = make_float2(0.0f, 0.0f);
+    if (ok)
+      for (int c = cg; c < Cn; c += kPmMsC) {
+        const long long r = (long long)m * Cn + c;
+        const float2 t = cmulc(chain_gp(gA, lda, Gacc, r, K, k), X[(x_by_code ? (long long)c : r) * K + k]);
+        s.x += t.x;
+        s.y += t.y;
+      }
+    red[cg][kl] = s;
+    __syncthreads();
+    if (cg == 0 && ok) {
+      float2 v = red[0][kl];
+      for (int q = 1; q < kPmMsC; q++) {
+        v.x += red[q][kl].x;
+        v.y += red[q][kl].y;
+      }
+      const size_t i = (size_t)m * K + k;
+      if (accumulate) {
+        v.x += gu[i].x;
+        v.y += gu[i].y;
+      }
+      gu[i] = v;
+      if (g_ms) {
+        const float2 u = U[i];
+        g_ms[i] = u.x * v.y - u.y * v.x;
+      }
+    }
+    __syncthreads();
+  }
+  if (!gWG || !ok) return;
+  for (int c = cg; c < Cn; c += kPmMsC) {
+    float2 s = make_float2(0.0f, 0.0f);
+    for (int m = 0; m < nm; m++) {
+      const float2 t = cmulc(chain_gp(gA, lda, Gacc, (long long)m * Cn + c, K, k), U[(size_t)m * K + k]);
+      s.x += t.x;
+      s.y += t.y;
+    }
+    gWG[(size_t)c * K + k] = s;
+  }
+}
+
 bool pm_dims_ok(int32_t Cn, int32_t M, int32_t K, int32_t nm) {
   return Cn > 0 && M > 0 && M <= kPmMaxM && K > 0 && nm > 0 && (long long)nm * Cn <= (1LL << 20) &&
          (long long)K <= (1LL << 24);
@@ -281,7 +397,7 @@ int ams_pm_operand(const float *th_ap, const float *amp, const float *th_ms, con
   }
   dim3 grid((unsigned)((K + kPmBlock - 1) / kPmBlock), (unsigned)Cn);
   k_pm_operand<<<grid, kPmBlock, (size_t)M * sizeof(float2), (hipStream_t)stream>>>(
-      th_ap, amp, th_ms, ms_amp, (const float2 *)G, Cn, M, K, nm, max_phase, (float2 *)A, (float2 *)WG,
+      th_ap, amp, th_ms, ms_amp, (const float2 *)G, Cn, M, K, nm, max_phase, nullptr, (float2 *)A, (float2 *)WG,
       (float2 *)U);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -333,6 +449,104 @@ int ams_pm_operand_bwd(const float *gA, const float *th_ap, const float *amp, co
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ams::set_error(std::string("ams_pm_operand_bwd: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_chain_first(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp,
+                    const float *period, const float *G, int32_t Cn, int32_t M, int32_t K, int32_t nm, float *A,
+                    float *WG, float *U, void *stream) {
+  if (!th_ap || !amp || !th_ms || !ms_amp || !period || !G || !A || !WG || !U || !pm_dims_ok(Cn, M, K, nm)) {
+    ams::set_error("ams_chain_first: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  dim3 grid((unsigned)((K + kPmBlock - 1) / kPmBlock), (unsigned)Cn);
+  k_pm_operand<<<grid, kPmBlock, (size_t)M * sizeof(float2), (hipStream_t)stream>>>(
+      th_ap, amp, th_ms, ms_amp, (const float2 *)G, Cn, M, K, nm, 1.0f, period, (float2 *)A, (float2 *)WG,
+      (float2 *)U);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_chain_first: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_chain_step(const float *Q, const float *U, int32_t Cn, int32_t nm, int32_t K, float *A, int64_t lda,
+                   float *P, void *stream) {
+  if (!Q || !U || !A || Cn <= 0 || nm <= 0 || K <= 0 || lda < K || (long long)nm * Cn > (1LL << 20) ||
+      (long long)K > (1LL << 24)) {
+    ams::set_error("ams_chain_step: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  const long long n = (long long)nm * Cn * K;
+  if ((n + kPmBlock - 1) / kPmBlock > 0x7fffffffLL) {
+    ams::set_error("ams_chain_step: operand too large for one launch");
+    return AMS_ERR_ARG;
+  }
+  k_chain_step<<<(unsigned)((n + kPmBlock - 1) / kPmBlock), kPmBlock, 0, (hipStream_t)stream>>>(
+      (const float2 *)Q, (const float2 *)U, Cn, nm, K, (float2 *)A, lda, (float2 *)P);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_chain_step: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_chain_bwd_scale(const float *gA, int64_t lda, const float *Gacc, const float *U, int32_t Cn, int32_t nm,
+                        int32_t K, float *gQ, void *stream) {
+  if (!gA || !U || !gQ || Cn <= 0 || nm <= 0 || K <= 0 || lda < K || (long long)nm * Cn > (1LL << 20) ||
+      (long long)K > (1LL << 24)) {
+    ams::set_error("ams_chain_bwd_scale: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  const long long n = (long long)nm * Cn * K;
+  if ((n + kPmBlock - 1) / kPmBlock > 0x7fffffffLL) {
+    ams::set_error("ams_chain_bwd_scale: operand too large for one launch");
+    return AMS_ERR_ARG;
+  }
+  k_chain_bwd_scale<<<(unsigned)((n + kPmBlock - 1) / kPmBlock), kPmBlock, 0, (hipStream_t)stream>>>(
+      (const float2 *)gA, lda, (const float2 *)Gacc, (const float2 *)U, Cn, nm, K, (float2 *)gQ);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_chain_bwd_scale: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_chain_bwd_ms(const float *gA, int64_t lda, const float *Gacc, const float *X, int32_t x_by_code,
+                     const float *U, int32_t Cn, int32_t nm, int32_t K, float *gu, int32_t accumulate, float *g_ms,
+                     float *gWG, void *stream) {
+  if (!gA || !X || !U || !gu || Cn <= 0 || nm <= 0 || K <= 0 || lda < K || (long long)nm * Cn > (1LL << 20) ||
+      (long long)K > (1LL << 24)) {
+    ams::set_error("ams_chain_bwd_ms: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  k_chain_bwd_ms<<<(unsigned)((K + kPmMsK - 1) / kPmMsK), kPmMsK * kPmMsC, 0, (hipStream_t)stream>>>(
+      (const float2 *)gA, lda, (const float2 *)Gacc, (const float2 *)X, x_by_code, (const float2 *)U, Cn, nm, K,
+      (float2 *)gu, accumulate, g_ms, (float2 *)gWG);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_chain_bwd_ms: ") + hipGetErrorString(e));
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_chain_bwd_ap(const float *gA, const float *gWG, const float *G, const float *th_ap, const float *amp,
+                     int32_t Cn, int32_t M, int32_t K, int32_t nm, float *g_ap, void *stream) {
+  if (!gA || !gWG || !G || !th_ap || !amp || !g_ap || !pm_dims_ok(Cn, M, K, nm)) {
+    ams::set_error("ams_chain_bwd_ap: bad arguments");
+    return AMS_ERR_ARG;
+  }
+  k_pm_bwd_ap<<<dim3((unsigned)M, (unsigned)Cn), kPmApBlock, 0, (hipStream_t)stream>>>(
+      (const float2 *)gA, (const float2 *)gWG, (const float2 *)G, th_ap, amp, Cn, M, K, nm, g_ap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ams::set_error(std::string("ams_chain_bwd_ap: ") + hipGetErrorString(e));
     return AMS_ERR_HIP;
   }
   return AMS_OK;

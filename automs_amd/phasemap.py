@@ -181,6 +181,110 @@ def _operand_grads(gA: torch.Tensor, th_ap, amp, G, WG, U, dims, need_ap: bool):
     return g_ap, g_ms
 
 
+class ChainTransfer:
+    """The ms-to-ms channels of several metasurfaces as one [K][K] block
+    matrix T (K = sum K_b; block (a, b) = ms{a}->ms{b}, block (b, a) its
+    transpose, as go.py:147-153 uses it; zero diagonal blocks), packed once in
+    the fragment layout.  T is symmetric, so the one packing serves the
+    chain products of the forward (P . T) and of the backward
+    (gQ . conj(T)^T)."""
+
+    def __init__(self, T: Dict[tuple, torch.Tensor], K_list: Sequence[int], dev):
+        off = [0]
+        for K in K_list:
+            off.append(off[-1] + int(K))
+        self.K = off[-1]
+        Tf = torch.zeros(self.K, self.K, dtype=torch.complex64, device=dev)
+        for (a, b), t in T.items():
+            if a < b:
+                Tf[off[a]:off[a + 1], off[b]:off[b + 1]] = t
+                Tf[off[b]:off[b + 1], off[a]:off[a + 1]] = t.T
+        self.frag = frag_pack(Tf, self.K, self.K, self.K, 1)
+
+
+class ChainOperand(torch.autograd.Function):
+    """The stacked operand A [n_maps * C, K + M] of several metasurfaces with
+    chains up to `rounds` surfaces long (go.py:107-157): the first round
+    (ams_chain_first), each later round's product with the ms-to-ms block
+    matrix on the MFMA contraction and its elementwise step
+    (ams_chain_step); the backward runs the rounds in reverse
+    (ams_chain_bwd_ms, ams_chain_bwd_scale, the conjugate product) and ends
+    with the AP phase gradient (ams_chain_bwd_ap).  The metasurfaces'
+    elements are concatenated in order."""
+
+    @staticmethod
+    def forward(ctx, th_ap: torch.Tensor, amp: torch.Tensor, ms_amp: torch.Tensor, G: torch.Tensor,
+                period: torch.Tensor, chain: Optional[ChainTransfer], rounds: int, *th_ms: torch.Tensor):
+        L = _lib.lib()
+        Cn, M = th_ap.shape
+        th = torch.cat([t.detach() for t in th_ms], 1).contiguous()
+        nm, K = th.shape
+        R, Kc = nm * Cn, K + M
+        dev = th_ap.device
+        th_ap = th_ap.contiguous()
+        A = torch.empty(R, Kc, dtype=torch.complex64, device=dev)
+        WG = torch.empty(Cn, K, dtype=torch.complex64, device=dev)
+        U = torch.empty(nm, K, dtype=torch.complex64, device=dev)
+        _lib.check(L.ams_chain_first(_ptr(th_ap), _ptr(amp), _ptr(th), _ptr(ms_amp), _ptr(period),
+                                     _ptr(torch.view_as_real(G)), Cn, M, K, nm, _ptr(A), _ptr(WG), _ptr(U),
+                                     _stream()), "ams_chain_first")
+        Qs = []
+        if chain is not None:
+            prev, ld = A, Kc
+            for r in range(2, rounds + 1):
+                fp = frag_pack(prev, R, K, ld, 1)
+                Q = cgemm_frag(fp, chain.frag, R, K, K, conj_b=False, splits=split_count(R, K, K))
+                P = torch.empty(R, K, dtype=torch.complex64, device=dev) if r < rounds else None
+                _lib.check(L.ams_chain_step(_ptr(Q), _ptr(U), Cn, nm, K, _ptr(A), Kc,
+                                            _ptr(P) if P is not None else None, _stream()), "ams_chain_step")
+                Qs.append(Q)
+                prev, ld = P, K
+        ctx.save_for_backward(th_ap, amp, G, WG, U, *Qs)
+        ctx.chain = chain
+        ctx.dims = (Cn, M, K, nm, [int(t.shape[1]) for t in th_ms])
+        return A
+
+    @staticmethod
+    def backward(ctx, gA: torch.Tensor):
+        L = _lib.lib()
+        th_ap, amp, G, WG, U, *Qs = ctx.saved_tensors
+        Cn, M, K, nm, Ks = ctx.dims
+        R, Kc = nm * Cn, K + M
+        gA = gA.contiguous()
+        dev = gA.device
+        need_ap = ctx.needs_input_grad[0]
+        n_ap = Cn * M if need_ap else 0
+        gbuf = torch.empty(n_ap + nm * K, dtype=torch.float32, device=dev)
+        g_ms = gbuf[n_ap:].view(nm, K)
+        gu = torch.empty(nm, K, dtype=torch.complex64, device=dev)
+        Gacc = None
+        first = True
+        for Q in reversed(Qs):
+            _lib.check(L.ams_chain_bwd_ms(_ptr(gA), Kc, _ptr(Gacc) if Gacc is not None else None, _ptr(Q), 0,
+                                          _ptr(U), Cn, nm, K, _ptr(gu), 0 if first else 1, None, None, _stream()),
+                       "ams_chain_bwd_ms")
+            first = False
+            gQ = torch.empty(R, K, dtype=torch.complex64, device=dev)
+            _lib.check(L.ams_chain_bwd_scale(_ptr(gA), Kc, _ptr(Gacc) if Gacc is not None else None, _ptr(U),
+                                             Cn, nm, K, _ptr(gQ), _stream()), "ams_chain_bwd_scale")
+            fq = frag_pack(gQ, R, K, K, 1)
+            Gacc = cgemm_frag(fq, ctx.chain.frag, R, K, K, conj_b=True, splits=split_count(R, K, K))
+        gWG = torch.empty(Cn, K, dtype=torch.complex64, device=dev) if need_ap else None
+        _lib.check(L.ams_chain_bwd_ms(_ptr(gA), Kc, _ptr(Gacc) if Gacc is not None else None, _ptr(WG), 1,
+                                      _ptr(U), Cn, nm, K, _ptr(gu), 0 if first else 1, _ptr(g_ms),
+                                      _ptr(gWG) if need_ap else None, _stream()), "ams_chain_bwd_ms")
+        g_ap = None
+        if need_ap:
+            g_ap = gbuf[:n_ap].view(Cn, M)
+            _lib.check(L.ams_chain_bwd_ap(_ptr(gA), _ptr(gWG), _ptr(torch.view_as_real(G)), _ptr(th_ap), _ptr(amp),
+                                          Cn, M, K, nm, _ptr(g_ap), _stream()), "ams_chain_bwd_ap")
+        outs, off = [], 0
+        for k in Ks:
+            outs.append(g_ms[:, off:off + k] if nm > 1 else g_ms[:, off:off + k].reshape(1, k))
+            off += k
+        return (g_ap,) + (None,) * 6 + tuple(outs)
+
+
 def _row_mask(rowmask: Optional[torch.Tensor], groups: int, C: int, dev) -> Optional[torch.Tensor]:
     """The capacity objective's row masks as u8 [groups * C] on the device:
     group g masks its rows rowmask[g * C + c]."""
@@ -383,8 +487,9 @@ class PhaseMapModel(nn.Module):
             self.ms_theta.append(nn.Parameter(t0.to(dev, torch.float32).reshape(n_maps, K).clone()))
         self.G_list = [channels["src_to_ms{%d}" % i].to(torch.complex64).contiguous() for i in range(len(ms_list))]
         # one metasurface: every chain has length 1, and the operand comes from
-        # the fused kernels (PhaseOperand); otherwise from torch ops
-        self.fused_operand = fused and not plain and len(ms_list) == 1
+        # the fused kernels (PhaseOperand); several: from the chain kernels
+        # (ChainOperand); the plain comparison path forms it with torch ops
+        self.fused_operand = fused and not plain and len(ms_list) == 1 and max_ms_reflex >= 1
         # the whole step as one Function over fragment-writing kernels (up to
         # 128 AP elements); otherwise the operand, the contraction and the
         # objective as separate Functions
@@ -397,6 +502,14 @@ class PhaseMapModel(nn.Module):
                 if a < b and ("ms{%d}_to_ms{%d}" % (a, b)) in channels:
                     self.T[(a, b)] = channels["ms{%d}_to_ms{%d}" % (a, b)].to(torch.complex64)
                     self.T[(b, a)] = self.T[(a, b)].T
+        self.chain_operand = fused and not plain and len(ms_list) > 1 and max_ms_reflex >= 1
+        if self.chain_operand:
+            self.G_cat = torch.cat(self.G_list, 1).contiguous()
+            self.ms_amp_cat = torch.cat(self.ms_A).contiguous()
+            self.period = torch.cat([torch.full((int(g.shape[1]),), ms.max_phase_rad, dtype=torch.float32)
+                                     for g, ms in zip(self.G_list, ms_list)]).to(dev)
+            self.chain = ChainTransfer(self.T, [int(g.shape[1]) for g in self.G_list], dev) \
+                if (self.T and max_ms_reflex > 1) else None
         F_list = [channels["ms{%d}_to_dst" % i] for i in range(len(ms_list))]
         self.op = ChannelOperand(F_list, channels["src_to_dst"], keep_plain=plain)
         self.loss_list: List[float] = []
@@ -409,7 +522,7 @@ class PhaseMapModel(nn.Module):
         A_b = sum_{r <= max_ms_reflex} Pr_b; so y = A . [F ; H_d]."""
         W = self.antenna_A * torch.exp(-1j * self.antenna_theta)  # go.py:163
         rows = []
-        n = len(self.ms_list)
+        n = len(self.ms_list) if self.max_ms_reflex >= 1 else 0
         for m in range(self.n_maps):
             u = []
             for i, ms in enumerate(self.ms_list):
@@ -422,7 +535,9 @@ class PhaseMapModel(nn.Module):
                      if any(a != b and (a, b) in self.T for a in range(n)) else torch.zeros_like(P[b])
                      for b in range(n)]
                 acc = [acc[b] + P[b] for b in range(n)]
-            rows.append(torch.cat(acc + [W.to(torch.complex64)], 1))
+            zeros = [torch.zeros(W.shape[0], g.shape[1], dtype=torch.complex64, device=W.device)
+                     for g in self.G_list[n:]]  # max_ms_reflex 0: no chain reaches a metasurface
+            rows.append(torch.cat(acc + zeros + [W.to(torch.complex64)], 1))
         return torch.cat(rows, 0)
 
     @torch.no_grad()
@@ -438,6 +553,9 @@ class PhaseMapModel(nn.Module):
         if self.fused_operand:
             A = PhaseOperand.apply(self.antenna_theta, self.ms_theta[0], self.antenna_A, self.ms_A[0],
                                    self.G_list[0], self.ms_list[0].max_phase_rad)
+        elif self.chain_operand:
+            A = ChainOperand.apply(self.antenna_theta, self.antenna_A, self.ms_amp_cat, self.G_cat, self.period,
+                                   self.chain, self.max_ms_reflex, *self.ms_theta)
         else:
             A = self.stacked_operand()
         fn = ContractPlain if self.plain else Contract

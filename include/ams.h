@@ -216,6 +216,39 @@ int ams_pm_operand_frag(const float *th_ap, const float *amp, const float *th_ms
 int ams_pm_operand_bwd(const float *gA, const float *th_ap, const float *amp, const float *G, const float *WG,
                        const float *U, int32_t Cn, int32_t M, int32_t K, int32_t nm, float *g_ms, float *g_ap,
                        float *gWG, void *stream);
+/* Metasurface chains over several surfaces (replaces the chain loop of
+ * go.py:107-157, max_ms_reflex >= 1).  The surfaces' elements are
+ * concatenated: K = sum of their element counts, th_ms [nm][K], ms_amp [K],
+ * period [K] (the max phase of each element's surface), G [M][K] (the
+ * src_to_ms channels side by side).  Rows r = m*Cn + c as above.
+ * ams_chain_first: the first round, A [nm*Cn][K+M] = [WG o u | W] (the same
+ * arithmetic as ams_pm_operand), WG [Cn][K], U [nm][K].
+ * ams_chain_step: one later round after the caller's product Q = P . T
+ * (ams_cgemm_frag against the [K][K] ms-to-ms block matrix T, zero diagonal
+ * blocks): P = Q o u (P may be null in the last round), A[:, :K] += P (A at
+ * row stride lda complex elements). */
+int ams_chain_first(const float *th_ap, const float *amp, const float *th_ms, const float *ms_amp,
+                    const float *period, const float *G, int32_t Cn, int32_t M, int32_t K, int32_t nm, float *A,
+                    float *WG, float *U, void *stream);
+int ams_chain_step(const float *Q, const float *U, int32_t Cn, int32_t nm, int32_t K, float *A, int64_t lda,
+                   float *P, void *stream);
+/* Backward of the chains.  gA [nm*Cn] rows at stride lda (torch's gradient
+ * convention for complex tensors); Gacc [nm*Cn][K] (may be null) the part of
+ * round r's output gradient that comes from the later rounds, so that
+ * gP = gA + Gacc.  ams_chain_bwd_scale: gQ = gP o conj(u), the operand of the
+ * next product gP_{r-1} - gA = gQ . conj(T)^T.  ams_chain_bwd_ms: the round's
+ * share of the metasurface gradient, gu [nm][K] (+)= sum_c gP conj(X), X = Q_r
+ * [nm*Cn][K] (x_by_code 0) or, in the first round, WG [Cn][K] (x_by_code 1);
+ * g_ms (may be null) = Im(conj(u) gu) [nm][K] after the update, gWG (may be
+ * null) [Cn][K] = sum_m gP conj(u).  ams_chain_bwd_ap: g_ap [Cn][M] from gWG
+ * and gA's direct columns K..K+M-1 (lda = K + M), as ams_pm_operand_bwd. */
+int ams_chain_bwd_scale(const float *gA, int64_t lda, const float *Gacc, const float *U, int32_t Cn, int32_t nm,
+                        int32_t K, float *gQ, void *stream);
+int ams_chain_bwd_ms(const float *gA, int64_t lda, const float *Gacc, const float *X, int32_t x_by_code,
+                     const float *U, int32_t Cn, int32_t nm, int32_t K, float *gu, int32_t accumulate, float *g_ms,
+                     float *gWG, void *stream);
+int ams_chain_bwd_ap(const float *gA, const float *gWG, const float *G, const float *th_ap, const float *amp,
+                     int32_t Cn, int32_t M, int32_t K, int32_t nm, float *g_ap, void *stream);
 /* torch.optim.Adam step (no weight decay), step counted from 1 */
 int ams_adam_step(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
                   float beta2, float eps, int32_t step, void *stream);

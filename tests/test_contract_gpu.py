@@ -643,6 +643,82 @@ def test_metasurface_cascades_vs_reference_enumeration(gpu, reflex):
     assert rel < 3e-5
 
 
+def _three_surfaces(gpu, Ks=(40, 56, 48), M=16, N=300, pairs=((0, 1), (0, 2), (1, 2))):
+    ch64 = {"src_to_dst": _crand(M, N, scale=3e-5, seed=30)}
+    for i, K in enumerate(Ks):
+        ch64["src_to_ms{%d}" % i] = _crand(M, K, scale=3e-4, seed=31 + i)
+        ch64["ms{%d}_to_dst" % i] = _crand(K, N, scale=3e-4, seed=41 + i)
+    for a, b in pairs:
+        ch64["ms{%d}_to_ms{%d}" % (a, b)] = _crand(Ks[a], Ks[b], scale=1e-2, seed=50 + 3 * a + b)
+    return ch64, {k: v.to(torch.complex64).to(gpu) for k, v in ch64.items()}
+
+
+@pytest.mark.parametrize("reflex,n_maps,objective", [(2, 1, "logsnr"), (3, 2, "logsnr"), (2, 1, "capacity"),
+                                                     (1, 2, "logsnr")])
+def test_chain_kernels_gradients_vs_oracle_autograd(gpu, reflex, n_maps, objective):
+    """Three metasurfaces on the chain kernels (ChainOperand): the loss and
+    the gradients of every phase against complex128 autograd through go.py's
+    literal chain enumeration (received_chains) and the oracle's objective,
+    with different max phases per surface and one ms-to-ms pair missing."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    from oracle import contract_ref as R
+    _flush(f"case: chain kernels, max_ms_reflex={reflex}, {n_maps} maps, {objective}")
+    Ks = (40, 56, 48)
+    ch64, ch = _three_surfaces(gpu, Ks, pairs=((0, 1), (1, 2)))
+    ch64["ms{0}_to_ms{2}"] = torch.zeros(Ks[0], Ks[2], dtype=torch.complex128)  # absent from the model's channels
+    mp = (300.0, 360.0, 250.0)
+    ms = [Metasurface(None, (1, K), None, None, [], max_phase=p) for K, p in zip(Ks, mp)]
+    g = torch.Generator().manual_seed(12)
+    th = [(torch.rand(n_maps, K, generator=g) - 0.2) * 2.5 * math.pi for K in Ks]
+    C = 6
+    model = PhaseMapModel("c", ch, ms, code_num=C, ms_theta=th, max_ms_reflex=reflex, n_maps=n_maps,
+                          objective=objective, w=0.5)
+    assert model.chain_operand
+    mask = torch.zeros(C, dtype=torch.bool)
+    mask[[1, 4]] = True
+    model.rowmask_fn = lambda: mask.repeat(n_maps)
+    loss = model()
+    loss.backward()
+    th_ap = model.antenna_theta.detach().cpu().double().requires_grad_(True)
+    th64 = [t.double().requires_grad_(True) for t in th]
+    lo = 0.0
+    for m in range(n_maps):
+        Y = R.received_chains(model.antenna_A.cpu(), th_ap, ch64, [t[m] for t in th64],
+                              [p / 180 * math.pi for p in mp], reflex)
+        lo = lo + (R.loss_logsnr(Y, 1, model.n0) if objective == "logsnr" else
+                   R.loss_capacity(Y, model.n0, 0.5, mask))
+    lo.backward()
+    rel_l = abs(loss.item() - lo.item()) / abs(lo.item())
+    ga = model.antenna_theta.grad.cpu().double()
+    rel_ap = ((ga - th_ap.grad).norm() / th_ap.grad.norm()).item()
+    rel_ms = [((model.ms_theta[i].grad.cpu().double() - th64[i].grad).norm() / th64[i].grad.norm()).item()
+              for i in range(3)]
+    _flush(f"  loss {loss.item():.6f} oracle {lo.item():.6f} rel {rel_l:.1e}; grad rel AP {rel_ap:.1e}, "
+           f"MS {', '.join(f'{r:.1e}' for r in rel_ms)}")
+    assert rel_l < 1e-5
+    assert rel_ap < 1e-4
+    assert max(rel_ms) < 1e-4
+
+
+def test_chain_kernels_operand_equals_torch_restatement(gpu):
+    """The chain kernels' stacked operand equals the torch-op restatement
+    of the same chains (PhaseMapModel.stacked_operand) to FP32 rounding."""
+    from automs_amd.phasemap import ChainOperand, Metasurface, PhaseMapModel
+    _flush("case: chain operand vs torch restatement")
+    Ks = (40, 56, 48)
+    _, ch = _three_surfaces(gpu, Ks)
+    ms = [Metasurface(None, (1, K), None, None, [], max_phase=300.0) for K in Ks]
+    g = torch.Generator().manual_seed(13)
+    th = [torch.rand(3, K, generator=g) * 2 * math.pi for K in Ks]
+    model = PhaseMapModel("c", ch, ms, code_num=5, ms_theta=th, max_ms_reflex=3, n_maps=3)
+    A = ChainOperand.apply(model.antenna_theta, model.antenna_A, model.ms_amp_cat, model.G_cat, model.period,
+                           model.chain, 3, *model.ms_theta).detach()
+    B = model.stacked_operand().detach()
+    rel = ((A - B).abs().max() / B.abs().max()).item()
+    _flush(f"  max |A - A_torch| / max |A_torch| = {rel:.1e}")
+    assert rel < 1e-4
+
+
 def test_postprocess_quantises_and_matches_oracle(gpu):
     """postprocess (go.py:89-97): AP phases become the nearest multiples of
     pi/2, metasurface phases are wrapped to [0, max_phase); the loss it
