@@ -403,6 +403,50 @@ def placement_search(sa_steps: int) -> dict:
     return r
 
 
+def office_phase_map(outs: dict, steps: int = 200) -> dict:
+    """The phase-map optimisation of config 2's three metasurfaces on the
+    traced office channels (go.py's Model with max_ms_reflex 2, its default):
+    30 codes of the 8 x 8 AP, chains over the three 48 x 48 surfaces through
+    the ms-to-ms channels, mean log-SNR over the 65,536 Rx, Adam lr 0.3, the
+    step replayed from a HIP graph; against the codebook optimised alone."""
+    import torch
+
+    from automs_amd import scenes
+    from automs_amd.phasemap import Metasurface, PhaseMapModel, optimize_model
+    ch = {scenes.OFFICE_CHANNEL_KEYS[k]: torch.complex(v[..., 0], v[..., 1]) for k, v in outs.items()}
+    ms = [Metasurface(None, (48, 48), None, None, [], max_phase=360.0) for _ in range(3)]
+    res = {}
+    for name, chans, surfaces in (("codebook_alone", {"src_to_dst": ch["src_to_dst"]}, []),
+                                  ("three_surfaces", ch, ms)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        model = PhaseMapModel(name, chans, surfaces, code_num=30, objective="logsnr", max_ms_reflex=2, seed=0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        cov0 = float(model.coverage_db()[0])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        optimize_model(model, lr=0.3, step=steps, graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        with torch.no_grad():
+            model()
+        res[name] = {"setup_seconds": t1 - t0, "seconds": e0.elapsed_time(e1) * 1e-3,
+                     "ms_per_step": e0.elapsed_time(e1) / steps, "coverage_db_start": cov0,
+                     "coverage_db_end": float(model.coverage_db()[0]), "loss_end": float(model.loss.item()),
+                     "best_loss": float(min(model.loss_list))}
+        if surfaces:
+            res[name]["operand_gb"] = model.op.nbytes() / 1e9
+            res[name]["chain_transfer_gb"] = model.chain.frag.numel() / 1e9
+        del model
+        torch.cuda.empty_cache()
+    res["coverage_gain_db"] = res["three_surfaces"]["coverage_db_end"] - res["codebook_alone"]["coverage_db_end"]
+    res["workload"] = ("config 2's traced office channels: 8 x 8 AP, 30 codes, three 48 x 48 metasurfaces, chains "
+                       "of up to 2 surfaces, 65,536 Rx; K = 3 x 2304 + 64, the split-bf16 operand in both layouts")
+    res["steps"] = steps
+    return res
+
+
 def office_full(steps: int, warmup: int, cpu: bool, profile: bool = False) -> dict:
     """The office workload at its stated size (100,046 triangles, 5
     reflections, 2^24 rays per trace): H_d [64 x 65,536], G_i [64 x 2304],
@@ -485,6 +529,11 @@ def office_full(steps: int, warmup: int, cpu: bool, profile: bool = False) -> di
         "totals": {f: tot[f] for f in ("fwd_queries", "bwd_queries", "fwd_nodes", "bwd_nodes",
                                        "n_discover_launches", "n_evaluate_launches")},
     }
+    if not profile:
+        for k in handles:
+            handles[k].close()
+        out["phase_map"] = office_phase_map(outs)
+        print(f"office phase map: {out['phase_map']}", file=sys.stderr, flush=True)
     if cpu:
         from oracle import oracle as O
         w = scenes.office(fib_n=1 << 10, ms_fib_n=1 << 12)["H_d"]
