@@ -196,22 +196,39 @@ __global__ void __launch_bounds__(256) k_cgemm(const u32x4 *__restrict__ A, cons
 // The next step's blocks are loaded into registers while the current step
 // computes.  Workgroups are numbered so that the row groups of one column
 // group run on the same XCD (blockIdx.x mod 8) and share B through its L2.
+// With split_n > 0 (a reduction split split_n ways, one-dimensional grid)
+// each XCD instead takes whole splits, z = xcd, xcd + 8, ..., with every
+// (row group, column group) of a split in turn: the split's slice of A
+// (e.g. gY in the backward, read by every column group) then crosses HBM
+// once, into that XCD's L2.
 constexpr int kWgA = 8, kWgB = 4, kWgBlocks = kWgA + kWgB;
 
 template <bool CONJB>
 __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, const u32x4 *__restrict__ B, int Ra,
-                                                  int Rb, int n_kb, int row_groups, int col_groups,
+                                                  int Rb, int n_kb, int row_groups, int col_groups, int split_n,
                                                   float *__restrict__ C, long long ldc, float *__restrict__ part) {
   __shared__ u32x4 lds[kWgBlocks * 256];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int xcd = (int)(blockIdx.x & 7), s = (int)(blockIdx.x >> 3);
-  const int rg = s % row_groups;
-  const int cg = (s / row_groups) * 8 + xcd;
-  if (cg >= col_groups) return;  // uniform over the workgroup, before any barrier
+  int rg, cg, z, splits;
+  if (split_n > 0) {
+    const int per = row_groups * col_groups;
+    const int zz = s / per, rem = s - zz * per;
+    z = zz * 8 + xcd;
+    rg = rem % row_groups;
+    cg = rem / row_groups;
+    splits = split_n;
+    if (z >= splits) return;  // uniform over the workgroup, before any barrier
+  } else {
+    rg = s % row_groups;
+    cg = (s / row_groups) * 8 + xcd;
+    z = (int)blockIdx.z;
+    splits = (int)gridDim.z;
+    if (cg >= col_groups) return;  // uniform over the workgroup, before any barrier
+  }
   const int ra_blocks = (Ra + 15) >> 4, rb_blocks = (Rb + 15) >> 4;
-  const int splits = gridDim.z;
-  const int kb0 = (int)((long long)n_kb * blockIdx.z / splits);
-  const int kb1 = (int)((long long)n_kb * (blockIdx.z + 1) / splits);
+  const int kb0 = (int)((long long)n_kb * z / splits);
+  const int kb1 = (int)((long long)n_kb * (z + 1) / splits);
   long long base[kWgBlocks];  // first fragment (kb = 0) of each staged block; clamped blocks are never stored
 #pragma unroll
   for (int q = 0; q < kWgA; q++) {
@@ -291,7 +308,7 @@ __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, c
       }
     }
   }
-  float *out = splits > 1 ? part + (long long)blockIdx.z * 2 * (long long)Ra * Rb : C;
+  float *out = splits > 1 ? part + (long long)z * 2 * (long long)Ra * Rb : C;
   long long ld = splits > 1 ? Rb : ldc;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -303,6 +320,166 @@ __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, c
         int row = (rg * kWgA + wr * 4 + i) * 16 + 4 * (lane >> 4) + v;
         if (row < Ra && col < Rb) {
           float2 val = make_float2(acc_re[i][j][v], acc_im[i][j][v]);
+          *reinterpret_cast<float2 *>(out + 2 * ((long long)row * ld + col)) = val;
+        }
+      }
+    }
+  }
+}
+
+// Three-multiplication workgroup kernel (conj_b bit 3 of ams_cgemm_frag): the
+// complex product of a tile pair as three real products,
+//   P1 = ar br,  P2 = ai bi,  P3 = (ar + ai)(br +/- bi),
+//   re = P1 -/+ P2,  im = P3 - P1 -/+ P2  (lower signs: A . conj(B)^T),
+// each split-bf16 (hi.hi + hi.lo + lo.hi): 9 MFMAs per tile pair instead of
+// 12.  The sum planes come from the operands' own split values, formed once
+// per workgroup while a step is staged: (hi + lo) of the real and imaginary
+// part are exact in FP32, their sum is rounded once and split again
+// (relative error ~1e-5 per element, as the stored planes).  LDS holds six
+// planes per block (72 KB per workgroup); the mapping of workgroups to
+// tiles and XCDs is k_cgemm_wg's.
+__device__ __forceinline__ float bf_lo(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+
+// the sum planes (hi, lo) of one lane's 8 elements: s = (rh + rl) + sgn (ih + il)
+__device__ __forceinline__ void sum_planes(u32x4 rh, u32x4 rl, u32x4 ih, u32x4 il, float sgn, u32x4 &sh,
+                                           u32x4 &sl) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    unsigned h2 = 0, l2 = 0;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      const float r = half ? bf_hi(rh[q]) + bf_hi(rl[q]) : bf_lo(rh[q]) + bf_lo(rl[q]);
+      const float i = half ? bf_hi(ih[q]) + bf_hi(il[q]) : bf_lo(ih[q]) + bf_lo(il[q]);
+      const float v = fmaf(sgn, i, r);
+      const unsigned short h = ams::bf16_rne(v);
+      const unsigned short l = ams::bf16_rne(v - ams::bf16_to_f(h));
+      h2 |= (unsigned)h << (16 * half);
+      l2 |= (unsigned)l << (16 * half);
+    }
+    sh[q] = h2;
+    sl[q] = l2;
+  }
+}
+
+constexpr int kWg3Planes = 6, kWg3Items = 3;  // (block, lane) items staged per thread and step
+
+template <bool CONJB>
+__global__ void __launch_bounds__(256) k_cgemm_wg3(const u32x4 *__restrict__ A, const u32x4 *__restrict__ B,
+                                                   int Ra, int Rb, int n_kb, int row_groups, int col_groups,
+                                                   int split_n, float *__restrict__ C, long long ldc,
+                                                   float *__restrict__ part) {
+  __shared__ u32x4 lds[kWgBlocks * kWg3Planes * 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int xcd = (int)(blockIdx.x & 7), s = (int)(blockIdx.x >> 3);
+  int rg, cg, z, splits;
+  if (split_n > 0) {
+    const int per = row_groups * col_groups;
+    const int zz = s / per, rem = s - zz * per;
+    z = zz * 8 + xcd;
+    rg = rem % row_groups;
+    cg = rem / row_groups;
+    splits = split_n;
+    if (z >= splits) return;  // uniform over the workgroup, before any barrier
+  } else {
+    rg = s % row_groups;
+    cg = (s / row_groups) * 8 + xcd;
+    z = (int)blockIdx.z;
+    splits = (int)gridDim.z;
+    if (cg >= col_groups) return;  // uniform over the workgroup, before any barrier
+  }
+  const int ra_blocks = (Ra + 15) >> 4, rb_blocks = (Rb + 15) >> 4;
+  const int kb0 = (int)((long long)n_kb * z / splits);
+  const int kb1 = (int)((long long)n_kb * (z + 1) / splits);
+  // item t = tid + 256 j: block t >> 6 (A blocks 0..7, then B blocks 0..3), lane t & 63
+  long long base[kWg3Items];
+#pragma unroll
+  for (int j = 0; j < kWg3Items; j++) {
+    const int blk = (tid + 256 * j) >> 6;
+    if (blk < kWgA) {
+      const int rb = rg * kWgA + blk;
+      base[j] = (long long)(rb < ra_blocks ? rb : ra_blocks - 1) * n_kb * 256 + lane;
+    } else {
+      const int cb = cg * kWgB + (blk - kWgA);
+      base[j] = (long long)(cb < rb_blocks ? cb : rb_blocks - 1) * n_kb * 256 + lane;
+    }
+  }
+  u32x4 pf[kWg3Items][4];
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int j = 0; j < kWg3Items; j++) {
+      const u32x4 *src = (j < 2 ? A : B) + base[j] + (long long)kb * 256;
+#pragma unroll
+      for (int q = 0; q < 4; q++) pf[j][q] = src[q * 64];
+    }
+  };
+  if (kb0 < kb1) fetch(kb0);
+  f32x4 acc1[4][2], acc2[4][2], acc3[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc1[i][j] = (f32x4){0, 0, 0, 0};
+      acc2[i][j] = (f32x4){0, 0, 0, 0};
+      acc3[i][j] = (f32x4){0, 0, 0, 0};
+    }
+  const int wr = w & 1, wc = w >> 1;
+  for (int kb = kb0; kb < kb1; kb++) {
+    __syncthreads();  // every wave is done reading the previous step
+#pragma unroll
+    for (int j = 0; j < kWg3Items; j++) {
+      const int blk = (tid + 256 * j) >> 6;
+      u32x4 *d = lds + blk * kWg3Planes * 64 + lane;
+#pragma unroll
+      for (int q = 0; q < 4; q++) d[q * 64] = pf[j][q];
+      u32x4 sh, sl;
+      sum_planes(pf[j][0], pf[j][1], pf[j][2], pf[j][3], (j == 2 && CONJB) ? -1.0f : 1.0f, sh, sl);
+      d[4 * 64] = sh;
+      d[5 * 64] = sl;
+    }
+    if (kb + 1 < kb1) fetch(kb + 1);
+    __syncthreads();
+    u32x4 b[2][kWg3Planes];
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < kWg3Planes; q++) b[j][q] = lds[((kWgA + wc * 2 + j) * kWg3Planes + q) * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      u32x4 a[kWg3Planes];
+#pragma unroll
+      for (int q = 0; q < kWg3Planes; q++) a[q] = lds[((wr * 4 + i) * kWg3Planes + q) * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        f32x4 p1 = acc1[i][j], p2 = acc2[i][j], p3 = acc3[i][j];
+        p1 = MFMA(as_bf(a[0]), as_bf(b[j][0]), p1);
+        p1 = MFMA(as_bf(a[0]), as_bf(b[j][1]), p1);
+        p1 = MFMA(as_bf(a[1]), as_bf(b[j][0]), p1);
+        p2 = MFMA(as_bf(a[2]), as_bf(b[j][2]), p2);
+        p2 = MFMA(as_bf(a[2]), as_bf(b[j][3]), p2);
+        p2 = MFMA(as_bf(a[3]), as_bf(b[j][2]), p2);
+        p3 = MFMA(as_bf(a[4]), as_bf(b[j][4]), p3);
+        p3 = MFMA(as_bf(a[4]), as_bf(b[j][5]), p3);
+        p3 = MFMA(as_bf(a[5]), as_bf(b[j][4]), p3);
+        acc1[i][j] = p1;
+        acc2[i][j] = p2;
+        acc3[i][j] = p3;
+      }
+    }
+  }
+  float *out = splits > 1 ? part + (long long)z * 2 * (long long)Ra * Rb : C;
+  long long ld = splits > 1 ? Rb : ldc;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      int col = (cg * kWgB + wc * 2 + j) * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row = (rg * kWgA + wr * 4 + i) * 16 + 4 * (lane >> 4) + v;
+        if (row < Ra && col < Rb) {
+          const float p1 = acc1[i][j][v], p2 = acc2[i][j][v], p3 = acc3[i][j][v];
+          float2 val = CONJB ? make_float2(p1 + p2, (p3 - p1) + p2) : make_float2(p1 - p2, (p3 - p1) - p2);
           *reinterpret_cast<float2 *>(out + 2 * ((long long)row * ld + col)) = val;
         }
       }
@@ -405,11 +582,17 @@ void launch_cgemm(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, int 
 
 template <bool CJ>
 void launch_cgemm_wg(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, int splits, float *C, long long ldc,
-                     float *part, hipStream_t st) {
+                     float *part, hipStream_t st, bool split_major, bool four_mult) {
   int row_groups = ((Ra + 15) / 16 + kWgA - 1) / kWgA;
   int col_groups = ((Rb + 15) / 16 + kWgB - 1) / kWgB;
-  unsigned blocks = (unsigned)(8LL * row_groups * ((col_groups + 7) / 8));
-  k_cgemm_wg<CJ><<<dim3(blocks, 1, splits), 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, C, ldc, part);
+  const bool sm = split_major && splits >= 8;  // each XCD takes whole splits
+  dim3 grid = sm ? dim3((unsigned)(8LL * ((splits + 7) / 8) * row_groups * col_groups), 1, 1)
+                 : dim3((unsigned)(8LL * row_groups * ((col_groups + 7) / 8)), 1, (unsigned)splits);
+  const int sn = sm ? splits : 0;
+  if (four_mult)
+    k_cgemm_wg<CJ><<<grid, 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, sn, C, ldc, part);
+  else
+    k_cgemm_wg3<CJ><<<grid, 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, sn, C, ldc, part);
 }
 
 constexpr int kMaxDevices = 64;
@@ -478,6 +661,8 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
   int n_kb = (Kd + 31) / 32;
   if (splits > n_kb) splits = n_kb;
   const bool per_wave = (conj_b & 2) != 0;
+  const bool split_major = (conj_b & 4) == 0;
+  const bool four_mult = (conj_b & 8) == 0;
   conj_b &= 1;
   hipStream_t st = (hipStream_t)stream;
   const u32x4 *a = (const u32x4 *)A, *b = (const u32x4 *)B;
@@ -493,8 +678,8 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
     if (conj_b) launch_cgemm<4, 2, true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
     else launch_cgemm<4, 2, false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
   } else {
-    if (conj_b) launch_cgemm_wg<true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
-    else launch_cgemm_wg<false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
+    if (conj_b) launch_cgemm_wg<true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st, split_major, four_mult);
+    else launch_cgemm_wg<false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st, split_major, four_mult);
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && splits > 1) {

@@ -48,15 +48,20 @@ def frag_pack(X: torch.Tensor, R: int, Kd: int, sr: int, sk: int) -> torch.Tenso
 
 
 def cgemm_frag(fa: torch.Tensor, fb: torch.Tensor, Ra: int, Rb: int, Kd: int, conj_b: bool,
-               splits: int = 1, per_wave: bool = False) -> torch.Tensor:
+               splits: int = 1, per_wave: bool = False, split_minor: bool = False,
+               three_mult: bool = False) -> torch.Tensor:
     """C[Ra, Rb] = A B^T (or A conj(B)^T) from fragment-packed operands.
-    per_wave selects the per-wave kernel also for Ra >= 128 (comparison)."""
+    per_wave selects the per-wave kernel also for Ra >= 128; split_minor and
+    three_mult the workgroup kernel's split-minor order and three-product
+    form (comparisons)."""
     L = _lib.lib()
     out = torch.empty(Ra, Rb, dtype=torch.complex64, device=fa.device)
     ws = None
     if splits > 1:
         ws = torch.empty(max(1, L.ams_cgemm_workspace_bytes(Ra, Rb, splits)), dtype=torch.uint8, device=fa.device)
-    _lib.check(L.ams_cgemm_frag(_ptr(fa), _ptr(fb), Ra, Rb, Kd, (1 if conj_b else 0) | (2 if per_wave else 0),
+    _lib.check(L.ams_cgemm_frag(_ptr(fa), _ptr(fb), Ra, Rb, Kd,
+                                (1 if conj_b else 0) | (2 if per_wave else 0) | (4 if split_minor else 0)
+                                | (8 if three_mult else 0),
                                 _ptr(out), Rb, splits,
                                 _ptr(ws) if ws is not None else None, _stream()), "ams_cgemm_frag")
     return out

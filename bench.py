@@ -35,6 +35,9 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# MFMA flops issued per useful complex-GEMM flop: four split-bf16 real
+# products of 3 MFMAs each per complex product
+ISSUED_PER_USEFUL = 3.0
 # VALU issue: two wave64 VALU instructions per CU per cycle (FP32 and 32-bit
 # integer adds issue at 1.86 per CU per cycle in the ams_valu_probe counter
 # run, profiles/valu_issue_ceiling.txt; FMA at 0.97, half rate), 256 CUs at
@@ -279,7 +282,39 @@ def contraction_office(iters: int = 20) -> dict:
     e1.record()
     torch.cuda.synchronize()
     t_fwd = e0.elapsed_time(e1) / iters * 1e-3
+    e0.record()
+    for _ in range(iters):
+        cgemm_frag(fa, op.frag_fwd, M, N, Kc, conj_b=False, three_mult=True)
+    e1.record()
+    torch.cuda.synchronize()
+    t_fwd_3m = e0.elapsed_time(e1) / iters * 1e-3
     useful = 8.0 * M * Kc * N
+    # the backward product gA = gY . conj([F0 ; H_d])^T alone, split over Rx
+    g = torch.Generator(device=A.device).manual_seed(3)
+    gY = torch.complex(torch.randn(M, N, generator=g, device=A.device), torch.randn(M, N, generator=g, device=A.device))
+    fg = frag_pack(gY, M, N, N, 1)
+    for _ in range(3):
+        cgemm_frag(fg, op.frag_bwd, M, Kc, N, conj_b=True, splits=op.bwd_splits)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(iters):
+        cgemm_frag(fg, op.frag_bwd, M, Kc, N, conj_b=True, splits=op.bwd_splits)
+    e1.record()
+    torch.cuda.synchronize()
+    t_bwd = e0.elapsed_time(e1) / iters * 1e-3
+    e0.record()
+    for _ in range(iters):
+        cgemm_frag(fg, op.frag_bwd, M, Kc, N, conj_b=True, splits=op.bwd_splits, split_minor=True)
+    e1.record()
+    torch.cuda.synchronize()
+    t_bwd_minor = e0.elapsed_time(e1) / iters * 1e-3
+    e0.record()
+    for _ in range(iters):
+        cgemm_frag(fg, op.frag_bwd, M, Kc, N, conj_b=True, splits=op.bwd_splits, three_mult=True)
+    e1.record()
+    torch.cuda.synchronize()
+    t_bwd_3m = e0.elapsed_time(e1) / iters * 1e-3
+    del gY, fg
     opt = FusedAdam([p for p in model.parameters() if p.requires_grad], lr=0.3)
     for _ in range(2):
         model().backward()
@@ -334,13 +369,18 @@ def contraction_office(iters: int = 20) -> dict:
     lib = {"ms": t_lib * 1e3, "per_rx_rel_err_max": float(err_lib.max()), "speedup_of_mfma_path": t_lib / t_fwd,
            "kind": "torch complex64 matmul on the GPU (the FP32 library GEMM), same operands"}
     return {"shape": {"M": M, "K": Kc, "N": N}, "fwd_gemm_ms": t_fwd * 1e3,
+            "bwd_gemm_ms": t_bwd * 1e3, "bwd_splits": op.bwd_splits, "bwd_useful_tflops": useful / t_bwd / 1e12,
+            "bwd_gemm_ms_split_minor": t_bwd_minor * 1e3,
+            "three_product_form": {"fwd_gemm_ms": t_fwd_3m * 1e3, "bwd_gemm_ms": t_bwd_3m * 1e3,
+                                   "what": "the workgroup kernel with 9 MFMAs per complex tile pair instead of 12 "
+                                           "(six planes per block staged in LDS)"},
             "rx_channels_per_s": M * N / t_fwd, "parity_vs_fp64": parity, "host_baseline": host,
             "gpu_fp32_library_gemm": lib,
-            "roofline": {"bound": "mfma", "achieved": 3 * useful / t_fwd / 1e12, "peak": MFMA_BF16_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": 3 * useful / t_fwd / 2.5e15, "traffic": tr_meas["bytes"],
+            "roofline": {"bound": "mfma", "achieved": ISSUED_PER_USEFUL * useful / t_fwd / 1e12, "peak": MFMA_BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": ISSUED_PER_USEFUL * useful / t_fwd / 2.5e15, "traffic": tr_meas["bytes"],
                          "mfma_busy_pmc": tr_meas["mfma_util"]},
-            "useful_tflops": useful / t_fwd / 1e12, "issued_tflops": 3 * useful / t_fwd / 1e12,
-            "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": 3 * useful / t_fwd / 2.5e15,
+            "useful_tflops": useful / t_fwd / 1e12, "issued_tflops": ISSUED_PER_USEFUL * useful / t_fwd / 1e12,
+            "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": ISSUED_PER_USEFUL * useful / t_fwd / 2.5e15,
             "f_operand_gb_per_s": 16.0 * Kc * N / t_fwd / 1e9,
             "optimizer_step_ms": t_step * 1e3, "maps": 4, "beams": 64,
             "office_traces": {"scene_tris": tr["H_d"].scene.n_tri, "max_reflection": tr["H_d"].max_reflection,
@@ -411,7 +451,7 @@ def office_phase_map(outs: dict, steps: int = 200) -> dict:
     step replayed from a HIP graph; against the codebook optimised alone."""
     import torch
 
-    from automs_amd import scenes
+    from automs_amd import experiment, scenes
     from automs_amd.phasemap import Metasurface, PhaseMapModel, optimize_model
     ch = {scenes.OFFICE_CHANNEL_KEYS[k]: torch.complex(v[..., 0], v[..., 1]) for k, v in outs.items()}
     ms = [Metasurface(None, (48, 48), None, None, [], max_phase=360.0) for _ in range(3)]
@@ -423,7 +463,7 @@ def office_phase_map(outs: dict, steps: int = 200) -> dict:
         model = PhaseMapModel(name, chans, surfaces, code_num=30, objective="logsnr", max_ms_reflex=2, seed=0)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        cov0 = float(model.coverage_db()[0])
+        st0 = {f: experiment.coverage_stats(model, f) for f in (-70.0, -100.0)}
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         optimize_model(model, lr=0.3, step=steps, graph=True)
@@ -431,16 +471,24 @@ def office_phase_map(outs: dict, steps: int = 200) -> dict:
         torch.cuda.synchronize()
         with torch.no_grad():
             model()
+        st1 = {f: experiment.coverage_stats(model, f) for f in (-70.0, -100.0)}
         res[name] = {"setup_seconds": t1 - t0, "seconds": e0.elapsed_time(e1) * 1e-3,
-                     "ms_per_step": e0.elapsed_time(e1) / steps, "coverage_db_start": cov0,
-                     "coverage_db_end": float(model.coverage_db()[0]), "loss_end": float(model.loss.item()),
-                     "best_loss": float(min(model.loss_list))}
+                     "ms_per_step": e0.elapsed_time(e1) / steps, "loss_end": float(model.loss.item()),
+                     "best_loss": float(min(model.loss_list)),
+                     "coverage_start": {f"floor_{-int(f)}_db": {"mean_db": v["mean_db"], "frac_above": v["frac_above_floor"]}
+                                        for f, v in st0.items()},
+                     "coverage_end": {f"floor_{-int(f)}_db": {"mean_db": v["mean_db"], "frac_above": v["frac_above_floor"]}
+                                      for f, v in st1.items()}}
         if surfaces:
             res[name]["operand_gb"] = model.op.nbytes() / 1e9
             res[name]["chain_transfer_gb"] = model.chain.frag.numel() / 1e9
         del model
         torch.cuda.empty_cache()
-    res["coverage_gain_db"] = res["three_surfaces"]["coverage_db_end"] - res["codebook_alone"]["coverage_db_end"]
+    res["coverage_gain_db"] = {k: res["three_surfaces"]["coverage_end"][k]["mean_db"]
+                               - res["codebook_alone"]["coverage_end"][k]["mean_db"]
+                               for k in res["three_surfaces"]["coverage_end"]}
+    res["coverage"] = ("per Rx the best code's 20 log10 |y|, clamped at the floor (go.py:944-964's rule with its "
+                       "-70 dB floor, and -100 dB), averaged over the 65,536 Rx")
     res["workload"] = ("config 2's traced office channels: 8 x 8 AP, 30 codes, three 48 x 48 metasurfaces, chains "
                        "of up to 2 surfaces, 65,536 Rx; K = 3 x 2304 + 64, the split-bf16 operand in both layouts")
     res["steps"] = steps
@@ -572,6 +620,7 @@ def main():
     ap.add_argument("--no-office", action="store_true", help="skip the office workload at its stated size")
     ap.add_argument("--no-reference-run", action="store_true", help="skip the reference's own phase-map run")
     ap.add_argument("--no-placement-search", action="store_true", help="skip the surrogate placement search")
+    ap.add_argument("--only-contract", action="store_true", help="only the config-3 contraction timings")
     ap.add_argument("--sa-steps", type=int, default=2000, help="annealing steps of each placement search")
     ap.add_argument("--office-steps", type=int, default=2)
     ap.add_argument("--office-warmup", type=int, default=1)
@@ -612,6 +661,9 @@ def main():
                 e1.record()
                 torch.cuda.synchronize()
                 print(f"valu probe kind {kind}: {e0.elapsed_time(e1):.3f} ms", flush=True)
+        return
+    if args.only_contract:
+        print(json.dumps(contraction_office(iters=20)), flush=True)
         return
     if args.only_office:
         r = office_full(1 if args.profile_pass else args.office_steps, 0 if args.profile_pass else args.office_warmup,

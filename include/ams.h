@@ -143,7 +143,13 @@ int ams_frag_pack(const float *X, int32_t R, int32_t Kd, int64_t sr, int64_t sk,
  * accumulation; splits > 1 splits the reduction and needs
  * ams_cgemm_workspace_bytes(Ra, Rb, splits) of workspace.  Ra >= 128 uses
  * the LDS-staged workgroup kernel unless conj_b bit 1 asks for the per-wave
- * kernel (kept for comparison; identical results up to FP32 rounding). */
+ * kernel (kept for comparison; identical results up to FP32 rounding).
+ * With 8 or more splits the workgroup kernel gives each XCD whole splits, so
+ * a split's slice of A crosses HBM once; conj_b bit 2 keeps the split-minor
+ * order (identical results).  conj_b bit 3: the workgroup kernel forms each complex
+ * tile product from three real products (9 MFMAs per tile pair instead of
+ * 12, six planes per block staged in LDS; within 1e-5 of the four-product
+ * form; measured slower on dense operands). */
 int64_t ams_cgemm_workspace_bytes(int32_t Ra, int32_t Rb, int32_t splits);
 int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b, float *C,
                    int64_t ldc, int32_t splits, void *workspace, void *stream);

@@ -38,29 +38,40 @@ def test_cgemm_frag_matches_complex128(gpu, Ra, Rb, Kd, conj_b):
     ref = A @ (B.conj() if conj_b else B).T
     fa = frag_pack(A.to(torch.complex64).to(gpu), Ra, Kd, Kd, 1)
     fb = frag_pack(B.to(torch.complex64).to(gpu), Rb, Kd, Kd, 1)
-    for splits in (1, 4):
-        out = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits).cpu().to(torch.complex128)
-        rel = (out - ref).norm() / ref.norm()
-        _flush(f"  splits={splits} rel={rel:.3e}")
-        assert rel < 3e-5
+    for splits in (1, 4, 16):
+        for three in (False, True):
+            out = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits, three_mult=three).cpu().to(torch.complex128)
+            rel = (out - ref).norm() / ref.norm()
+            _flush(f"  splits={splits} three-product={three} rel={rel:.3e}")
+            assert rel < 3e-5
 
 
 @pytest.mark.parametrize("Ra,Rb,Kd", [(256, 1000, 700), (130, 77, 300)])
 @pytest.mark.parametrize("conj_b", [False, True])
 def test_cgemm_workgroup_kernel_equals_per_wave(gpu, Ra, Rb, Kd, conj_b):
     """The LDS-staged workgroup kernel (Ra >= 128) issues the same MFMAs in
-    the same order per output tile as the per-wave kernel: bit-identical."""
+    the same order per output tile as the per-wave kernel: bit-identical,
+    in both split orders.  Its three-product form agrees with both
+    to the split-bf16 precision."""
     from automs_amd.phasemap import cgemm_frag, frag_pack
     _flush(f"case: cgemm wg vs per-wave Ra={Ra} Rb={Rb} Kd={Kd} conj={conj_b}")
     A = _crand(Ra, Kd, seed=5).to(torch.complex64).to(gpu)
     B = _crand(Rb, Kd, seed=6).to(torch.complex64).to(gpu)
     fa = frag_pack(A, Ra, Kd, Kd, 1)
     fb = frag_pack(B, Rb, Kd, Kd, 1)
-    for splits in (1, 3):
+    for splits in (1, 3, 10, 16):  # 10 and 16: each XCD takes whole splits
         x = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits)
         y = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits, per_wave=True)
+        xm = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits, split_minor=True)
+        z = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits, three_mult=True)
+        zm = cgemm_frag(fa, fb, Ra, Rb, Kd, conj_b, splits=splits, three_mult=True, split_minor=True)
         torch.cuda.synchronize()
         assert torch.equal(torch.view_as_real(x), torch.view_as_real(y))
+        assert torch.equal(torch.view_as_real(xm), torch.view_as_real(y))
+        assert torch.equal(torch.view_as_real(zm), torch.view_as_real(z))
+        rel = ((z - y).abs().max() / y.abs().max()).item()
+        _flush(f"  splits={splits}: three-product vs four-product max diff / max {rel:.2e}")
+        assert rel < 3e-5
 
 
 def test_cgemm_f32_plain(gpu):
