@@ -132,6 +132,11 @@ def _bind_contract(L):
         "ams_conv3x3_fwd": ([vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp], C.c_int),
         "ams_conv3x3_bwd_weight": ([vp, vp, vp, vp, i32, i32, i32, i32, i32, vp], C.c_int),
         "ams_relu_bwd": ([vp, vp, i64, vp], C.c_int),
+        "ams_conv3x3_wfrag_bytes": ([i32, i32], i64),
+        "ams_conv3x3_pack_weights": ([vp, i32, i32, vp, vp], C.c_int),
+        "ams_conv3x3_fwd_mfma": ([vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp], C.c_int),
+        "ams_conv3x3_dw_workspace_bytes": ([i32, i32, i32, i32, i32], i64),
+        "ams_conv3x3_bwd_weight_mfma": ([vp, vp, vp, vp, i32, i32, i32, i32, i32, vp, vp], C.c_int),
         "ams_mse_partials": ([i64], i64),
         "ams_mse": ([vp, vp, vp, i64, vp, vp, vp], C.c_int),
         "ams_to_bf16": ([vp, vp, i64, vp], C.c_int),
@@ -160,4 +165,6 @@ EXPORTED = [
     "ams_conv3x3_fwd", "ams_conv3x3_bwd_weight", "ams_relu_bwd", "ams_mse_partials", "ams_mse", "ams_to_bf16",
     "ams_cgemm_noalloc_max_bytes", "ams_cgemm_frag_noalloc", "ams_valu_probe",
     "ams_chain_first", "ams_chain_step", "ams_chain_bwd_scale", "ams_chain_bwd_ms", "ams_chain_bwd_ap",
+    "ams_conv3x3_wfrag_bytes", "ams_conv3x3_pack_weights", "ams_conv3x3_fwd_mfma", "ams_conv3x3_dw_workspace_bytes",
+    "ams_conv3x3_bwd_weight_mfma",
 ]

@@ -63,9 +63,11 @@ def _s() -> C.c_void_p:
 # ----------------------------------------------------------------- network --
 
 class SurrogateNet:
-    """Conv net on the GPU with hand-written kernels."""
+    """Conv net on the GPU with hand-written kernels: bf16 MFMA implicit
+    GEMMs (mfma=True, the default) or the FP32 vector-ALU kernels."""
 
-    def __init__(self, channels=(9, 32, 32, 32, 1), seed: int = 0, device="cuda"):
+    def __init__(self, channels=(9, 32, 32, 32, 1), seed: int = 0, device="cuda", mfma: bool = True):
+        self.mfma = mfma
         self.ch = list(channels)
         self.dev = torch.device(device)
         g = torch.Generator().manual_seed(seed)
@@ -118,11 +120,18 @@ class SurrogateNet:
     def refresh(self):
         """bf16 weight copies from the FP32 masters, and the flipped /
         channel-transposed copies the input-gradient pass uses."""
-        self.w_bf, self.wt_bf = [], []
+        L = _lib.lib()
+        self.w_bf, self.wt_bf, self.w_frag, self.wt_frag = [], [], [], []
         for l in range(len(self.ch) - 1):
             W = self.param(2 * l)
             self.w_bf.append(self._bf16(W))
             self.wt_bf.append(self._bf16(W.flip(2, 3).transpose(0, 1).contiguous()))
+            if self.mfma:
+                for src, co, ci, dst in ((self.w_bf[-1], self.ch[l + 1], self.ch[l], self.w_frag),
+                                         (self.wt_bf[-1], self.ch[l], self.ch[l + 1], self.wt_frag)):
+                    f = torch.empty(L.ams_conv3x3_wfrag_bytes(co, ci), dtype=torch.uint8, device=self.dev)
+                    _lib.check(L.ams_conv3x3_pack_weights(_p(src), co, ci, _p(f), _s()), "ams_conv3x3_pack_weights")
+                    dst.append(f)
 
     def weights_f64(self) -> List[torch.Tensor]:
         """The bf16-rounded weights the kernels use, as float64 CPU tensors."""
@@ -144,13 +153,25 @@ class SurrogateNet:
         nl = len(self.ch) - 1
         for l in range(nl):
             y = torch.empty(B, self.ch[l + 1], H, W, dtype=torch.float32, device=self.dev)
-            _lib.check(L.ams_conv3x3_fwd(_p(a), _p(self.w_bf[l]), _p(self.param(2 * l + 1)), _p(y), B, self.ch[l],
-                                         self.ch[l + 1], H, W, 1 if l < nl - 1 else 0, _s()), "ams_conv3x3_fwd")
+            self._conv(a, l, False, _p(self.param(2 * l + 1)), y, B, H, W, 1 if l < nl - 1 else 0)
             acts.append(y)
             a = y
         if keep:
             self.acts = acts
         return a
+
+    def _conv(self, x: torch.Tensor, l: int, transposed: bool, bias, y: torch.Tensor, B: int, H: int, W: int,
+              relu: int):
+        """Layer l's convolution (transposed: its input-gradient pass)."""
+        L = _lib.lib()
+        ci, co = (self.ch[l + 1], self.ch[l]) if transposed else (self.ch[l], self.ch[l + 1])
+        if self.mfma:
+            wf = self.wt_frag[l] if transposed else self.w_frag[l]
+            _lib.check(L.ams_conv3x3_fwd_mfma(_p(x), _p(wf), bias, _p(y), B, ci, co, H, W, relu, _s()),
+                       "ams_conv3x3_fwd_mfma")
+        else:
+            wb = self.wt_bf[l] if transposed else self.w_bf[l]
+            _lib.check(L.ams_conv3x3_fwd(_p(x), _p(wb), bias, _p(y), B, ci, co, H, W, relu, _s()), "ams_conv3x3_fwd")
 
     def backward(self, gY: torch.Tensor):
         """Fills self.grad from dL/dY of the last forward(keep=True)."""
@@ -160,14 +181,20 @@ class SurrogateNet:
         g = gY.contiguous()
         nl = len(self.ch) - 1
         for l in reversed(range(nl)):
-            _lib.check(L.ams_conv3x3_bwd_weight(_p(acts[l]), _p(g), _p(self.grad_of(2 * l)),
-                                                _p(self.grad_of(2 * l + 1)), B, self.ch[l], self.ch[l + 1], H, W,
-                                                _s()), "ams_conv3x3_bwd_weight")
+            if self.mfma:
+                ws = torch.empty(L.ams_conv3x3_dw_workspace_bytes(B, self.ch[l], self.ch[l + 1], H, W),
+                                 dtype=torch.uint8, device=self.dev)
+                _lib.check(L.ams_conv3x3_bwd_weight_mfma(_p(acts[l]), _p(g), _p(self.grad_of(2 * l)),
+                                                         _p(self.grad_of(2 * l + 1)), B, self.ch[l], self.ch[l + 1],
+                                                         H, W, _p(ws), _s()), "ams_conv3x3_bwd_weight_mfma")
+            else:
+                _lib.check(L.ams_conv3x3_bwd_weight(_p(acts[l]), _p(g), _p(self.grad_of(2 * l)),
+                                                    _p(self.grad_of(2 * l + 1)), B, self.ch[l], self.ch[l + 1], H, W,
+                                                    _s()), "ams_conv3x3_bwd_weight")
             if l == 0:
                 break
             gx = torch.empty(B, self.ch[l], H, W, dtype=torch.float32, device=self.dev)
-            _lib.check(L.ams_conv3x3_fwd(_p(g), _p(self.wt_bf[l]), None, _p(gx), B, self.ch[l + 1], self.ch[l], H, W,
-                                         0, _s()), "ams_conv3x3_fwd (input gradient)")
+            self._conv(g, l, True, None, gx, B, H, W, 0)
             _lib.check(L.ams_relu_bwd(_p(acts[l]), _p(gx), gx.numel(), _s()), "ams_relu_bwd")
             g = gx
 

@@ -424,6 +424,38 @@ def reference_run() -> dict:
     }
 
 
+def surrogate_training(n: int = 2048, epochs: int = 2, batch: int = 64) -> dict:
+    """Training throughput of the placement surrogate at its stated shape
+    (9 input channels, four 3 x 3 convolutions of 32 channels, 64 x 64 maps,
+    batch 64), on synthetic inputs and labels from the same start: the bf16
+    MFMA kernels against the FP32 vector-ALU kernels.  Work per sample and
+    step: the forward, the input gradients of layers 1-3 and every weight
+    gradient, 2 flops per multiply-add."""
+    import torch
+
+    from automs_amd import surrogate as S
+    g = torch.Generator(device="cuda").manual_seed(0)
+    X = torch.randn(n, 9, 64, 64, generator=g, device="cuda")
+    Y = torch.randn(n, 1, 64, 64, generator=g, device="cuda")
+    ch = (9, 32, 32, 32, 1)
+    macs = [ch[i] * 9 * ch[i + 1] * 64 * 64 for i in range(4)]
+    flop = 2.0 * (sum(macs) + sum(macs[1:]) + sum(macs))
+    out = {"samples": n, "epochs": epochs, "batch": batch, "flop_per_sample_step": flop}
+    for mfma in (True, False):
+        net = S.SurrogateNet(seed=0, mfma=mfma)
+        S.train(net, X[:2 * batch], Y[:2 * batch], epochs=1, batch=batch)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hist = S.train(net, X, Y, epochs=epochs, batch=batch)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ss = epochs * (n // batch) * batch
+        out["mfma" if mfma else "fp32"] = {"seconds": dt, "sample_steps_per_s": ss / dt,
+                                           "tflops": ss * flop / dt / 1e12, "final_mse": hist[-1]}
+    out["speedup_mfma_over_fp32"] = out["fp32"]["seconds"] / out["mfma"]["seconds"]
+    return out
+
+
 def placement_search(sa_steps: int) -> dict:
     """The placement search (hyper-config tuner of physical_exp_nni_main.py
     with its trial objective): plain simulated annealing with every candidate
@@ -621,6 +653,7 @@ def main():
     ap.add_argument("--no-reference-run", action="store_true", help="skip the reference's own phase-map run")
     ap.add_argument("--no-placement-search", action="store_true", help="skip the surrogate placement search")
     ap.add_argument("--only-contract", action="store_true", help="only the config-3 contraction timings")
+    ap.add_argument("--only-surrogate", action="store_true", help="only the surrogate training throughput")
     ap.add_argument("--sa-steps", type=int, default=2000, help="annealing steps of each placement search")
     ap.add_argument("--office-steps", type=int, default=2)
     ap.add_argument("--office-warmup", type=int, default=1)
@@ -661,6 +694,9 @@ def main():
                 e1.record()
                 torch.cuda.synchronize()
                 print(f"valu probe kind {kind}: {e0.elapsed_time(e1):.3f} ms", flush=True)
+        return
+    if args.only_surrogate:
+        print(json.dumps(surrogate_training()), flush=True)
         return
     if args.only_contract:
         print(json.dumps(contraction_office(iters=20)), flush=True)
@@ -815,6 +851,9 @@ def main():
         print(f"reference run: {r}", file=sys.stderr, flush=True)
         line["reference_run"] = r
     if not args.no_placement_search and not args.profile_pass:
+        r = surrogate_training()
+        print(f"surrogate training: {r}", file=sys.stderr, flush=True)
+        line["surrogate_training"] = r
         r = placement_search(args.sa_steps)
         print(f"placement search: {r}", file=sys.stderr, flush=True)
         line["placement_search"] = r

@@ -284,6 +284,22 @@ int ams_conv3x3_fwd(const float *X, const void *Wbf16, const float *bias, float 
 int ams_conv3x3_bwd_weight(const float *X, const float *dY, float *dW, float *db, int32_t B, int32_t Cin,
                            int32_t Cout, int32_t H, int32_t W, void *stream);
 /* grad *= (Y > 0) */
+/* The same convolution passes on bf16 MFMA (implicit GEMMs; activations and
+ * output gradients split exactly into three bf16 parts, so the products
+ * carry FP32 precision).  The
+ * weights first go into the A-operand fragment layout,
+ * ams_conv3x3_wfrag_bytes(Cout, Cin) bytes (for the input gradient: the
+ * flipped, channel-transposed weights with Cin and Cout swapped).  The weight
+ * gradient needs ams_conv3x3_dw_workspace_bytes(B, Cin, Cout, H, W) of
+ * workspace for its per-tile partial sums, added in a fixed order.  B <=
+ * 65535, channels <= 32, H and W multiples of 16. */
+int64_t ams_conv3x3_wfrag_bytes(int32_t Cout, int32_t Cin);
+int ams_conv3x3_pack_weights(const void *Wbf16, int32_t Cout, int32_t Cin, void *frag, void *stream);
+int ams_conv3x3_fwd_mfma(const float *X, const void *Wfrag, const float *bias, float *Y, int32_t B, int32_t Cin,
+                         int32_t Cout, int32_t H, int32_t W, int32_t relu, void *stream);
+int64_t ams_conv3x3_dw_workspace_bytes(int32_t B, int32_t Cin, int32_t Cout, int32_t H, int32_t W);
+int ams_conv3x3_bwd_weight_mfma(const float *X, const float *dY, float *dW, float *db, int32_t B, int32_t Cin,
+                                int32_t Cout, int32_t H, int32_t W, void *workspace, void *stream);
 int ams_relu_bwd(const float *Y, float *grad, int64_t n, void *stream);
 /* mean((pred - target)^2 mask): grad [n] and per-block partial sums
  * [ams_mse_partials(n)] of the squared errors (sum them and divide by n) */
