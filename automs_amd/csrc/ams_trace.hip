@@ -1448,6 +1448,161 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
   evaluate_body<false, EXPORT, true>(a);
 }
 
+// ---- neighbour expansion on the device (optixPathTracer.cpp:264-284) ----
+// The traced (path, tile) pairs and the pairs new in the next round are two
+// tile-major bit maps, bit g * n_paths + p.  A round's tasks are grouped by
+// tile, paths ascending within a tile; tile_start / unit_start give each
+// tile's tasks and its units of at most `slice` tasks.
+
+// Every successful task (p, g) marks (p, g +- 1) and (p, g +- n1g) within the
+// linear bounds (the reference's g +- 1 wraps across tile rows); a pair it is
+// the first to mark goes into the new map.  round_nt: the round's task count.
+__global__ void __launch_bounds__(256) k_expand(const int *__restrict__ task_path, const int *__restrict__ task_tile,
+                                                const unsigned char *__restrict__ ok,
+                                                const int *__restrict__ round_nt, int n_grid, int n1g, int n_paths,
+                                                unsigned *__restrict__ traced, unsigned *__restrict__ fresh) {
+  const int nt = *round_nt;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    if (!ok[t]) continue;
+    const int p = task_path[t], g = task_tile[t];
+    const int nb[4] = {g - 1, g + 1, g - n1g, g + n1g};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int h = nb[k];
+      if (h < 0 || h >= n_grid) continue;
+      const long long b = (long long)h * n_paths + p;
+      const unsigned m = 1u << (b & 31);
+      if (!(atomicOr(&traced[b >> 5], m) & m)) atomicOr(&fresh[b >> 5], m);
+    }
+  }
+}
+
+// the bits of tile g in word w of a tile-major map
+__device__ __forceinline__ unsigned tile_word_mask(long long b0, long long b1, long long w) {
+  const long long lo = b0 > 32 * w ? b0 - 32 * w : 0, hi = b1 < 32 * w + 32 ? b1 - 32 * w : 32;
+  if (hi <= lo) return 0u;
+  const unsigned top = hi >= 32 ? 0xffffffffu : ((1u << hi) - 1u);
+  return top & ~((1u << lo) - 1u);
+}
+
+// Block-wide exclusive prefix of v over 256 threads; *total gets the sum.
+__device__ __forceinline__ int block_excl_scan256(int v, int *sh, int *total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int x = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  const int incl = sh[t];
+  *total = sh[255];
+  __syncthreads();
+  return incl - v;
+}
+
+// New tasks of each tile: the population count of its bits of the new map.
+// One workgroup per tile.
+__global__ void __launch_bounds__(256) k_tile_count(const unsigned *__restrict__ fresh, int n_paths,
+                                                    int *__restrict__ tile_cnt) {
+  __shared__ int sh[256];
+  const long long g = blockIdx.x, b0 = g * n_paths, b1 = b0 + n_paths;
+  int c = 0;
+  for (long long w = (b0 >> 5) + threadIdx.x; w <= ((b1 - 1) >> 5); w += 256)
+    c += __popc(fresh[w] & tile_word_mask(b0, b1, w));
+  int total;
+  block_excl_scan256(c, sh, &total);
+  if (threadIdx.x == 0) tile_cnt[g] = total;
+}
+
+// Exclusive prefix over the tiles (one workgroup of 1024): tile_start and
+// unit_start of the next round; info = {tasks, units}.
+__global__ void __launch_bounds__(1024) k_tile_scan(const int *__restrict__ tile_cnt, int n_grid, int slice,
+                                                    int *__restrict__ tile_start, int *__restrict__ unit_start,
+                                                    int *__restrict__ info) {
+  __shared__ int st[1024], su[1024];
+  const int t = threadIdx.x, chunk = (n_grid + 1023) / 1024;
+  const int g0 = min(n_grid, t * chunk), g1 = min(n_grid, g0 + chunk);
+  int ct = 0, cu = 0;
+  for (int g = g0; g < g1; g++) {
+    ct += tile_cnt[g];
+    cu += (tile_cnt[g] + slice - 1) / slice;
+  }
+  st[t] = ct;
+  su[t] = cu;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int xt = t >= off ? st[t - off] : 0, xu = t >= off ? su[t - off] : 0;
+    __syncthreads();
+    st[t] += xt;
+    su[t] += xu;
+    __syncthreads();
+  }
+  int rt = st[t] - ct, ru = su[t] - cu;
+  for (int g = g0; g < g1; g++) {
+    tile_start[g] = rt;
+    unit_start[g] = ru;
+    rt += tile_cnt[g];
+    ru += (tile_cnt[g] + slice - 1) / slice;
+  }
+  if (t == 1023) {
+    tile_start[n_grid] = st[1023];
+    unit_start[n_grid] = su[1023];
+    info[0] = st[1023];
+    info[1] = su[1023];
+  }
+}
+
+// The next round's tasks of each tile from the new map, paths ascending;
+// the tile's bits are cleared and the tasks' success flags zeroed.  One
+// workgroup per tile.
+__global__ void __launch_bounds__(256) k_tile_emit(unsigned *__restrict__ fresh, int n_paths,
+                                                   const int *__restrict__ tile_start, int *__restrict__ task_path,
+                                                   int *__restrict__ task_tile, unsigned char *__restrict__ ok) {
+  __shared__ int sh[256];
+  const long long g = blockIdx.x, b0 = g * n_paths, b1 = b0 + n_paths;
+  const long long w_first = b0 >> 5, w_last = (b1 - 1) >> 5;
+  int base = tile_start[g];
+  for (long long wc = w_first; wc <= w_last; wc += 256) {
+    const long long w = wc + threadIdx.x;
+    unsigned bits = 0u;
+    if (w <= w_last) {
+      const unsigned m = tile_word_mask(b0, b1, w);
+      bits = fresh[w] & m;
+      if (bits) atomicAnd(&fresh[w], ~m);
+    }
+    int total;
+    int pos = base + block_excl_scan256(__popc(bits), sh, &total);
+    while (bits) {
+      const int k = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      task_path[pos] = (int)(32 * w + k - b0);
+      task_tile[pos] = (int)g;
+      ok[pos] = 0;
+      pos++;
+    }
+    base += total;
+  }
+}
+
+// The units of a round from its tile and unit offsets: unit u of tile g
+// covers tasks tile_start[g] + slice (u - unit_start[g]) onwards, at most
+// slice of them.  Thread per tile.
+__global__ void __launch_bounds__(256) k_units(const int *__restrict__ tile_start, const int *__restrict__ unit_start,
+                                               int n_grid, int slice, int *__restrict__ unit_tile,
+                                               int *__restrict__ unit_begin, int *__restrict__ unit_end) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n_grid; g += gridDim.x * blockDim.x) {
+    const int t0 = tile_start[g], t1 = tile_start[g + 1];
+    for (int u = unit_start[g]; u < unit_start[g + 1]; u++) {
+      const int b = t0 + (u - unit_start[g]) * slice;
+      unit_tile[u] = g;
+      unit_begin[u] = b;
+      unit_end[u] = min(b + slice, t1);
+    }
+  }
+}
+
 // Path keys of small scenes packed losslessly into 64 bits: every field
 // keeps its order (plane ids < 2^pb - 1, the "no plane" value 0xFFFFF maps
 // to 2^pb - 1, tiles < 2^tb), so a single keys-only radix sort over
@@ -2270,6 +2425,114 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     // (many Tx elements) to cut the FP64 atomics into H; short ones otherwise
     const int slice = o->tasks_per_unit > 0 ? o->tasks_per_unit : (bpt >= 8 ? 64 : 16);
     float eval_ms = 0.0f;
+    // Rounds built on the device: the expansion over the bitmap of traced
+    // pairs, the grouping by tile and the next round's units are kernels; the
+    // host reads back two counts per round and launches the next evaluation
+    // with its exact grid (HIP has no launch from the device, and a unit loop
+    // inside the evaluation kernel cost it about 24 more spilled registers).
+    // The default for pair maps above 2^21 pairs, where the host's
+    // expansion work is long (office H_d: 77.3 against 79.5 ms per trace);
+    // for small maps the host loop below is shorter (the room step: 32.3
+    // against 33.4 ms).  Flags bit 13 forces the device rounds, bit 12 the
+    // host loop; the task export and maps beyond 2^27 pairs use the host loop.
+    const bool dev_rounds = !info_file && traced_bits_on && n_pairs <= (1ull << 27) &&
+                            ((o->flags & 8192) || (!(o->flags & 4096) && n_pairs > (1ull << 21)));
+    if (dev_rounds && !tp.empty()) {
+      const size_t np = (size_t)n_pairs, words = (size_t)((n_pairs + 31) / 32);
+      const size_t ng1 = (size_t)n_grid + 1;
+      // two rounds' task lists (paths, tiles, success flags) and their tile
+      // and unit offsets, the two bit maps, the counts
+      int *d_path[2], *d_tile[2], *d_tstart[2], *d_ustart[2];
+      unsigned char *d_okr[2];
+      for (int b = 0; b < 2; b++) {
+        d_path[b] = (int *)sc.get(b ? "dr_path1" : "dr_path0", 4 * np);
+        d_tile[b] = (int *)sc.get(b ? "dr_tile1" : "dr_tile0", 4 * np);
+        d_okr[b] = (unsigned char *)sc.get(b ? "dr_ok1" : "dr_ok0", np);
+        d_tstart[b] = (int *)sc.get(b ? "dr_ts1" : "dr_ts0", 4 * ng1);
+        d_ustart[b] = (int *)sc.get(b ? "dr_us1" : "dr_us0", 4 * ng1);
+        if (!d_path[b] || !d_tile[b] || !d_okr[b] || !d_tstart[b] || !d_ustart[b]) return AMS_ERR_ALLOC;
+      }
+      unsigned *d_traced = (unsigned *)sc.get("dr_traced", 4 * words);
+      unsigned *d_fresh = (unsigned *)sc.get("dr_fresh", 4 * words);
+      int *d_cnt = (int *)sc.get("dr_cnt", 4 * ng1);
+      int *d_info = (int *)sc.get("dr_info", 8);
+      // the units of the round being evaluated (units <= tasks <= pairs);
+      // the next round's are written after this round's evaluation
+      int *d_units = (int *)sc.get("dr_units", 12 * np);
+      if (!d_traced || !d_fresh || !d_cnt || !d_info || !d_units) return AMS_ERR_ALLOC;
+      const unsigned units_grid = (unsigned)std::min<long long>(1024, (n_grid + 255) / 256);
+      // round 1 from the host: tasks, offsets, the traced map (tile-major)
+      const size_t nt1 = tp.size();
+      std::vector<int> ts(ng1, 0), us(ng1, 0);
+      for (size_t k = 0; k < nt1; k++) ts[tt[k] + 1]++;
+      for (long long g = 0; g < n_grid; g++) {
+        us[g + 1] = us[g] + (ts[g + 1] + slice - 1) / slice;
+        ts[g + 1] += ts[g];
+      }
+      std::vector<unsigned> tmap(words, 0u);
+      for (size_t k = 0; k < nt1; k++) {
+        const uint64_t b = (uint64_t)tt[k] * (uint64_t)n_paths + (uint64_t)tp[k];
+        tmap[b >> 5] |= 1u << (b & 31);
+      }
+      HIPCHK(hipMemcpyAsync(d_path[0], tp.data(), 4 * nt1, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(d_tile[0], tt.data(), 4 * nt1, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemsetAsync(d_okr[0], 0, nt1, st));
+      HIPCHK(hipMemcpyAsync(d_tstart[0], ts.data(), 4 * ng1, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(d_ustart[0], us.data(), 4 * ng1, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(d_traced, tmap.data(), 4 * words, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemsetAsync(d_fresh, 0, 4 * words, st));
+      ea.info = nullptr;
+      const bool expand = !(o->flags & 2);
+      long long nt_r = (long long)nt1, nu_r = us[n_grid];
+      for (int cur = 0;;) {
+        S.rounds++;
+        S.tasks += nt_r;
+        k_units<<<units_grid, 256, 0, st>>>(d_tstart[cur], d_ustart[cur], (int)n_grid, slice, d_units,
+                                             d_units + np, d_units + 2 * np);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e0, st));
+        for (long long u0 = 0; u0 < nu_r; u0 += 65535) {
+          const unsigned un = (unsigned)std::min<long long>(65535, nu_r - u0);
+          EvalArgs eb = ea;
+          eb.task_path = d_path[cur];
+          eb.task_ok = d_okr[cur];
+          eb.unit_tile = d_units + u0;
+          eb.unit_begin = d_units + np + u0;
+          eb.unit_end = d_units + 2 * np + u0;
+          if (eval_lds) k_evaluate<true, false><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+          else if (eval_wide) k_evaluate_wide<false><<<dim3(bpt, un), block, 0, st>>>(eb);
+          else k_evaluate<false, false><<<dim3(bpt, un), block, 0, st>>>(eb);
+          HIPCHK(hipGetLastError());
+          S.n_evaluate_launches++;
+        }
+        HIPCHK(hipEventRecord(e1, st));
+        if (!expand) {
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0.0f;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          eval_ms += ms;
+          break;
+        }
+        const int nxt = cur ^ 1;
+        k_expand<<<1024, 256, 0, st>>>(d_path[cur], d_tile[cur], d_okr[cur], d_tstart[cur] + n_grid, (int)n_grid,
+                                        n1g, n_paths, d_traced, d_fresh);
+        k_tile_count<<<(unsigned)n_grid, 256, 0, st>>>(d_fresh, n_paths, d_cnt);
+        k_tile_scan<<<1, 1024, 0, st>>>(d_cnt, (int)n_grid, slice, d_tstart[nxt], d_ustart[nxt], d_info);
+        k_tile_emit<<<(unsigned)n_grid, 256, 0, st>>>(d_fresh, n_paths, d_tstart[nxt], d_path[nxt], d_tile[nxt],
+                                                       d_okr[nxt]);
+        HIPCHK(hipGetLastError());
+        int hi2[2];
+        HIPCHK(read_back(sc, hi2, d_info, 8, st));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        eval_ms += ms;
+        nt_r = hi2[0];
+        nu_r = hi2[1];
+        if (nt_r == 0) break;
+        cur = nxt;
+      }
+      tp.clear();  // the host loop has nothing left to do
+    }
     while (!tp.empty()) {
       S.rounds++;
       size_t nt = tp.size();

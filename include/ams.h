@@ -62,7 +62,14 @@ typedef struct {
                                      backward queries of scenes in global memory walk the
                                      four-wide hierarchy (by default they walk the binary one);
                              bit 7: keep tracing a branch with no reflection left even when it
-                                    can no longer reach the Rx array */
+                                    can no longer reach the Rx array;
+                             bits 12, 13: build the neighbour-expansion rounds on the host
+                                     (bit 12) or with kernels that expand over the traced-pair
+                                     bitmap, group the new tasks by tile and form the units, the
+                                     host reading two counts per round (bit 13); by default the
+                                     kernels for maps above 2^21 (path, tile) pairs and the host
+                                     otherwise (same tasks and results either way); the task
+                                     export and maps beyond 2^27 pairs use the host loop */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
   void *stream;           /* hipStream_t; NULL = default stream */
   uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */

@@ -246,6 +246,44 @@ def test_culling_leaves_results_unchanged(small_room, trace_scene, flags):
     assert sa["fwd_queries"] + sa["bwd_queries"] <= sb["fwd_queries"] + sb["bwd_queries"]
 
 
+@pytest.mark.parametrize("mode", ["fibonacci", "reference", "ms"])
+def test_device_rounds_match_host_rounds(small_room, trace_scene, mode):
+    """The neighbour-expansion rounds built on the device (flags bit 13: the
+    expansion over the traced-pair bitmap, the grouping by tile and the units
+    are kernels; the default for large maps, such as the office's) against
+    the host loop (flags bit 12): the same rounds, tasks, successes and record
+    sets, and H equal to FP64 accumulation order (within a tile the device
+    orders tasks by path, the host by discovery)."""
+    import torch
+
+    from automs_amd import scenes
+    from automs_amd.tracer import TraceScene, trace
+    _flush(f"case: device-built rounds vs host loop ({mode})")
+    if mode == "ms":
+        w = scenes.room_boxes_ms(fib_n=1 << 12, ms_fib_n=1 << 16)["F"]
+        ts = TraceScene(w.scene)
+    else:
+        w, ts = small_room, trace_scene
+    kw = dict(reference_launch=(64, 64, 1, 8)) if mode == "reference" else {}
+    fib = 0 if mode == "reference" else w.fib_n
+    out = {}
+    for f in (8192, 4096):
+        res, st, keys = trace(ts, w.tx, w.rx, w.wave_len, w.max_reflection, fib, w.per_element, w.gridsize, flags=f,
+                              keys_cap=1 << 20, **kw)
+        torch.cuda.synchronize()
+        out[f] = (res.double().cpu().numpy(), _keyset(keys), st)
+    (Ha, ka, sa), (Hb, kb, sb) = out[8192], out[4096]
+    rel = np.linalg.norm(Ha - Hb) / max(np.linalg.norm(Hb), 1e-300)
+    _flush(f"  rounds {sa['rounds']} vs {sb['rounds']}, tasks {sa['tasks']} vs {sb['tasks']}, successes "
+           f"{sa['successes']} vs {sb['successes']}, H rel {rel:.1e}")
+    assert sa["rounds"] == sb["rounds"] and sa["tasks"] == sb["tasks"]
+    assert sa["successes"] == sb["successes"] and sa["chains"] == sb["chains"]
+    assert ka == kb
+    assert rel < 1e-6
+    if mode == "ms":
+        ts.close()
+
+
 @pytest.mark.parametrize("base", [2048, 16, 16 | 2048])
 def test_four_wide_walk_matches_binary(small_room, trace_scene, base):
     """The four-wide hierarchy walk returns the binary walk's hits (flags bit
