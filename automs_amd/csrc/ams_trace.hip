@@ -1132,6 +1132,12 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
 // ------------------------------------------------------------------------
 // backward evaluation
 // ------------------------------------------------------------------------
+// H is accumulated in 64-bit fixed point, 2^56 units per 1 (resolution
+// 1.4e-17, |H| < 2^7); a lane's sum of magnitude kHLimit or more is counted
+// in counters[15] and the trace is refused.
+constexpr double kHScale = 72057594037927936.0;
+constexpr double kHLimit = 64.0;
+
 struct EvalArgs {
   const AmsNode *tnodes;
   int n_tnodes;
@@ -1160,7 +1166,7 @@ struct EvalArgs {
   const int *unit_begin;      // work unit -> first task
   const int *unit_end;        // work unit -> end task
   int items;                  // gs*gs*n_src
-  double *H;                  // [n_src][n_dst][2]
+  long long *H;               // [n_src][n_dst][2], fixed point (kHScale)
   unsigned char *task_ok;
   unsigned long long *counters;  // nodes, tris, spheres, queries, chains, successes
   ams_task_info *info;           // task export of this round (EXPORT instances), [src][task][gs2]
@@ -1421,9 +1427,15 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
     if (any && __lane_id() == __ffsll((long long)any) - 1) a.task_ok[t] = 1;
   }
   if (active && (acc_re != 0.0 || acc_im != 0.0)) {
-    double *hp = a.H + 2 * ((size_t)i * a.n_dst + j);
-    atomicAdd(hp, acc_re);
-    atomicAdd(hp + 1, acc_im);
+    // the lane's FP64 sum in fixed point: integer additions in any order
+    // give the same H, so H does not depend on the order the units finish in
+    unsigned long long *hp = reinterpret_cast<unsigned long long *>(a.H + 2 * ((size_t)i * a.n_dst + j));
+    if (fabs(acc_re) < kHLimit && fabs(acc_im) < kHLimit) {
+      atomicAdd(hp, (unsigned long long)__double2ll_rn(acc_re * kHScale));
+      atomicAdd(hp + 1, (unsigned long long)__double2ll_rn(acc_im * kHScale));
+    } else {
+      atomicAdd(a.counters + 15, 1ull);  // refused on the host
+    }
   }
   wave_add_counters(a.counters, ct);
   unsigned long long v[2] = {nchain, nsucc};
@@ -1648,9 +1660,9 @@ __global__ void k_unique_flags64(const uint64_t *__restrict__ c, size_t n, unsig
   if (k < n) flag[k] = (k == 0) || c[k] != c[k - 1];
 }
 
-__global__ void k_add_h(const double *__restrict__ H64, float *__restrict__ out, size_t n) {
+__global__ void k_add_h(const long long *__restrict__ Hfix, float *__restrict__ out, size_t n) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) out[k] += (float)H64[k];
+  if (k < n) out[k] += (float)((double)Hfix[k] * (1.0 / kHScale));
 }
 
 __global__ void k_unique_flags(const uint64_t *__restrict__ hi, const uint64_t *__restrict__ lo, size_t n,
@@ -2326,9 +2338,9 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   }
   const int n_paths = (int)p_hi.size();
   S.paths = n_paths;
-  double *d_H = (double *)sc.get("H64", sizeof(double) * 2 * (size_t)n_src * n_dst);
+  long long *d_H = (long long *)sc.get("H64", sizeof(long long) * 2 * (size_t)n_src * n_dst);
   if (!d_H) return AMS_ERR_ALLOC;
-  HIPCHK(hipMemsetAsync(d_H, 0, sizeof(double) * 2 * (size_t)n_src * n_dst, st));
+  HIPCHK(hipMemsetAsync(d_H, 0, sizeof(long long) * 2 * (size_t)n_src * n_dst, st));
   HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
   // task export file, created for every call as the reference's path.dat (optixPathTracer.cpp:831)
   std::unique_ptr<FILE, int (*)(FILE *)> info_file(nullptr, &std::fclose);
@@ -2651,13 +2663,18 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     }
     S.t_evaluate_ms = eval_ms;
   }
+  HIPCHK(read_back(sc, hctr, d_ctr, sizeof hctr, st));
+  if (hctr[15]) {  // before the caller's buffer is touched
+    set_error("a channel entry's partial sum reached the fixed-point range of H (|H| >= 64): " +
+              std::to_string(hctr[15]) + " lanes");
+    return AMS_ERR_ARG;
+  }
   // accumulate into the caller's FP32 result, optixPathTracer.cu:259-264
   size_t nH = 2 * (size_t)n_src * n_dst;
   if (nH > 0) {
     k_add_h<<<(unsigned)((nH + 255) / 256), 256, 0, st>>>(d_H, H_out, nH);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(read_back(sc, hctr, d_ctr, sizeof hctr, st));
   S.bwd_nodes = (long long)hctr[0];
   S.bwd_tris = (long long)hctr[1];
   S.bwd_queries = (long long)hctr[3];
