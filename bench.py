@@ -57,13 +57,13 @@ MEASURED_TRAFFIC = {
     ("room_boxes_ms", "k_evaluate"): {"bytes": 850.8 * 2048 + 4.155e4 * 1024, "l2_hit": 0.582},
     ("office", "k_discover"): {"bytes": 2.424e5 * 2048 + 1.976e5 * 1024, "l2_hit": 0.922},
     ("office", "k_evaluate"): {"bytes": 4585 * 2048 + 3.583e5 * 1024, "l2_hit": 0.864},
-    ("office_batched", "k_cgemm"): {"bytes": 1547.4e6 + 134.2e6, "l2_hit": 0.782, "mfma_util": 0.592},
+    ("office_batched", "k_cgemm"): {"bytes": 7.554e5 * 2048 + 1.311e5 * 1024, "l2_hit": 0.782, "mfma_util": 0.589},
 }
 
 
 # VALU wave64 instructions per closest-hit query, from rocprofv3 --pmc
 # SQ_INSTS_VALU over a profiled run divided by that run's queries (the
-# tracer's counters of the same run); profiles/valu_tracer_*.txt.
+# tracer's counters of the same run); profiles/tracer_pmc_*.txt.
 MEASURED_VALU = {
     ("room_boxes_ms", "k_evaluate"): 1.096e9 * 26 / 1517460974,     # 18.78, k_evaluate<true, false>
     ("room_boxes_ms", "k_discover"): 2.284e9 * 6 / 500351026,       # 27.39, k_discover<true>
@@ -72,12 +72,14 @@ MEASURED_VALU = {
 }
 
 
-def pmc_summary(path: str) -> str:
+def pmc_summary(path: str, by_grid: bool = False) -> str:
     """Per-kernel table of the rocprofv3 counter-collection CSV files under
     path: dispatches, mean duration and the mean of every counter per
     dispatch, with the VALU-issue fraction (SQ_INSTS_VALU per cycle per CU
-    at the peak clock), VALUBusy, lane utilisation and wait fraction when
-    those counters are present."""
+    at the peak clock), VALUBusy, lane utilisation, wait fraction and MFMA
+    busy fraction (SQ_VALU_MFMA_BUSY_CYCLES over the GUI-active cycles of
+    1024 SIMDs) when those counters are present.  by_grid: one row per
+    (kernel, grid size), e.g. to tell two launch shapes of a kernel apart."""
     import csv
     import glob
     import re
@@ -88,6 +90,8 @@ def pmc_summary(path: str) -> str:
                 name = re.sub(r"\(anonymous namespace\)::", "", r["Kernel_Name"])
                 name = re.sub(r"\((?!anonymous).*$", "", name).strip()
                 name = re.sub(r"^void ", "", name)
+                if by_grid:
+                    name += f" [grid {r.get('Grid_Size', '?')}]"
                 k = rows.setdefault(name, {"disp": set(), "dur": {}, "c": {}})
                 did = r.get("Dispatch_Id") or r.get("Correlation_Id")
                 k["disp"].add(did)
@@ -113,6 +117,8 @@ def pmc_summary(path: str) -> str:
             line += f"  lanes {c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.3f}"
         if "SQ_WAIT_INST_ANY" in c and c.get("SQ_WAVE_CYCLES"):
             line += f"  wait {c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:.3f}"
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in c and c.get("GRBM_GUI_ACTIVE"):
+            line += f"  MFMA busy {c['SQ_VALU_MFMA_BUSY_CYCLES'] / (c['GRBM_GUI_ACTIVE'] / 8 * 4 * N_CU):.3f}"
         out.append(line)
     return "\n".join(out)
 
@@ -658,6 +664,7 @@ def main():
     ap.add_argument("--office-steps", type=int, default=2)
     ap.add_argument("--office-warmup", type=int, default=1)
     ap.add_argument("--pmc-summary", metavar="DIR", help="print the per-kernel table of rocprofv3 counter CSVs")
+    ap.add_argument("--pmc-by-grid", action="store_true", help="--pmc-summary: one row per kernel and grid size")
     ap.add_argument("--only-office", action="store_true",
                     help="run only the office workload at its stated size and print its object")
     ap.add_argument("--valu-probe", action="store_true",
@@ -666,7 +673,7 @@ def main():
     ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
     args = ap.parse_args()
     if args.pmc_summary:
-        print(pmc_summary(args.pmc_summary))
+        print(pmc_summary(args.pmc_summary, args.pmc_by_grid))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the path shards by Rx tiles "
