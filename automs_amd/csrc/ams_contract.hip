@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) k_cgemm(const u32x4 *__restrict__ A, cons
 // once, into that XCD's L2.
 constexpr int kWgA = 8, kWgB = 4, kWgBlocks = kWgA + kWgB;
 
-template <bool CONJB>
+template <bool CONJB, bool SCHED>
 __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, const u32x4 *__restrict__ B, int Ra,
                                                   int Rb, int n_kb, int row_groups, int col_groups, int split_n,
                                                   float *__restrict__ C, long long ldc, float *__restrict__ part) {
@@ -268,13 +268,26 @@ __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, c
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) b[j][q] = lds[(kWgA + wc * 2 + j) * 256 + q * 64 + lane];
+    // B's imaginary planes negated once per step ((-a) b = a (-b) exactly:
+    // the per-wave kernel's products, bit for bit)
+    bf16x8 nbih[2], nbil[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      nbih[j] = neg_bf(b[j][2]);
+      nbil[j] = neg_bf(b[j][3]);
+    }
+    // A fragments in two register sets: tile i + 1's LDS reads are issued
+    // before tile i's MFMAs
+    u32x4 a_cur[4], a_nxt[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a_cur[q] = lds[(wr * 4) * 256 + q * 64 + lane];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      u32x4 a[4];
+      if (i < 3) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) a[q] = lds[(wr * 4 + i) * 256 + q * 64 + lane];
-      bf16x8 arh = as_bf(a[0]), arl = as_bf(a[1]), aih = as_bf(a[2]), ail = as_bf(a[3]);
-      bf16x8 narh = neg_bf(a[0]), narl = neg_bf(a[1]), naih = neg_bf(a[2]), nail = neg_bf(a[3]);
+        for (int q = 0; q < 4; q++) a_nxt[q] = lds[(wr * 4 + i + 1) * 256 + q * 64 + lane];
+      }
+      bf16x8 arh = as_bf(a_cur[0]), arl = as_bf(a_cur[1]), aih = as_bf(a_cur[2]), ail = as_bf(a_cur[3]);
 #pragma unroll
       for (int j = 0; j < 2; j++) {
         bf16x8 brh = as_bf(b[j][0]), brl = as_bf(b[j][1]), bih = as_bf(b[j][2]), bil = as_bf(b[j][3]);
@@ -287,17 +300,17 @@ __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, c
           re = MFMA(aih, bil, re);
           re = MFMA(ail, bih, re);
         } else {
-          re = MFMA(naih, bih, re);
-          re = MFMA(naih, bil, re);
-          re = MFMA(nail, bih, re);
+          re = MFMA(aih, nbih[j], re);
+          re = MFMA(aih, nbil[j], re);
+          re = MFMA(ail, nbih[j], re);
         }
         im = MFMA(aih, brh, im);
         im = MFMA(aih, brl, im);
         im = MFMA(ail, brh, im);
         if (CONJB) {
-          im = MFMA(narh, bih, im);
-          im = MFMA(narh, bil, im);
-          im = MFMA(narl, bih, im);
+          im = MFMA(arh, nbih[j], im);
+          im = MFMA(arh, nbil[j], im);
+          im = MFMA(arl, nbih[j], im);
         } else {
           im = MFMA(arh, bih, im);
           im = MFMA(arh, bil, im);
@@ -306,7 +319,14 @@ __global__ void __launch_bounds__(256) k_cgemm_wg(const u32x4 *__restrict__ A, c
         acc_re[i][j] = re;
         acc_im[i][j] = im;
       }
+      if (i < 3) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) a_cur[q] = a_nxt[q];
+      }
     }
+    // SCHED: have the scheduler issue the step's LDS reads among its first
+    // MFMAs instead of before each A tile's (the LDS latency is then hidden)
+    if (SCHED) __builtin_amdgcn_iglp_opt(0);
   }
   float *out = splits > 1 ? part + (long long)z * 2 * (long long)Ra * Rb : C;
   long long ld = splits > 1 ? Rb : ldc;
@@ -582,15 +602,17 @@ void launch_cgemm(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, int 
 
 template <bool CJ>
 void launch_cgemm_wg(const u32x4 *A, const u32x4 *B, int Ra, int Rb, int n_kb, int splits, float *C, long long ldc,
-                     float *part, hipStream_t st, bool split_major, bool four_mult) {
+                     float *part, hipStream_t st, bool split_major, bool four_mult, bool sched) {
   int row_groups = ((Ra + 15) / 16 + kWgA - 1) / kWgA;
   int col_groups = ((Rb + 15) / 16 + kWgB - 1) / kWgB;
   const bool sm = split_major && splits >= 8;  // each XCD takes whole splits
   dim3 grid = sm ? dim3((unsigned)(8LL * ((splits + 7) / 8) * row_groups * col_groups), 1, 1)
                  : dim3((unsigned)(8LL * row_groups * ((col_groups + 7) / 8)), 1, (unsigned)splits);
   const int sn = sm ? splits : 0;
-  if (four_mult)
-    k_cgemm_wg<CJ><<<grid, 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, sn, C, ldc, part);
+  if (four_mult && sched)
+    k_cgemm_wg<CJ, true><<<grid, 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, sn, C, ldc, part);
+  else if (four_mult)
+    k_cgemm_wg<CJ, false><<<grid, 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, sn, C, ldc, part);
   else
     k_cgemm_wg3<CJ><<<grid, 256, 0, st>>>(A, B, Ra, Rb, n_kb, row_groups, col_groups, sn, C, ldc, part);
 }
@@ -663,6 +685,7 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
   const bool per_wave = (conj_b & 2) != 0;
   const bool split_major = (conj_b & 4) == 0;
   const bool four_mult = (conj_b & 8) == 0;
+  const bool sched = (conj_b & 16) == 0;
   conj_b &= 1;
   hipStream_t st = (hipStream_t)stream;
   const u32x4 *a = (const u32x4 *)A, *b = (const u32x4 *)B;
@@ -678,8 +701,10 @@ int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t
     if (conj_b) launch_cgemm<4, 2, true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
     else launch_cgemm<4, 2, false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st);
   } else {
-    if (conj_b) launch_cgemm_wg<true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st, split_major, four_mult);
-    else launch_cgemm_wg<false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st, split_major, four_mult);
+    if (conj_b)
+      launch_cgemm_wg<true>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st, split_major, four_mult, sched);
+    else
+      launch_cgemm_wg<false>(a, b, Ra, Rb, n_kb, splits, C, ldc, part, st, split_major, four_mult, sched);
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && splits > 1) {

@@ -49,7 +49,7 @@ def frag_pack(X: torch.Tensor, R: int, Kd: int, sr: int, sk: int) -> torch.Tenso
 
 def cgemm_frag(fa: torch.Tensor, fb: torch.Tensor, Ra: int, Rb: int, Kd: int, conj_b: bool,
                splits: int = 1, per_wave: bool = False, split_minor: bool = False,
-               three_mult: bool = False) -> torch.Tensor:
+               three_mult: bool = False, plain_schedule: bool = False) -> torch.Tensor:
     """C[Ra, Rb] = A B^T (or A conj(B)^T) from fragment-packed operands.
     per_wave selects the per-wave kernel also for Ra >= 128; split_minor and
     three_mult the workgroup kernel's split-minor order and three-product
@@ -61,7 +61,7 @@ def cgemm_frag(fa: torch.Tensor, fb: torch.Tensor, Ra: int, Rb: int, Kd: int, co
         ws = torch.empty(max(1, L.ams_cgemm_workspace_bytes(Ra, Rb, splits)), dtype=torch.uint8, device=fa.device)
     _lib.check(L.ams_cgemm_frag(_ptr(fa), _ptr(fb), Ra, Rb, Kd,
                                 (1 if conj_b else 0) | (2 if per_wave else 0) | (4 if split_minor else 0)
-                                | (8 if three_mult else 0),
+                                | (8 if three_mult else 0) | (16 if plain_schedule else 0),
                                 _ptr(out), Rb, splits,
                                 _ptr(ws) if ws is not None else None, _stream()), "ams_cgemm_frag")
     return out
@@ -96,7 +96,10 @@ class ChannelOperand:
         # backward: rows = stacked index, reduction over Rx
         self.frag_bwd = frag_pack(Fcat, self.Kc, self.N, self.N, 1)
         self.plain = Fcat if keep_plain else None
-        self.bwd_splits = max(1, min(64, (self.N // 32) // 8))
+        # reduction splits of the backward product at M >= 128 (config 3,
+        # N = 65,536, 64 AP beams x 4 maps): 16 measured 0.774 ms against
+        # 0.804 at 64 (less partial-sum traffic) and 0.920 at 8
+        self.bwd_splits = max(1, min(16, (self.N // 32) // 8))
 
     def nbytes(self) -> int:
         return self.frag_fwd.numel() + self.frag_bwd.numel()
