@@ -156,7 +156,8 @@ int ams_frag_pack(const float *X, int32_t R, int32_t Kd, int64_t sr, int64_t sk,
  * order (identical results).  conj_b bit 3: the workgroup kernel forms each complex
  * tile product from three real products (9 MFMAs per tile pair instead of
  * 12, six planes per block staged in LDS; within 1e-5 of the four-product
- * form; measured slower on dense operands). */
+ * form; measured slower on dense operands).  conj_b bit 4: the workgroup
+ * kernel without its MFMA / LDS-read interleaving (identical results). */
 int64_t ams_cgemm_workspace_bytes(int32_t Ra, int32_t Rb, int32_t splits);
 int ams_cgemm_frag(const void *A, const void *B, int32_t Ra, int32_t Rb, int32_t Kd, int32_t conj_b, float *C,
                    int64_t ldc, int32_t splits, void *workspace, void *stream);
