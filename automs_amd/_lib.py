@@ -58,13 +58,15 @@ STAT_INT_FIELDS = [
     "n_discover_launches", "n_evaluate_launches",
 ]
 STAT_DBL_FIELDS = ["t_discover_ms", "t_evaluate_ms", "t_total_ms"]
+STAT_TAIL_FIELDS = ["bwd_walks", "beam_pairs", "beam_nodes", "beam_tris", "beam_walk_legs", "beam_cands"]
 
 
 class AmsTraceStats(C.Structure):
-    _fields_ = [(n, C.c_int64) for n in STAT_INT_FIELDS] + [(n, C.c_double) for n in STAT_DBL_FIELDS]
+    _fields_ = ([(n, C.c_int64) for n in STAT_INT_FIELDS] + [(n, C.c_double) for n in STAT_DBL_FIELDS] +
+                [(n, C.c_int64) for n in STAT_TAIL_FIELDS])
 
     def as_dict(self) -> dict:
-        return {n: getattr(self, n) for n in STAT_INT_FIELDS + STAT_DBL_FIELDS}
+        return {n: getattr(self, n) for n in STAT_INT_FIELDS + STAT_DBL_FIELDS + STAT_TAIL_FIELDS}
 
 
 _lib = None

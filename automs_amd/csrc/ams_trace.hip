@@ -614,6 +614,8 @@ constexpr int kDiscWideRays = 1;
 constexpr int kEvalBlock = 256;
 // records of one round the task export may hold (52 bytes each)
 constexpr size_t kMaxTaskInfo = size_t(1) << 26;
+// candidate lists one evaluation launch may hold
+constexpr size_t kBeamBytes = size_t(1) << 30;
 static_assert(sizeof(ams_task_info) == 52, "ams_task_info keeps the 52-byte layout of the reference's record");
 static_assert(sizeof(AmsNode) == 2 * sizeof(float4) && sizeof(AmsNode4) == 8 * sizeof(float4) &&
                   sizeof(AmsTri) == 3 * sizeof(float4),
@@ -1172,7 +1174,60 @@ struct EvalArgs {
   ams_task_info *info;           // task export of this round (EXPORT instances), [src][task][gs2]
   long long info_nt;             // tasks of this round
   int gs2;                       // Rx per tile
+  // candidate lists of the launch's (task, Rx) pairs (k_beam), or null:
+  // [unit][task in unit][Rx in tile][beam_stride] ints
+  const int *beam;
+  int beam_stride, beam_slice;
 };
+
+// Beam candidate lists.  For a (task, Rx) pair of a path whose interactions
+// are all reflections, every Tx element's backward chain runs leg by leg
+// inside one convex beam per leg: the hull of where the leg starts and ends
+// for the corners of the Tx array's bounding box (the reflection points of
+// an element inside that box lie inside the hull of the corners'
+// reflection points, a central projection of the unfolded images).  k_beam
+// lists, per leg, the triangles that come within a margin of that beam; a
+// lane then finds its leg's closest hit among them instead of walking the
+// hierarchy.  Header: 5 bits per leg (leg 0 = the Rx side), the candidate
+// count or kBeamWalk (walk the hierarchy: too many candidates, a
+// transmission on the path, a beam the geometry does not close).
+constexpr int kBeamCand = 24;
+constexpr unsigned kBeamWalk = 31u;
+constexpr int kBeamBits = 5;
+constexpr int kBeamBlock = 128;
+
+// First int of a (task, Rx) pair's lists: task tl of the launch's unit
+// blockIdx.y, Rx q of the tile.
+__device__ __forceinline__ size_t beam_index(const EvalArgs &a, int tl, int q) {
+  return (((size_t)blockIdx.y * a.beam_slice + tl) * a.gs2 + q) * a.beam_stride;
+}
+
+// Closest hit among a leg's candidate triangles (leaf-order slots), with
+// closest_tri's rules: [tmin, tbest], plane skip_pl ignored, ties on t to the
+// smaller original index.  Every triangle a ray of the beam can hit is a
+// candidate, so the result is the walk's.
+__device__ __forceinline__ int cand_closest(const AmsTri *__restrict__ tris, const int *__restrict__ cand, int n,
+                                            const Ray &r, float tmin, float &tbest, int skip_pl, int &orig_out,
+                                            Counters &ct) {
+  int hit = -1, horig = 0x7fffffff;
+  for (int k = 0; k < n; k++) {
+    const int slot = cand[k];
+    const AmsTri tr = tris[slot];
+    ct.tris++;
+    if (__float_as_int(tr.b.w) == skip_pl) continue;
+    float t;
+    if (tri_test(r, tr.a, tr.b, tr.c, tmin, tbest, t)) {
+      int orig = __float_as_int(tr.a.w);
+      if (t < tbest || orig < horig) {
+        tbest = t;
+        hit = slot;
+        horig = orig;
+      }
+    }
+  }
+  orig_out = horig;
+  return hit;
+}
 
 // Path length of a successful chain along its reflection triangles' own
 // planes, in FP64: the length the chain traced (optixPathTracer.cu:344 sums
@@ -1217,11 +1272,13 @@ __device__ __forceinline__ double own_plane_length(const EvalArgs &a, uint64_t k
 // One lane per (Rx in the tile, Tx element), looping over the tasks of its
 // work unit.  WIDE: the scene query walks the four-wide hierarchy (global
 // memory scenes: fewer dependent L2 loads per query).
-template <bool LDS, bool EXPORT, bool WIDE>
+template <bool LDS, bool EXPORT, bool WIDE, bool BEAM>
 __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
   extern __shared__ float4 ams_smem[];
   // reflection triangles of each lane's current chain
   __shared__ int refl_tri[AMS_MAX_HIT * kEvalBlock];
+  // each lane's candidate-list header of its current chain (BEAM)
+  __shared__ unsigned beam_hdr[BEAM ? kEvalBlock : 1];
   const AmsNode *tnodes = a.tnodes;
   const AmsTri *tris = a.tris;
   if (LDS) {
@@ -1280,6 +1337,8 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
       int ttl = h + 1, ri = 0, last_pl = -1;
       bool inexact = false;  // a reflection on a triangle off its plane id's plane
       nchain++;
+      // the pair's candidate-list header (k_beam), held in LDS
+      if (BEAM) beam_hdr[threadIdx.x] = (unsigned)a.beam[beam_index(a, t - tb, q)];
       // Unfold the path analytically before any query: from the Rx towards
       // the image, every reflection point must lie ahead and inside the
       // bounds of its plane's triangles, and the Tx ahead of the last one.
@@ -1340,8 +1399,17 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
         ray_setup(ray, o, d);
         float tt = tlim;
         int torig;
-        int slot = WIDE ? scene_closest(a.tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct)
-                        : closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
+        int slot;
+        const int leg = h + 1 - ttl;
+        const unsigned nc = BEAM ? (beam_hdr[threadIdx.x] >> (kBeamBits * leg)) & kBeamWalk : kBeamWalk;
+        if (nc != kBeamWalk) {
+          const int *cand = a.beam + beam_index(a, t - tb, q) + 1 + leg * kBeamCand;
+          slot = cand_closest(tris, cand, (int)nc, ray, AMS_TMIN, tt, last_pl, torig, ct);
+        } else {
+          slot = WIDE ? scene_closest(a.tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct)
+                      : closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
+          if (BEAM) ct.sph++;  // a walk of the hierarchy
+        }
         ct.queries++;
         if (slot < 0) tt = INFINITY;
         if (sh && bs < tt && bs <= tlim) {
@@ -1448,16 +1516,318 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
 
 // Backward legs are short bounded segments: the binary walk, whose loop fits
 // the 80 registers of this kernel (the four-wide one spills at that budget).
-template <bool LDS, bool EXPORT>
+template <bool LDS, bool EXPORT, bool BEAM = false>
 __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_evaluate(EvalArgs a) {
-  evaluate_body<LDS, EXPORT, false>(a);
+  evaluate_body<LDS, EXPORT, false, BEAM>(a);
 }
 
 // Scenes in global memory with flags bit 11: the four-wide walk, with the
 // register budget it needs (4 waves per SIMD and up).
 template <bool EXPORT>
 __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_evaluate_wide(EvalArgs a) {
-  evaluate_body<false, EXPORT, true>(a);
+  evaluate_body<false, EXPORT, true, false>(a);
+}
+
+// ---- beam candidate lists -------------------------------------------------
+struct BeamArgs {
+  int *beam;              // [unit][task in unit][Rx in tile][stride]
+  int stride, slice, legs;
+  float hx[8], hy[8], hz[8];  // corners of a box holding the Tx elements (the first 4 for a planar array)
+  float margin0, margin_rel;  // candidate margin: margin0 + margin_rel * path length (m)
+};
+
+// The beam of one leg: NC start and NC end points, their extent along six
+// axes (x, y, z and the leg direction crossed with each), with a margin.
+template <int NC>
+struct BeamLeg {
+  float px[2 * NC], py[2 * NC], pz[2 * NC];
+  float ax[6], ay[6], az[6];
+  float lo[6], hi[6];
+  float cx, cy, cz;  // leg direction (unit)
+  float margin;
+};
+
+template <int NC>
+__device__ __forceinline__ void beam_extent(const BeamLeg<NC> &B, float x, float y, float z, float &mn, float &mx) {
+  mn = INFINITY;
+  mx = -INFINITY;
+#pragma unroll
+  for (int m = 0; m < 2 * NC; m++) {
+    const float v = B.px[m] * x + B.py[m] * y + B.pz[m] * z;
+    mn = fminf(mn, v);
+    mx = fmaxf(mx, v);
+  }
+}
+
+template <int NC>
+__device__ __forceinline__ void beam_axes(BeamLeg<NC> &B) {
+  float sx = 0, sy = 0, sz = 0, ex = 0, ey = 0, ez = 0;
+#pragma unroll
+  for (int m = 0; m < NC; m++) {
+    sx += B.px[m];
+    sy += B.py[m];
+    sz += B.pz[m];
+    ex += B.px[NC + m];
+    ey += B.py[NC + m];
+    ez += B.pz[NC + m];
+  }
+  float cx = ex - sx, cy = ey - sy, cz = ez - sz;
+  const float cl = sqrtf(cx * cx + cy * cy + cz * cz);
+  if (cl > 0.0f) {
+    cx /= cl;
+    cy /= cl;
+    cz /= cl;
+  } else {
+    cx = 1.0f;
+    cy = 0.0f;
+    cz = 0.0f;
+  }
+  B.cx = cx;
+  B.cy = cy;
+  B.cz = cz;
+  // x, y, z, then c x (x, y, z); a near-zero cross product falls back to its axis
+  const float e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    B.ax[k] = e[k][0];
+    B.ay[k] = e[k][1];
+    B.az[k] = e[k][2];
+    float vx = cy * e[k][2] - cz * e[k][1], vy = cz * e[k][0] - cx * e[k][2], vz = cx * e[k][1] - cy * e[k][0];
+    const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
+    if (vl > 0.05f) {
+      vx /= vl;
+      vy /= vl;
+      vz /= vl;
+    } else {
+      vx = e[k][0];
+      vy = e[k][1];
+      vz = e[k][2];
+    }
+    B.ax[3 + k] = vx;
+    B.ay[3 + k] = vy;
+    B.az[3 + k] = vz;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) beam_extent(B, B.ax[k], B.ay[k], B.az[k], B.lo[k], B.hi[k]);
+}
+
+// A node box apart from the beam along one of the six axes (by more than the
+// margin).  NaN compares false: a box is only ever rejected on a finite gap.
+template <int NC>
+__device__ __forceinline__ bool beam_box_apart(const BeamLeg<NC> &B, float4 lo, float4 hi) {
+  const float cx = 0.5f * (lo.x + hi.x), cy = 0.5f * (lo.y + hi.y), cz = 0.5f * (lo.z + hi.z);
+  const float hx = 0.5f * (hi.x - lo.x), hy = 0.5f * (hi.y - lo.y), hz = 0.5f * (hi.z - lo.z);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const float c = cx * B.ax[k] + cy * B.ay[k] + cz * B.az[k];
+    const float r = hx * fabsf(B.ax[k]) + hy * fabsf(B.ay[k]) + hz * fabsf(B.az[k]);
+    if (c - r > B.hi[k] + B.margin || c + r < B.lo[k] - B.margin) return true;
+  }
+  return false;
+}
+
+// Triangle and beam apart along the unit axis (x, y, z), whose beam extent
+// is [bl, bh].
+template <int NC>
+__device__ __forceinline__ bool tri_apart_along(const BeamLeg<NC> &B, float x, float y, float z, float bl, float bh,
+                                                const float3 (&v)[3]) {
+  const float t0 = v[0].x * x + v[0].y * y + v[0].z * z, t1 = v[1].x * x + v[1].y * y + v[1].z * z,
+              t2 = v[2].x * x + v[2].y * y + v[2].z * z;
+  const float tl = fminf(fminf(t0, t1), t2), th = fmaxf(fmaxf(t0, t1), t2);
+  return tl > bh + B.margin || th < bl - B.margin;
+}
+
+// The same along an axis that is not one of the beam's six (normalised here;
+// a degenerate axis separates nothing).
+template <int NC>
+__device__ __forceinline__ bool tri_apart_new(const BeamLeg<NC> &B, float x, float y, float z,
+                                              const float3 (&v)[3]) {
+  const float l = sqrtf(x * x + y * y + z * z);
+  if (!(l > 1e-6f)) return false;
+  x /= l;
+  y /= l;
+  z /= l;
+  float bl, bh;
+  beam_extent(B, x, y, z, bl, bh);
+  return tri_apart_along(B, x, y, z, bl, bh, v);
+}
+
+// A triangle apart from the beam along a separating axis: the six beam axes,
+// the triangle's normal, and the leg direction crossed with each edge.
+// (A subset of the separating-axis test: a pair it cannot separate stays a
+// candidate.)
+template <int NC>
+__device__ __forceinline__ bool beam_tri_apart(const BeamLeg<NC> &B, const AmsTri &tr) {
+  const float3 v[3] = {make_float3(tr.a.x, tr.a.y, tr.a.z), make_float3(tr.b.x, tr.b.y, tr.b.z),
+                       make_float3(tr.c.x, tr.c.y, tr.c.z)};
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+    if (tri_apart_along(B, B.ax[k], B.ay[k], B.az[k], B.lo[k], B.hi[k], v)) return true;
+  const float3 e0 = v[1] - v[0], e1 = v[2] - v[1], e2 = v[0] - v[2];
+  if (tri_apart_new(B, e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x, v))
+    return true;
+  const float3 es[3] = {e0, e1, e2};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float3 e = es[k];
+    if (tri_apart_new(B, B.cy * e.z - B.cz * e.y, B.cz * e.x - B.cx * e.z, B.cx * e.y - B.cy * e.x, v)) return true;
+  }
+  return false;
+}
+
+struct BeamCount {
+  unsigned long long pairs = 0, nodes = 0, tris = 0, walk = 0, cand = 0;
+};
+
+// The candidate lists of one (task, Rx) pair, walked from the Rx as
+// k_evaluate walks them.  Leg l (l < nref) ends on the key's plane
+// nref-1-l; its end points continue past the plane by the evaluation's
+// query bound (0.1% + 1 mm); the last leg ends at the Tx corners.  Triangles
+// of the plane a leg starts on are left out (the lane's query skips that
+// plane).  The corners' images and crossings are formed in FP64 and kept in
+// FP32 between legs (1e-6 relative, far inside the margin).  Returns the
+// header.
+template <int NC>
+__device__ unsigned beam_pair(const EvalArgs &a, const BeamArgs &b, int p, int j, int *out, BeamCount &bc) {
+  const unsigned walk_all = 0xFFFFFFFFu;
+  const uint64_t khi = a.path_hi[p], klo = a.path_lo[p];
+  const int h = (int)(khi >> 61), bm = (int)((khi >> 56) & 31);
+  const int nref = __popc(bm & ((1 << h) - 1));
+  if (h > AMS_MAX_HIT || nref != h || h + 1 > b.legs) return walk_all;  // transmissions: every leg walks
+  bc.pairs++;
+  const float rx = a.dst[3 * j], ry = a.dst[3 * j + 1], rz = a.dst[3 * j + 2];
+  // per corner: the current start of its leg, its remaining image, the path length so far
+  float ox[NC], oy[NC], oz[NC], ix[NC], iy[NC], iz[NC], len[NC];
+#pragma unroll
+  for (int m = 0; m < NC; m++) {
+    double x = b.hx[m], y = b.hy[m], z = b.hz[m];
+    for (int k = 0; k < nref; k++) {
+      const AmsPlane P = a.planes[key_get_plane(khi, klo, k)];
+      const double f = 2.0 * (x * P.n[0] + y * P.n[1] + z * P.n[2] - P.d);
+      x -= f * P.n[0];
+      y -= f * P.n[1];
+      z -= f * P.n[2];
+    }
+    ix[m] = (float)x;
+    iy[m] = (float)y;
+    iz[m] = (float)z;
+    ox[m] = rx;
+    oy[m] = ry;
+    oz[m] = rz;
+    len[m] = 0.0f;
+  }
+  unsigned hdr = 0u;
+  BeamLeg<NC> B;
+  bool closed = true;  // once a leg's beam does not close, every later leg walks
+  for (int l = 0; l <= nref; l++) {
+    float lmax = 0.0f;
+    AmsPlane P;
+    if (l < nref) P = a.planes[key_get_plane(khi, klo, nref - 1 - l)];
+#pragma unroll
+    for (int m = 0; m < NC; m++) {
+      B.px[m] = ox[m];
+      B.py[m] = oy[m];
+      B.pz[m] = oz[m];
+      if (l < nref) {
+        const double sx = ox[m], sy = oy[m], sz = oz[m];
+        const double ux = ix[m] - sx, uy = iy[m] - sy, uz = iz[m] - sz;
+        const double nd = ux * P.n[0] + uy * P.n[1] + uz * P.n[2];
+        const double tp = (P.d - (sx * P.n[0] + sy * P.n[1] + sz * P.n[2])) / nd;
+        if (!(tp > 0.0 && tp < 1.0)) closed = false;  // the plane is not between start and image
+        const double ul = sqrt(ux * ux + uy * uy + uz * uz);
+        const double dist = tp * ul;
+        const double ext = tp + (1e-3 * dist + 1e-3) / ul;  // the evaluation's bound past the plane
+        B.px[NC + m] = (float)(sx + ux * ext);
+        B.py[NC + m] = (float)(sy + uy * ext);
+        B.pz[NC + m] = (float)(sz + uz * ext);
+        len[m] += (float)dist;
+        // the image loses this plane's reflection; the next leg starts at the crossing
+        const double f = 2.0 * ((double)ix[m] * P.n[0] + (double)iy[m] * P.n[1] + (double)iz[m] * P.n[2] - P.d);
+        ix[m] = (float)(ix[m] - f * P.n[0]);
+        iy[m] = (float)(iy[m] - f * P.n[1]);
+        iz[m] = (float)(iz[m] - f * P.n[2]);
+        ox[m] = (float)(sx + ux * tp);
+        oy[m] = (float)(sy + uy * tp);
+        oz[m] = (float)(sz + uz * tp);
+      } else {
+        B.px[NC + m] = b.hx[m];
+        B.py[NC + m] = b.hy[m];
+        B.pz[NC + m] = b.hz[m];
+        const float ux = b.hx[m] - ox[m], uy = b.hy[m] - oy[m], uz = b.hz[m] - oz[m];
+        len[m] += sqrtf(ux * ux + uy * uy + uz * uz);
+      }
+      lmax = fmaxf(lmax, len[m]);
+    }
+    unsigned nc = 0;
+    if (closed) {
+      B.margin = b.margin0 + b.margin_rel * lmax;
+      beam_axes(B);
+      const int skip_pl = l == 0 ? -1 : (int)key_get_plane(khi, klo, nref - l);
+      int *cand = out + 1 + l * kBeamCand;
+      int i = 0;
+      while (i < a.n_tnodes && nc != kBeamWalk) {
+        const float4 *np = reinterpret_cast<const float4 *>(a.tnodes + i);
+        const float4 lo = np[0], hi = np[1];
+        bc.nodes++;
+        const int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
+        if (!beam_box_apart(B, lo, hi)) {
+          if (info == 0) {
+            i++;
+            continue;
+          }
+          const int first = info >> 3, cnt = info & 7;
+          for (int k = 0; k < cnt; k++) {
+            const AmsTri tr = a.tris[first + k];
+            bc.tris++;
+            if (__float_as_int(tr.b.w) == skip_pl || beam_tri_apart(B, tr)) continue;
+            if (nc == (unsigned)kBeamCand) {
+              nc = kBeamWalk;
+              break;
+            }
+            cand[nc++] = first + k;
+          }
+        }
+        i = miss;
+      }
+    } else {
+      nc = kBeamWalk;
+    }
+    if (nc == kBeamWalk) bc.walk++;
+    else bc.cand += nc;
+    hdr |= nc << (kBeamBits * l);
+  }
+  for (int l = nref + 1; l * kBeamBits < 32; l++) hdr |= kBeamWalk << (kBeamBits * l);
+  return hdr;
+}
+
+// One thread per (task, Rx in tile) of a launch's units (blockIdx.y = unit).
+// NC: 4 box corners for a planar Tx array, 8 otherwise.
+template <int NC>
+__global__ void __launch_bounds__(kBeamBlock) __attribute__((amdgpu_waves_per_eu(NC == 4 ? 4 : 2, 8))) k_beam(EvalArgs a, BeamArgs b) {
+  const int unit = blockIdx.y;
+  const int tb = a.unit_begin[unit], te = a.unit_end[unit], g = a.unit_tile[unit];
+  const int gs2 = a.gs * a.gs;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  BeamCount bc;
+  if (idx < (te - tb) * gs2) {
+    const int tl = idx / gs2, q = idx - tl * gs2;
+    int *out = b.beam + (((size_t)unit * b.slice + tl) * gs2 + q) * b.stride;
+    const int gi = g / a.n1g, gj = g % a.n1g;
+    const int di = gi * a.gs + q / a.gs, dj = gj * a.gs + q % a.gs;
+    const int j = di * a.n1_dst + dj;
+    unsigned hdr = 0xFFFFFFFFu;
+    if (j < a.n_dst && (dj < a.n1_dst || (a.flags & 1))) hdr = beam_pair<NC>(a, b, a.task_path[tb + tl], j, out, bc);
+    out[0] = (int)hdr;
+  }
+  // counters 6..10: beam pairs, beam node tests, beam triangle tests, legs
+  // left to the walk, candidates listed
+  unsigned long long v[5] = {bc.pairs, bc.nodes, bc.tris, bc.walk, bc.cand};
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    unsigned long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (__lane_id() == 0 && x) atomicAdd(a.counters + 6 + k, x);
+  }
 }
 
 // ---- neighbour expansion on the device (optixPathTracer.cpp:264-284) ----
@@ -1893,6 +2263,60 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Corners of a box that holds every Tx element: the box's axes follow the
+// array's first row and first column (orthonormalised, FP64), or x, y, z for
+// a single element or a line.  Every element is a convex combination of the
+// eight corners.  Returns true for a planar array (extent below 1 um across
+// the row and column): its four corners at mid-depth come first and hold
+// every element.
+static bool tx_box_corners(const ams_array *tx, float hx[8], float hy[8], float hz[8]) {
+  const int n0 = tx->n0, n1 = tx->n1, n = n0 * n1;
+  const float *P = tx->pos;
+  double ax[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  auto vec = [&](int i, int k, double v[3]) {
+    for (int c = 0; c < 3; c++) v[c] = (double)P[3 * k + c] - (double)P[3 * i + c];
+  };
+  auto len = [](const double v[3]) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+  if (n0 > 1 && n1 > 1) {
+    double u[3], v[3];
+    vec(0, n1 - 1, u);
+    vec(0, (n0 - 1) * n1, v);
+    const double lu = len(u);
+    if (lu > 0) {
+      for (int c = 0; c < 3; c++) u[c] /= lu;
+      const double d = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+      for (int c = 0; c < 3; c++) v[c] -= d * u[c];
+      const double lv = len(v);
+      if (lv > 1e-9 * lu) {
+        for (int c = 0; c < 3; c++) v[c] /= lv;
+        const double w[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        for (int c = 0; c < 3; c++) {
+          ax[0][c] = u[c];
+          ax[1][c] = v[c];
+          ax[2][c] = w[c];
+        }
+      }
+    }
+  }
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < n; i++) {
+    for (int k = 0; k < 3; k++) {
+      const double s = ax[k][0] * P[3 * i] + ax[k][1] * P[3 * i + 1] + ax[k][2] * P[3 * i + 2];
+      lo[k] = std::min(lo[k], s);
+      hi[k] = std::max(hi[k], s);
+    }
+  }
+  const bool planar = hi[2] - lo[2] < 1e-6;
+  for (int m = 0; m < 8; m++) {
+    const double c0 = (m & 1) ? hi[0] : lo[0], c1 = (m & 2) ? hi[1] : lo[1];
+    const double c2 = planar ? 0.5 * (lo[2] + hi[2]) : ((m & 4) ? hi[2] : lo[2]);
+    hx[m] = (float)(c0 * ax[0][0] + c1 * ax[1][0] + c2 * ax[2][0]);
+    hy[m] = (float)(c0 * ax[0][1] + c1 * ax[1][1] + c2 * ax[2][1]);
+    hz[m] = (float)(c0 * ax[0][2] + c1 * ax[1][2] + c2 * ax[2][2]);
+  }
+  return planar;
+}
+
 // Fit c0 + i u + j v to an Rx array (u, v from its corner elements, FP64)
 // and return the lattice query's parameters when every element lies within
 // a quarter of the line spacing of its lattice point.
@@ -1960,6 +2384,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   Scratch &sc = s->scratch;
   ams_trace_stats S;
   memset(&S, 0, sizeof S);
+  bool beams_used = false;  // backward queries answered from beam candidate lists
 
   // Tx / Rx positions and the Rx sphere hierarchy
   float *d_src = (float *)sc.get("src", sizeof(float) * 3 * n_src);
@@ -2437,6 +2862,58 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     // (many Tx elements) to cut the FP64 atomics into H; short ones otherwise
     const int slice = o->tasks_per_unit > 0 ? o->tasks_per_unit : (bpt >= 8 ? 64 : 16);
     float eval_ms = 0.0f;
+    ea.gs2 = gs * gs;
+    ea.beam = nullptr;
+    // Beam candidate lists (k_beam) ahead of every evaluation launch: by
+    // default for Tx arrays of 256 elements and more, whose lanes share one
+    // pair's lists (with 64, the office AP, building the lists cost more
+    // than the walks they save: office H_d 114 against 70 ms); flags bit 14
+    // turns them off, bit 15 on for any Tx array.  A launch covers at most
+    // kBeamBytes of lists.
+    const bool beams = !(o->flags & 16384) && ((o->flags & 32768) || n_src >= 256) && !info_file && !eval_wide;
+    beams_used = beams;
+    BeamArgs ba{};
+    bool beam_planar = false;
+    size_t beam_per_unit = 0;
+    int unit_chunk = 65535;
+    if (beams) {
+      ba.legs = o->max_reflection + 1;
+      ba.stride = 1 + ba.legs * kBeamCand;
+      ba.slice = slice;
+      beam_planar = tx_box_corners(tx, ba.hx, ba.hy, ba.hz);
+      ba.margin0 = 2e-3f;
+      ba.margin_rel = 2e-5f;
+      beam_per_unit = (size_t)slice * gs * gs * ba.stride * sizeof(int);
+      unit_chunk = (int)std::max<size_t>(1, std::min<size_t>(65535, kBeamBytes / beam_per_unit));
+      ea.beam_stride = ba.stride;
+      ea.beam_slice = slice;
+    }
+    // one launch of units [u0, u0 + un) of a round: the lists, then the evaluation
+    auto launch_eval = [&](EvalArgs eb, unsigned un) -> int {
+      if (beams) {
+        ba.beam = (int *)sc.get("beam", beam_per_unit * un);
+        if (!ba.beam) return AMS_ERR_ALLOC;
+        eb.beam = ba.beam;
+        const unsigned bx = (unsigned)((slice * gs * gs + kBeamBlock - 1) / kBeamBlock);
+        if (beam_planar) k_beam<4><<<dim3(bx, un), kBeamBlock, 0, st>>>(eb, ba);
+        else k_beam<8><<<dim3(bx, un), kBeamBlock, 0, st>>>(eb, ba);
+        if (eval_lds) k_evaluate<true, false, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+        else k_evaluate<false, false, true><<<dim3(bpt, un), block, 0, st>>>(eb);
+      } else if (eb.info) {
+        if (eval_lds) k_evaluate<true, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+        else if (eval_wide) k_evaluate_wide<true><<<dim3(bpt, un), block, 0, st>>>(eb);
+        else k_evaluate<false, true><<<dim3(bpt, un), block, 0, st>>>(eb);
+      } else if (eval_lds) {
+        k_evaluate<true, false><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+      } else if (eval_wide) {
+        k_evaluate_wide<false><<<dim3(bpt, un), block, 0, st>>>(eb);
+      } else {
+        k_evaluate<false, false><<<dim3(bpt, un), block, 0, st>>>(eb);
+      }
+      HIPCHK(hipGetLastError());
+      S.n_evaluate_launches++;
+      return AMS_OK;
+    };
     // Rounds built on the device: the expansion over the bitmap of traced
     // pairs, the grouping by tile and the next round's units are kernels; the
     // host reads back two counts per round and launches the next evaluation
@@ -2503,19 +2980,16 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
                                              d_units + np, d_units + 2 * np);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(e0, st));
-        for (long long u0 = 0; u0 < nu_r; u0 += 65535) {
-          const unsigned un = (unsigned)std::min<long long>(65535, nu_r - u0);
+        for (long long u0 = 0; u0 < nu_r; u0 += unit_chunk) {
+          const unsigned un = (unsigned)std::min<long long>(unit_chunk, nu_r - u0);
           EvalArgs eb = ea;
           eb.task_path = d_path[cur];
           eb.task_ok = d_okr[cur];
           eb.unit_tile = d_units + u0;
           eb.unit_begin = d_units + np + u0;
           eb.unit_end = d_units + 2 * np + u0;
-          if (eval_lds) k_evaluate<true, false><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
-          else if (eval_wide) k_evaluate_wide<false><<<dim3(bpt, un), block, 0, st>>>(eb);
-          else k_evaluate<false, false><<<dim3(bpt, un), block, 0, st>>>(eb);
-          HIPCHK(hipGetLastError());
-          S.n_evaluate_launches++;
+          const int rc = launch_eval(eb, un);
+          if (rc != AMS_OK) return rc;
         }
         HIPCHK(hipEventRecord(e1, st));
         if (!expand) {
@@ -2597,25 +3071,14 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         HIPCHK(hipMemsetAsync(ea.info, 0, n_info * sizeof(ams_task_info), st));
       }
       HIPCHK(hipEventRecord(e0, st));
-      for (int u0 = 0; u0 < nu; u0 += 65535) {
-        int un = std::min(65535, nu - u0);
+      for (int u0 = 0; u0 < nu; u0 += unit_chunk) {
+        const unsigned un = (unsigned)std::min(unit_chunk, nu - u0);
         EvalArgs eb = ea;
         eb.unit_tile += u0;
         eb.unit_begin += u0;
         eb.unit_end += u0;
-        if (eb.info) {
-          if (eval_lds) k_evaluate<true, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
-          else if (eval_wide) k_evaluate_wide<true><<<dim3(bpt, un), block, 0, st>>>(eb);
-          else k_evaluate<false, true><<<dim3(bpt, un), block, 0, st>>>(eb);
-        } else if (eval_lds) {
-          k_evaluate<true, false><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
-        } else if (eval_wide) {
-          k_evaluate_wide<false><<<dim3(bpt, un), block, 0, st>>>(eb);
-        } else {
-          k_evaluate<false, false><<<dim3(bpt, un), block, 0, st>>>(eb);
-        }
-        HIPCHK(hipGetLastError());
-        S.n_evaluate_launches++;
+        const int rc = launch_eval(eb, un);
+        if (rc != AMS_OK) return rc;
       }
       HIPCHK(hipEventRecord(e1, st));
       const unsigned char *ok = (const unsigned char *)h_round + up;
@@ -2680,6 +3143,12 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   S.bwd_queries = (long long)hctr[3];
   S.chains = (long long)hctr[4];
   S.successes = (long long)hctr[5];
+  S.bwd_walks = (long long)(beams_used ? hctr[2] : hctr[3]);
+  S.beam_pairs = (long long)hctr[6];
+  S.beam_nodes = (long long)hctr[7];
+  S.beam_tris = (long long)hctr[8];
+  S.beam_walk_legs = (long long)hctr[9];
+  S.beam_cands = (long long)hctr[10];
   S.t_total_ms = ms_since(t_start);
   if (stats) *stats = S;
   if (o->keys_out && o->keys_cap > 0) {

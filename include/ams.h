@@ -69,7 +69,15 @@ typedef struct {
                                      host reading two counts per round (bit 13); by default the
                                      kernels for maps above 2^21 (path, tile) pairs and the host
                                      otherwise (same tasks and results either way); the task
-                                     export and maps beyond 2^27 pairs use the host loop */
+                                     export and maps beyond 2^27 pairs use the host loop;
+                             bits 14, 15: backward queries without beam candidate lists
+                                     (bit 14: every leg walks the hierarchy) or with them for
+                                     any Tx array (bit 15); by default the lists are built for
+                                     Tx arrays of 256 elements and more.  A list holds every
+                                     triangle within a margin of the convex beam that one leg
+                                     of one (path, Rx) pair sweeps over all Tx elements, so a
+                                     lane's closest hit among the candidates is the walk's
+                                     (same results either way) */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
   void *stream;           /* hipStream_t; NULL = default stream */
   uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */
@@ -117,6 +125,10 @@ typedef struct {
   int64_t bwd_queries, bwd_nodes, bwd_tris;
   int64_t n_discover_launches, n_evaluate_launches;
   double t_discover_ms, t_evaluate_ms, t_total_ms; /* kernel times from HIP events; wall total */
+  /* backward queries answered by a walk of the hierarchy (the others from
+   * their beam's candidate list); beam lists: (task, Rx) pairs given lists,
+   * their node and triangle tests, legs left to the walk, candidates listed */
+  int64_t bwd_walks, beam_pairs, beam_nodes, beam_tris, beam_walk_legs, beam_cands;
 } ams_trace_stats;
 
 /* Scene: scene_triangle_buffer [3T,3] f32, mat [T] i32, mat_prop [n_mat,19]
