@@ -1195,6 +1195,9 @@ constexpr int kBeamCand = 24;
 constexpr unsigned kBeamWalk = 31u;
 constexpr int kBeamBits = 5;
 constexpr int kBeamBlock = 128;
+#ifndef AMS_BEAM_WAVES
+#define AMS_BEAM_WAVES 6  // waves per SIMD of the evaluation with candidate lists
+#endif
 
 // First int of a (task, Rx) pair's lists: task tl of the launch's unit
 // blockIdx.y, Rx q of the tile.
@@ -1202,17 +1205,34 @@ __device__ __forceinline__ size_t beam_index(const EvalArgs &a, int tl, int q) {
   return (((size_t)blockIdx.y * a.beam_slice + tl) * a.gs2 + q) * a.beam_stride;
 }
 
+// Wave-uniform reads through the constant address space, so the compiler
+// issues scalar loads (the lists and the scene are not written by the
+// kernels that read them).
+typedef const __attribute__((address_space(4))) int cint4_t;
+typedef const __attribute__((address_space(4))) float cfloat4_t;
+__device__ __forceinline__ int uni_load(const int *p) { return *(cint4_t *)p; }
+__device__ __forceinline__ AmsTri uni_tri(const AmsTri *p) {
+  cfloat4_t *q = (cfloat4_t *)p;
+  AmsTri t;
+  t.a = make_float4(q[0], q[1], q[2], q[3]);
+  t.b = make_float4(q[4], q[5], q[6], q[7]);
+  t.c = make_float4(q[8], q[9], q[10], q[11]);
+  return t;
+}
+
 // Closest hit among a leg's candidate triangles (leaf-order slots), with
 // closest_tri's rules: [tmin, tbest], plane skip_pl ignored, ties on t to the
 // smaller original index.  Every triangle a ray of the beam can hit is a
 // candidate, so the result is the walk's.
+// SCALAR: cand and the triangles are wave-uniform and in global memory.
+template <bool SCALAR>
 __device__ __forceinline__ int cand_closest(const AmsTri *__restrict__ tris, const int *__restrict__ cand, int n,
                                             const Ray &r, float tmin, float &tbest, int skip_pl, int &orig_out,
                                             Counters &ct) {
   int hit = -1, horig = 0x7fffffff;
   for (int k = 0; k < n; k++) {
-    const int slot = cand[k];
-    const AmsTri tr = tris[slot];
+    const int slot = SCALAR ? uni_load(cand + k) : cand[k];
+    const AmsTri tr = SCALAR ? uni_tri(tris + slot) : tris[slot];
     ct.tris++;
     if (__float_as_int(tr.b.w) == skip_pl) continue;
     float t;
@@ -1272,7 +1292,10 @@ __device__ __forceinline__ double own_plane_length(const EvalArgs &a, uint64_t k
 // One lane per (Rx in the tile, Tx element), looping over the tasks of its
 // work unit.  WIDE: the scene query walks the four-wide hierarchy (global
 // memory scenes: fewer dependent L2 loads per query).
-template <bool LDS, bool EXPORT, bool WIDE, bool BEAM>
+// UNI (with BEAM): every wave's lanes share one Rx of the tile (the Tx count
+// is a multiple of 64), so a chain's lists and candidate triangles are
+// wave-uniform: scalar loads, one branch per wave.
+template <bool LDS, bool EXPORT, bool WIDE, bool BEAM, bool UNI>
 __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
   extern __shared__ float4 ams_smem[];
   // reflection triangles of each lane's current chain
@@ -1337,8 +1360,11 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
       int ttl = h + 1, ri = 0, last_pl = -1;
       bool inexact = false;  // a reflection on a triangle off its plane id's plane
       nchain++;
-      // the pair's candidate-list header (k_beam), held in LDS
-      if (BEAM) beam_hdr[threadIdx.x] = (unsigned)a.beam[beam_index(a, t - tb, q)];
+      // the pair's candidate-list header (k_beam): in LDS per lane, or
+      // wave-uniform (UNI)
+      unsigned bhdr_u = kBeamWalk;
+      if (BEAM && UNI) bhdr_u = (unsigned)uni_load(a.beam + beam_index(a, t - tb, __builtin_amdgcn_readfirstlane(q)));
+      else if (BEAM) beam_hdr[threadIdx.x] = (unsigned)a.beam[beam_index(a, t - tb, q)];
       // Unfold the path analytically before any query: from the Rx towards
       // the image, every reflection point must lie ahead and inside the
       // bounds of its plane's triangles, and the Tx ahead of the last one.
@@ -1400,11 +1426,13 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
         float tt = tlim;
         int torig;
         int slot;
-        const int leg = h + 1 - ttl;
-        const unsigned nc = BEAM ? (beam_hdr[threadIdx.x] >> (kBeamBits * leg)) & kBeamWalk : kBeamWalk;
+        const int leg = UNI ? __builtin_amdgcn_readfirstlane(h + 1 - ttl) : h + 1 - ttl;
+        const unsigned nc = !BEAM ? kBeamWalk
+                            : ((UNI ? bhdr_u : beam_hdr[threadIdx.x]) >> (kBeamBits * leg)) & kBeamWalk;
         if (nc != kBeamWalk) {
-          const int *cand = a.beam + beam_index(a, t - tb, q) + 1 + leg * kBeamCand;
-          slot = cand_closest(tris, cand, (int)nc, ray, AMS_TMIN, tt, last_pl, torig, ct);
+          const int *cand =
+              a.beam + beam_index(a, t - tb, UNI ? __builtin_amdgcn_readfirstlane(q) : q) + 1 + leg * kBeamCand;
+          slot = cand_closest<UNI && !LDS>(tris, cand, (int)nc, ray, AMS_TMIN, tt, last_pl, torig, ct);
         } else {
           slot = WIDE ? scene_closest(a.tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct)
                       : closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
@@ -1516,16 +1544,17 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
 
 // Backward legs are short bounded segments: the binary walk, whose loop fits
 // the 80 registers of this kernel (the four-wide one spills at that budget).
-template <bool LDS, bool EXPORT, bool BEAM = false>
-__global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_evaluate(EvalArgs a) {
-  evaluate_body<LDS, EXPORT, false, BEAM>(a);
+template <bool LDS, bool EXPORT, bool BEAM = false, bool UNI = false>
+__global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(BEAM ? AMS_BEAM_WAVES : 6, 8)))
+k_evaluate(EvalArgs a) {
+  evaluate_body<LDS, EXPORT, false, BEAM, UNI>(a);
 }
 
 // Scenes in global memory with flags bit 11: the four-wide walk, with the
 // register budget it needs (4 waves per SIMD and up).
 template <bool EXPORT>
 __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_evaluate_wide(EvalArgs a) {
-  evaluate_body<false, EXPORT, true, false>(a);
+  evaluate_body<false, EXPORT, true, false, false>(a);
 }
 
 // ---- beam candidate lists -------------------------------------------------
@@ -2874,6 +2903,8 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     beams_used = beams;
     BeamArgs ba{};
     bool beam_planar = false;
+    // a wave's 64 lanes share one Rx when the Tx count is a multiple of 64
+    const bool beam_uni = n_src % 64 == 0 && !(o->flags & 65536);
     size_t beam_per_unit = 0;
     int unit_chunk = 65535;
     if (beams) {
@@ -2897,8 +2928,13 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         const unsigned bx = (unsigned)((slice * gs * gs + kBeamBlock - 1) / kBeamBlock);
         if (beam_planar) k_beam<4><<<dim3(bx, un), kBeamBlock, 0, st>>>(eb, ba);
         else k_beam<8><<<dim3(bx, un), kBeamBlock, 0, st>>>(eb, ba);
-        if (eval_lds) k_evaluate<true, false, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
-        else k_evaluate<false, false, true><<<dim3(bpt, un), block, 0, st>>>(eb);
+        if (beam_uni) {
+          if (eval_lds) k_evaluate<true, false, true, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+          else k_evaluate<false, false, true, true><<<dim3(bpt, un), block, 0, st>>>(eb);
+        } else {
+          if (eval_lds) k_evaluate<true, false, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
+          else k_evaluate<false, false, true><<<dim3(bpt, un), block, 0, st>>>(eb);
+        }
       } else if (eb.info) {
         if (eval_lds) k_evaluate<true, true><<<dim3(bpt, un), block, (unsigned)eval_scene_lds, st>>>(eb);
         else if (eval_wide) k_evaluate_wide<true><<<dim3(bpt, un), block, 0, st>>>(eb);

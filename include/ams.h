@@ -77,7 +77,11 @@ typedef struct {
                                      triangle within a margin of the convex beam that one leg
                                      of one (path, Rx) pair sweeps over all Tx elements, so a
                                      lane's closest hit among the candidates is the walk's
-                                     (same results either way) */
+                                     (same results either way);
+                             bit 16: with the lists, read a wave's lists and candidate
+                                     triangles per lane even when every wave's lanes share one
+                                     Rx (by default such reads are wave-uniform scalar loads
+                                     when the Tx count is a multiple of 64; same results) */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
   void *stream;           /* hipStream_t; NULL = default stream */
   uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */

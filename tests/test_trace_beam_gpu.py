@@ -86,15 +86,18 @@ def test_room_reference_rays(room):
 
 
 def test_metasurface_channel_default(gpu):
-    """F of config 1's room (1024 metasurface elements): lists by default."""
+    """F of config 1's room (1024 metasurface elements): lists by default,
+    read with wave-uniform scalar loads (every wave's lanes share one Rx) and
+    per lane (flags bit 16)."""
     from automs_amd import scenes
     from automs_amd.tracer import TraceScene
     w = scenes.room_boxes_ms(fib_n=1 << 12, ms_fib_n=1 << 16)["F"]
     _flush(f"case: beam lists, room_boxes_ms F, {w.tx.n} Tx x {w.rx.n} Rx, default")
     ts = TraceScene(w.scene)
-    beam, walk = _run(ts, w, 0), _run(ts, w, OFF)
+    beam, lane, walk = _run(ts, w, 0), _run(ts, w, 65536), _run(ts, w, OFF)
     ts.close()
     _same(beam, walk, "room_boxes_ms F")
+    _same(lane, walk, "room_boxes_ms F, per-lane list reads")
 
 
 def test_door_plate(gpu):
