@@ -52,11 +52,14 @@ LDS_PEAK_GBS = N_CU * 128 * PEAK_CLOCK_HZ / 1e9
 # in KiB per dispatch, scaled by 2 and 1 (a 1 GiB device copy reads back as
 # 0.5 GiB fetched and 1 GiB written), and the L2 hit rate TCC_HIT / (TCC_HIT
 # + TCC_MISS); profiles/hbm_traffic.md.
+# The evaluation's figures cover each launch pair k_beam + k_evaluate (both
+# inside the timed evaluation events): counter sums over the profiled run's
+# dispatches of both kernels divided by its evaluation launches.
 MEASURED_TRAFFIC = {
     ("room_boxes_ms", "k_discover"): {"bytes": 1.049e5 * 2048 + 1.771e5 * 1024, "l2_hit": 0.945},
-    ("room_boxes_ms", "k_evaluate"): {"bytes": 850.8 * 2048 + 4.155e4 * 1024, "l2_hit": 0.582},
+    ("room_boxes_ms", "k_evaluate"): {"bytes": (6.428e5 * 2048 + 1.6705e6 * 1024) / 26, "l2_hit": 0.749},
     ("office", "k_discover"): {"bytes": 2.424e5 * 2048 + 1.976e5 * 1024, "l2_hit": 0.922},
-    ("office", "k_evaluate"): {"bytes": 4585 * 2048 + 3.583e5 * 1024, "l2_hit": 0.864},
+    ("office", "k_evaluate"): {"bytes": (4.215e7 * 2048 + 8.315e7 * 1024) / 180, "l2_hit": 0.753},
     ("office_batched", "k_cgemm"): {"bytes": 7.554e5 * 2048 + 1.311e5 * 1024, "l2_hit": 0.782, "mfma_util": 0.589},
 }
 
@@ -65,10 +68,20 @@ MEASURED_TRAFFIC = {
 # SQ_INSTS_VALU over a profiled run divided by that run's queries (the
 # tracer's counters of the same run); profiles/tracer_pmc_*.txt.
 MEASURED_VALU = {
-    ("room_boxes_ms", "k_evaluate"): 1.096e9 * 26 / 1517460974,     # 18.78, k_evaluate<true, false>
-    ("room_boxes_ms", "k_discover"): 2.284e9 * 6 / 500351026,       # 27.39, k_discover<true>
-    ("office", "k_evaluate"): 3.218e10 * 157 / 113655616856,        # 44.45, k_evaluate<false, false>
-    ("office", "k_discover"): 7.414e9 * 10 / 1387450191,            # 53.43, k_discover_wide<false>
+    ("room_boxes_ms", "k_evaluate"): (6.745e8 + 1.236e10 + 5.186e8) / 1517460974,   # 8.93, k_beam + k_evaluate
+    ("room_boxes_ms", "k_discover"): 2.284e9 * 6 / 500351026,                       # 27.39, k_discover<true>
+    ("office", "k_evaluate"): (5.083e10 + 1.418e12 + 8.954e10) / 113655616856,      # 13.71, k_beam + k_evaluate
+    ("office", "k_discover"): 7.414e9 * 10 / 1387450191,                            # 53.43, k_discover_wide<false>
+}
+# VALUBusy (SQ_ACTIVE_INST_VALU per CU over the GUI-active cycles) of the
+# same runs: the share of cycles the vector ALUs are busy, whatever each
+# instruction's issue cost (FP64, transcendental and 64-bit operations take
+# several cycles).
+MEASURED_VALU_BUSY = {
+    ("room_boxes_ms", "k_evaluate"): 0.761,
+    ("room_boxes_ms", "k_discover"): 0.742,
+    ("office", "k_evaluate"): 0.912,
+    ("office", "k_discover"): 0.669,
 }
 
 
@@ -123,6 +136,58 @@ def pmc_summary(path: str, by_grid: bool = False) -> str:
     return "\n".join(out)
 
 
+def pmc_segments(parent: str) -> str:
+    """Tracer kernels of a --profile-pass run, split into its room part and its
+    office part (the office starts at the first k_discover_wide dispatch), from
+    the four counter runs parent/pmc_valu, pmc_fetch, pmc_write and pmc_l2:
+    per kernel the dispatches, mean duration, VALU instructions per dispatch,
+    VALU issue fraction, VALUBusy, lane utilisation, wait fraction, HBM MB
+    fetched and written per dispatch (FETCH_SIZE x 2, WRITE_SIZE x 1) and the
+    L2 hit rate."""
+    import collections
+    import csv
+    import glob
+    import re
+
+    def load(path):
+        rows = []
+        for fn in glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn) as f:
+                rows += list(csv.DictReader(f))
+
+        def did(r):
+            return int(r.get("Dispatch_Id") or r.get("Correlation_Id"))
+        first_office = min((did(r) for r in rows if "k_discover_wide" in r["Kernel_Name"]), default=1 << 62)
+        agg = collections.defaultdict(lambda: collections.defaultdict(float))
+        dur = collections.defaultdict(dict)
+        for r in rows:
+            name = re.sub(r"\(anonymous namespace\)::", "", r["Kernel_Name"])
+            name = re.sub(r"^void ", "", re.sub(r"\((?!anonymous).*$", "", name).strip())
+            if not name.startswith(("k_beam", "k_evaluate", "k_discover")):
+                continue
+            key = ("office" if did(r) >= first_office else "room", name)
+            agg[key][r["Counter_Name"]] += float(r["Counter_Value"])
+            dur[key][did(r)] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+        return agg, dur
+
+    runs = {p: load(os.path.join(parent, "pmc_" + p)) for p in ("valu", "fetch", "write", "l2")}
+    agg, dur = runs["valu"]
+    out = []
+    for key in sorted(agg):
+        c, n, t = agg[key], len(dur[key]), sum(dur[key].values())
+        fe = runs["fetch"][0][key]["FETCH_SIZE"] * 2048 / n / 1e6
+        wr = runs["write"][0][key]["WRITE_SIZE"] * 1024 / n / 1e6
+        h = runs["l2"][0][key]
+        out.append(f"{key[0]:6s} {key[1]:38s} disp {n:4d} avg {t / n * 1e3:8.3f} ms  "
+                   f"VALU/disp {c['SQ_INSTS_VALU'] / n:.4g}  "
+                   f"issue {c['SQ_INSTS_VALU'] / (t * PEAK_CLOCK_HZ * N_CU * VALU_ISSUE_PER_CU_CYCLE):.3f}  "
+                   f"VALUBusy {c['SQ_ACTIVE_INST_VALU'] / N_CU / (c['GRBM_GUI_ACTIVE'] / 8):.3f}  "
+                   f"lanes {c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.3f}  "
+                   f"wait {c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:.3f}  fetch {fe:.1f} MB  write {wr:.1f} MB  "
+                   f"L2 hit {h['TCC_HIT'] / (h['TCC_HIT'] + h['TCC_MISS']):.3f}")
+    return "\n".join(out)
+
+
 def valu_roofline(workload: str, kernel: str, queries: float, launches: int, kernel_ms: float,
                   alg_bytes: float, lds_staged: bool) -> dict:
     """The roofline object of a tracer kernel: VALU instruction issue (the
@@ -134,8 +199,11 @@ def valu_roofline(workload: str, kernel: str, queries: float, launches: int, ker
     tr = MEASURED_TRAFFIC.get((workload, kernel), {})
     out = {"kernel": kernel, "bound": "valu", "unit": "Ginst/s", "peak": VALU_PEAK_GINST,
            "avg_launch_ms": per_launch_s * 1e3, "queries_per_launch": queries / max(1, launches),
-           "valu_inst_per_query": ipq,
+           "valu_inst_per_query": ipq, "valu_busy_pmc": MEASURED_VALU_BUSY.get((workload, kernel)),
            "achieved": None, "frac": None, "traffic": tr.get("bytes"), "l2_hit_rate": tr.get("l2_hit")}
+    if kernel == "k_evaluate":
+        out["launch"] = ("k_beam (candidate lists, Tx arrays of 256 elements and more) + k_evaluate, timed together "
+                         "by the evaluation events")
     if ipq is not None:
         out["achieved"] = ipq * queries / max(1, launches) / per_launch_s / 1e9
         out["frac"] = out["achieved"] / VALU_PEAK_GINST
@@ -665,6 +733,8 @@ def main():
     ap.add_argument("--office-warmup", type=int, default=1)
     ap.add_argument("--pmc-summary", metavar="DIR", help="print the per-kernel table of rocprofv3 counter CSVs")
     ap.add_argument("--pmc-by-grid", action="store_true", help="--pmc-summary: one row per kernel and grid size")
+    ap.add_argument("--pmc-segments", metavar="DIR",
+                    help="tracer kernels of a --profile-pass run from DIR/pmc_{valu,fetch,write,l2}, room and office")
     ap.add_argument("--only-office", action="store_true",
                     help="run only the office workload at its stated size and print its object")
     ap.add_argument("--valu-probe", action="store_true",
@@ -672,6 +742,9 @@ def main():
     ap.add_argument("--cpu-sample-fib", type=int, default=1 << 18)
     ap.add_argument("--cpu-sample-ms-fib", type=int, default=1 << 16)
     args = ap.parse_args()
+    if args.pmc_segments:
+        print(pmc_segments(args.pmc_segments))
+        return
     if args.pmc_summary:
         print(pmc_summary(args.pmc_summary, args.pmc_by_grid))
         return
