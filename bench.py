@@ -220,6 +220,17 @@ def valu_roofline(workload: str, kernel: str, queries: float, launches: int, ker
     return out
 
 
+def step_valu(workload: str, tot: dict, seconds: float) -> dict:
+    """VALU issue of the whole timed region: the discovery and evaluation
+    kernels' instructions (measured per query) over its wall time, i.e. the
+    GPU's VALU use with the concurrent traces sharing it."""
+    ins = (MEASURED_VALU[(workload, "k_discover")] * tot["fwd_queries"] +
+           MEASURED_VALU[(workload, "k_evaluate")] * tot["bwd_queries"])
+    return {"achieved": ins / seconds / 1e9, "frac": ins / seconds / 1e9 / VALU_PEAK_GINST,
+            "what": "VALU instructions of every discovery and evaluation launch of the timed steps "
+                    "(per-query counts from the profiled run) over the steps' wall time"}
+
+
 def kernel_bytes(stats: dict) -> dict:
     """Algorithmic bytes per kernel: 32 B per BVH node visit, 48 B per
     triangle test, 16 B per sphere test, 64 B of ray / hit I/O per query."""
@@ -668,6 +679,7 @@ def office_full(steps: int, warmup: int, cpu: bool, profile: bool = False) -> di
         r_iso = valu_roofline("office", dom, iso[q_of[dom]], iso[l_of[dom]], iso[t_of[dom]], ikb[dom], False)
         roof["isolated"] = {k: r_iso[k] for k in ("avg_launch_ms", "achieved", "frac")}
         roof["isolated"]["step_seconds"] = (iso["t_total_ms"]) * 1e-3
+    roof["step"] = step_valu("office", tot, dt)
     n_channels = sum(w.tx.n * w.rx.n for w in tr.values())
     out = {
         "workload": "office, 100,046 triangles, <= 5 reflections, 8 x 8 AP, 3 metasurfaces of 48 x 48, "
@@ -861,6 +873,7 @@ def main():
     roof = valu_roofline(args.workload, dom, tot[q_of[dom]], kl[dom], kt[dom], kb[dom], lds)
     r_iso = valu_roofline(args.workload, dom, iso[q_of[dom]], ikl[dom], ikt[dom], ikb[dom], lds)
     roof["isolated"] = {k: r_iso.get(k) for k in ("avg_launch_ms", "achieved", "frac")}
+    roof["step"] = step_valu(args.workload, tot, dt)
     roof["streams"] = 1 if args.sequential else len(traces)
     roof["note"] = ("live durations are those of the timed steps, whose traces run concurrently on separate "
                     "streams and share the GPU; isolated = one untimed step with the traces in turn")
