@@ -766,9 +766,33 @@ __device__ __forceinline__ void discover_rays(const DiscoverArgs &a, const int r
             slot = tri_slot;
             torig = tri_orig;
           } else {
+            // With no reflection left the scene is pass-through: a scene hit
+            // changes what follows only when the ray's origin then lies in a
+            // reception sphere (optixPathTracer.cu:440 rejects that sphere),
+            // i.e. inside the Rx lattice's slab.  Such a branch queries the
+            // scene on the part of the ray in the slab only (flags bit 18
+            // queries the whole ray); before and after it the pass-through
+            // hits move the origin and nothing else.
+            float tlo = AMS_TMIN;
             tt = INFINITY;
-            slot = WIDE ? scene_closest(tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct)
-                        : closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+            if (cur.ttl == 1 && a.lat.on && !(a.flags & 262144)) {
+              float ta = AMS_TMIN, tb = 1e30f, i0, di, j0, dj;
+              if (lattice_segment(a.lat, cur.o, cur.d, ta, tb, i0, di, j0, dj)) {
+                tlo = ta;
+                tt = tb;
+              } else {
+                tlo = 1.0f;  // the ray misses the slab: nothing to query
+                tt = 0.0f;
+              }
+            }
+            if (tlo <= tt) {
+              slot = WIDE ? scene_closest(tnodes4, tnodes, a.n_tnodes, tris, ray, tlo, tt, cur.last_pl, torig, ct)
+                          : closest_tri(tnodes, a.n_tnodes, tris, ray, tlo, tt, cur.last_pl, torig, ct);
+            } else {
+              slot = -1;
+              torig = 0;
+              tt = INFINITY;
+            }
           }
           float ts = tt;
           int rx = rx_query(a, ray, ts, ct);

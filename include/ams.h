@@ -81,7 +81,11 @@ typedef struct {
                              bit 16: with the lists, read a wave's lists and candidate
                                      triangles per lane even when every wave's lanes share one
                                      Rx (by default such reads are wave-uniform scalar loads
-                                     when the Tx count is a multiple of 64; same results) */
+                                     when the Tx count is a multiple of 64; same results);
+                             bit 18: a discovery branch with no reflection left queries the
+                                     scene along the whole ray (by default only where the ray is
+                                     in the Rx lattice's slab: elsewhere its pass-through hits
+                                     cannot change the records; same records) */
   int32_t tasks_per_unit; /* tasks one workgroup loops over (0 = default 16) */
   void *stream;           /* hipStream_t; NULL = default stream */
   uint64_t *keys_out;     /* optional HOST buffer [keys_cap][2]: unique (path, tile) keys */
