@@ -56,9 +56,9 @@ LDS_PEAK_GBS = N_CU * 128 * PEAK_CLOCK_HZ / 1e9
 # inside the timed evaluation events): counter sums over the profiled run's
 # dispatches of both kernels divided by its evaluation launches.
 MEASURED_TRAFFIC = {
-    ("room_boxes_ms", "k_discover"): {"bytes": 1.049e5 * 2048 + 1.771e5 * 1024, "l2_hit": 0.945},
+    ("room_boxes_ms", "k_discover"): {"bytes": (280.2 + 256.7) * 1e6, "l2_hit": 0.934},
     ("room_boxes_ms", "k_evaluate"): {"bytes": (6.428e5 * 2048 + 1.6705e6 * 1024) / 26, "l2_hit": 0.749},
-    ("office", "k_discover"): {"bytes": 2.424e5 * 2048 + 1.976e5 * 1024, "l2_hit": 0.922},
+    ("office", "k_discover"): {"bytes": (486.1 + 154.8) * 1e6, "l2_hit": 0.916},
     ("office", "k_evaluate"): {"bytes": (4.215e7 * 2048 + 8.315e7 * 1024) / 180, "l2_hit": 0.753},
     ("office_batched", "k_cgemm"): {"bytes": 7.554e5 * 2048 + 1.311e5 * 1024, "l2_hit": 0.782, "mfma_util": 0.589},
 }
@@ -69,9 +69,9 @@ MEASURED_TRAFFIC = {
 # tracer's counters of the same run); profiles/tracer_pmc_*.txt.
 MEASURED_VALU = {
     ("room_boxes_ms", "k_evaluate"): (6.745e8 + 1.236e10 + 5.186e8) / 1517460974,   # 8.93, k_beam + k_evaluate
-    ("room_boxes_ms", "k_discover"): 2.284e9 * 6 / 500351026,                       # 27.39, k_discover<true>
+    ("room_boxes_ms", "k_discover"): 2.078e9 * 6 / 491237478,                       # 25.38, k_discover<true>
     ("office", "k_evaluate"): (5.083e10 + 1.418e12 + 8.954e10) / 113655616856,      # 13.71, k_beam + k_evaluate
-    ("office", "k_discover"): 7.414e9 * 10 / 1387450191,                            # 53.43, k_discover_wide<false>
+    ("office", "k_discover"): 6.68e9 * 10 / 1352819735,                             # 49.38, k_discover_wide<false>
 }
 # VALUBusy (SQ_ACTIVE_INST_VALU per CU over the GUI-active cycles) of the
 # same runs: the share of cycles the vector ALUs are busy, whatever each
@@ -79,9 +79,9 @@ MEASURED_VALU = {
 # several cycles).
 MEASURED_VALU_BUSY = {
     ("room_boxes_ms", "k_evaluate"): 0.761,
-    ("room_boxes_ms", "k_discover"): 0.742,
+    ("room_boxes_ms", "k_discover"): 0.710,
     ("office", "k_evaluate"): 0.912,
-    ("office", "k_discover"): 0.669,
+    ("office", "k_discover"): 0.621,
 }
 
 
