@@ -2979,13 +2979,13 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     // host reads back two counts per round and launches the next evaluation
     // with its exact grid (HIP has no launch from the device, and a unit loop
     // inside the evaluation kernel cost it about 24 more spilled registers).
-    // The default for pair maps above 2^21 pairs, where the host's
-    // expansion work is long (office H_d: 77.3 against 79.5 ms per trace);
-    // for small maps the host loop below is shorter (the room step: 32.3
-    // against 33.4 ms).  Flags bit 13 forces the device rounds, bit 12 the
-    // host loop; the task export and maps beyond 2^27 pairs use the host loop.
-    const bool dev_rounds = !info_file && traced_bits_on && n_pairs <= (1ull << 27) &&
-                            ((o->flags & 8192) || (!(o->flags & 4096) && n_pairs > (1ull << 21)));
+    // The default: with candidate lists the evaluation launches are short
+    // and the host's round trips show (the room step 25.0-25.7 against
+    // 25.8-26.4 ms with the host loop; office H_d 77.3 against 79.5 ms per
+    // trace).  Flags bit 12 forces the host loop (bit 13, the device rounds,
+    // is the default); the task export and maps beyond 2^27 pairs use the
+    // host loop.
+    const bool dev_rounds = !info_file && traced_bits_on && n_pairs <= (1ull << 27) && !(o->flags & 4096);
     if (dev_rounds && !tp.empty()) {
       const size_t np = (size_t)n_pairs, words = (size_t)((n_pairs + 31) / 32);
       const size_t ng1 = (size_t)n_grid + 1;

@@ -752,6 +752,8 @@ def main():
     ap.add_argument("--sequential", action="store_true",
                     help="trace H_d, G and F one after the other instead of concurrently (tracer.trace_many)")
     ap.add_argument("--no-contract", action="store_true")
+    ap.add_argument("--trace-flags", type=int, default=0,
+                    help="ams_trace_opts.flags for every trace of the timed workload (A/B of code paths)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short run for rocprof counter passes: 1 traced step, 5 contraction GEMMs and a "
                          "1 GiB device copy that calibrates FETCH_SIZE / WRITE_SIZE")
@@ -836,7 +838,7 @@ def main():
         for k, w in traces.items():
             outs[k].zero_()
             _, st, _ = trace(handles[k], w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
-                             w.gridsize, result=outs[k])
+                             w.gridsize, result=outs[k], flags=args.trace_flags)
             for f, v in st.items():
                 agg[f] = agg.get(f, 0) + v
         return agg
@@ -848,14 +850,15 @@ def main():
             for k, w in traces.items():
                 outs[k].zero_()
                 _, st, _ = trace(handles[k], w.tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element,
-                                 w.gridsize, result=outs[k])
+                                 w.gridsize, result=outs[k], flags=args.trace_flags)
                 sts.append(st)
         else:
             for k in traces:
                 outs[k].zero_()
             res = trace_many([dict(ts=handles[k], tx=w.tx, rx=w.rx, wave_len=w.wave_len,
                                    max_reflection=w.max_reflection, fib_n=w.fib_n, per_element=w.per_element,
-                                   gridsize=w.gridsize, result=outs[k]) for k, w in traces.items()])
+                                   gridsize=w.gridsize, result=outs[k], flags=args.trace_flags)
+                              for k, w in traces.items()])
             sts = [r[1] for r in res]
         for st in sts:
             for f, v in st.items():

@@ -66,10 +66,9 @@ typedef struct {
                              bits 12, 13: build the neighbour-expansion rounds on the host
                                      (bit 12) or with kernels that expand over the traced-pair
                                      bitmap, group the new tasks by tile and form the units, the
-                                     host reading two counts per round (bit 13); by default the
-                                     kernels for maps above 2^21 (path, tile) pairs and the host
-                                     otherwise (same tasks and results either way); the task
-                                     export and maps beyond 2^27 pairs use the host loop;
+                                     host reading two counts per round (bit 13, the default; same
+                                     tasks and results either way); the task export and maps
+                                     beyond 2^27 (path, tile) pairs use the host loop;
                              bits 14, 15: backward queries without beam candidate lists
                                      (bit 14: every leg walks the hierarchy) or with them for
                                      any Tx array (bit 15); by default the lists are built for
