@@ -518,6 +518,39 @@ __device__ __forceinline__ bool dedup_first(DedupTable &t, uint64_t hi, uint64_t
   return true;
 }
 
+// Launch counters: kCtrSlots slots of kCtrN 64-bit counters, one 128-byte
+// line per slot.  A wave adds into the slot of its global wave index, so
+// the waves of a launch do not queue their atomics on one line (a launch of
+// 10^8 short waves otherwise spends seconds there); the host sums the slots.
+constexpr int kCtrN = 16, kCtrSlots = 256;
+__device__ __forceinline__ unsigned long long *ctr_slot(unsigned long long *c) {
+  const unsigned w = ((blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  return c + kCtrN * (w & (kCtrSlots - 1));
+}
+
+// Counters 0..N-1 of a workgroup's waves (at most WAVES) summed in LDS: one
+// atomic per counter and workgroup, into the workgroup's slot.  Every
+// thread of the workgroup must call it (a barrier).
+template <int N, int WAVES>
+__device__ __forceinline__ void block_add_counters(unsigned long long *c, const unsigned long long (&v)[N]) {
+  __shared__ unsigned long long part[WAVES][N];
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    unsigned long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (__lane_id() == 0) part[w][k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    unsigned long long x = 0;
+#pragma unroll
+    for (int q = 0; q < WAVES; q++)
+      if (q < (int)(blockDim.x >> 6)) x += part[q][threadIdx.x];  // the block's own waves
+    if (x) atomicAdd(c + kCtrN * ((blockIdx.y * gridDim.x + blockIdx.x) & (kCtrSlots - 1)) + threadIdx.x, x);
+  }
+}
+
 __device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const Counters &c) {
   unsigned long long v[4] = {(unsigned long long)c.nodes, (unsigned long long)c.tris, (unsigned long long)c.sph,
                              (unsigned long long)c.queries};
@@ -853,10 +886,10 @@ __device__ __forceinline__ void discover_rays(const DiscoverArgs &a, const int r
       }
     }
   }
-  wave_add_counters(a.counters, ct);
+  wave_add_counters(ctr_slot(a.counters), ct);
   unsigned long long x = nrec;
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-  if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
+  if (__lane_id() == 0 && x) atomicAdd(ctr_slot(a.counters) + 4, x);
 }
 
 // Forward discovery (optixPathTracer.cu:142-190, one sample per ray): each
@@ -1000,10 +1033,10 @@ k_discover_persistent(DiscoverArgs a, unsigned long long *next_ray) {
       }
     }
   }
-  wave_add_counters(a.counters, ct);
+  wave_add_counters(ctr_slot(a.counters), ct);
   unsigned long long x = nrec;
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-  if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
+  if (__lane_id() == 0 && x) atomicAdd(ctr_slot(a.counters) + 4, x);
 }
 
 // ---- reference-compatible discovery (optixPathTracer.cu:142-190) ----------
@@ -1149,10 +1182,10 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
       }
     }
   }
-  wave_add_counters(a.counters, ct);
+  wave_add_counters(ctr_slot(a.counters), ct);
   unsigned long long x = nrec;
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-  if (__lane_id() == 0 && x) atomicAdd(a.counters + 4, x);
+  if (__lane_id() == 0 && x) atomicAdd(ctr_slot(a.counters) + 4, x);
 }
 
 // ------------------------------------------------------------------------
@@ -1554,16 +1587,11 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
       atomicAdd(hp, (unsigned long long)__double2ll_rn(acc_re * kHScale));
       atomicAdd(hp + 1, (unsigned long long)__double2ll_rn(acc_im * kHScale));
     } else {
-      atomicAdd(a.counters + 15, 1ull);  // refused on the host
+      atomicAdd(ctr_slot(a.counters) + 15, 1ull);  // refused on the host
     }
   }
-  wave_add_counters(a.counters, ct);
-  unsigned long long v[2] = {nchain, nsucc};
-  for (int k = 0; k < 2; k++) {
-    unsigned long long x = v[k];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    if (__lane_id() == 0 && x) atomicAdd(a.counters + 4 + k, x);
-  }
+  unsigned long long v[6] = {ct.nodes, ct.tris, ct.sph, ct.queries, nchain, nsucc};
+  block_add_counters<6, kEvalBlock / 64>(a.counters, v);
 }
 
 // Backward legs are short bounded segments: the binary walk, whose loop fits
@@ -1879,7 +1907,7 @@ __global__ void __launch_bounds__(kBeamBlock) __attribute__((amdgpu_waves_per_eu
   for (int k = 0; k < 5; k++) {
     unsigned long long x = v[k];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    if (__lane_id() == 0 && x) atomicAdd(a.counters + 6 + k, x);
+    if (__lane_id() == 0 && x) atomicAdd(ctr_slot(a.counters) + 6 + k, x);
   }
 }
 
@@ -2254,6 +2282,18 @@ static hipError_t read_back(Scratch &sc, void *dst, const void *src, size_t byte
   return e;
 }
 
+// The launch counters summed over their slots.
+static hipError_t read_counters(Scratch &sc, unsigned long long *hctr, const unsigned long long *d_ctr,
+                                hipStream_t st) {
+  std::vector<unsigned long long> all((size_t)kCtrN * kCtrSlots);
+  hipError_t e = read_back(sc, all.data(), d_ctr, sizeof(unsigned long long) * all.size(), st);
+  for (int k = 0; k < kCtrN; k++) {
+    hctr[k] = 0;
+    for (int s = 0; s < kCtrSlots; s++) hctr[k] += all[(size_t)s * kCtrN + k];
+  }
+  return e;
+}
+
 static int sort_unique(Scratch &sc, uint64_t *hi, uint64_t *lo, size_t n, hipStream_t st, size_t *n_out,
                        KeyBits kb = KeyBits{0, 0}) {
   if (n == 0) {
@@ -2478,11 +2518,11 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   }
   AmsNode *d_snodes = (AmsNode *)sc.get("snodes", sizeof(AmsNode) * SB.nodes.size());
   float4 *d_sph = (float4 *)sc.get("sph", sizeof(float4) * sph.size());
-  unsigned long long *d_ctr = (unsigned long long *)sc.get("ctr", sizeof(unsigned long long) * 16);
+  unsigned long long *d_ctr = (unsigned long long *)sc.get("ctr", sizeof(unsigned long long) * kCtrN * kCtrSlots);
   if (!d_snodes || !d_sph || !d_ctr) return AMS_ERR_ALLOC;
   HIPCHK(hipMemcpyAsync(d_snodes, SB.nodes.data(), sizeof(AmsNode) * SB.nodes.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_sph, sph.data(), sizeof(float4) * sph.size(), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
+  HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * kCtrN * kCtrSlots, st));
   float4 *d_sph_o = nullptr;
   if (lat.on) {
     std::vector<float4> sph_o(n_dst);
@@ -2796,8 +2836,8 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   S.rays = n_rays;
   S.records_unique = (long long)n_unique_total;
   S.t_discover_ms = disc_ms;
-  unsigned long long hctr[16];
-  HIPCHK(read_back(sc, hctr, d_ctr, sizeof hctr, st));
+  unsigned long long hctr[kCtrN];
+  HIPCHK(read_counters(sc, hctr, d_ctr, st));
   S.fwd_nodes = (long long)hctr[0];
   S.fwd_tris = (long long)hctr[1];
   S.fwd_spheres = (long long)hctr[2];
@@ -2819,7 +2859,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   long long *d_H = (long long *)sc.get("H64", sizeof(long long) * 2 * (size_t)n_src * n_dst);
   if (!d_H) return AMS_ERR_ALLOC;
   HIPCHK(hipMemsetAsync(d_H, 0, sizeof(long long) * 2 * (size_t)n_src * n_dst, st));
-  HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * 16, st));
+  HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long) * kCtrN * kCtrSlots, st));
   // task export file, created for every call as the reference's path.dat (optixPathTracer.cpp:831)
   std::unique_ptr<FILE, int (*)(FILE *)> info_file(nullptr, &std::fclose);
   if (o->task_info_path) {
@@ -3186,7 +3226,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     }
     S.t_evaluate_ms = eval_ms;
   }
-  HIPCHK(read_back(sc, hctr, d_ctr, sizeof hctr, st));
+  HIPCHK(read_counters(sc, hctr, d_ctr, st));
   if (hctr[15]) {  // before the caller's buffer is touched
     set_error("a channel entry's partial sum reached the fixed-point range of H (|H| >= 64): " +
               std::to_string(hctr[15]) + " lanes");
