@@ -1979,41 +1979,54 @@ __global__ void __launch_bounds__(256) k_tile_count(const unsigned *__restrict__
   if (threadIdx.x == 0) tile_cnt[g] = total;
 }
 
-// Exclusive prefix over the tiles (one workgroup of 1024): tile_start and
-// unit_start of the next round; info = {tasks, units}.
-__global__ void __launch_bounds__(1024) k_tile_scan(const int *__restrict__ tile_cnt, int n_grid, int slice,
-                                                    int *__restrict__ tile_start, int *__restrict__ unit_start,
-                                                    int *__restrict__ info) {
-  __shared__ int st[1024], su[1024];
-  const int t = threadIdx.x, chunk = (n_grid + 1023) / 1024;
+// Exclusive prefix over the tiles (one workgroup of 256: a workgroup of
+// 1024 waited up to 2 ms for a free CU while concurrent traces filled the
+// GPU): tile_start and unit_start of the next round; info = {tasks, units}.
+// Thread t sums a run of consecutive tiles, the runs are scanned per wave
+// and across the four waves.
+__global__ void __launch_bounds__(256) k_tile_scan(const int *__restrict__ tile_cnt, int n_grid, int slice,
+                                                   int *__restrict__ tile_start, int *__restrict__ unit_start,
+                                                   int *__restrict__ info) {
+  __shared__ int wt[4], wu[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, chunk = (n_grid + 255) / 256;
   const int g0 = min(n_grid, t * chunk), g1 = min(n_grid, g0 + chunk);
   int ct = 0, cu = 0;
   for (int g = g0; g < g1; g++) {
-    ct += tile_cnt[g];
-    cu += (tile_cnt[g] + slice - 1) / slice;
+    const int c = tile_cnt[g];
+    ct += c;
+    cu += (c + slice - 1) / slice;
   }
-  st[t] = ct;
-  su[t] = cu;
+  int it = ct, iu = cu;  // inclusive scans within the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const int xt = __shfl_up(it, off), xu = __shfl_up(iu, off);
+    if (lane >= off) {
+      it += xt;
+      iu += xu;
+    }
+  }
+  if (lane == 63) {
+    wt[w] = it;
+    wu[w] = iu;
+  }
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int xt = t >= off ? st[t - off] : 0, xu = t >= off ? su[t - off] : 0;
-    __syncthreads();
-    st[t] += xt;
-    su[t] += xu;
-    __syncthreads();
+  int bt = 0, bu = 0;
+  for (int q = 0; q < w; q++) {
+    bt += wt[q];
+    bu += wu[q];
   }
-  int rt = st[t] - ct, ru = su[t] - cu;
+  int rt = bt + it - ct, ru = bu + iu - cu;
   for (int g = g0; g < g1; g++) {
+    const int c = tile_cnt[g];
     tile_start[g] = rt;
     unit_start[g] = ru;
-    rt += tile_cnt[g];
-    ru += (tile_cnt[g] + slice - 1) / slice;
+    rt += c;
+    ru += (c + slice - 1) / slice;
   }
-  if (t == 1023) {
-    tile_start[n_grid] = st[1023];
-    unit_start[n_grid] = su[1023];
-    info[0] = st[1023];
-    info[1] = su[1023];
+  if (t == 255) {
+    tile_start[n_grid] = bt + it;
+    unit_start[n_grid] = bu + iu;
+    info[0] = bt + it;
+    info[1] = bu + iu;
   }
 }
 
@@ -3103,7 +3116,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
         k_expand<<<1024, 256, 0, st>>>(d_path[cur], d_tile[cur], d_okr[cur], d_tstart[cur] + n_grid, (int)n_grid,
                                         n1g, n_paths, d_traced, d_fresh);
         k_tile_count<<<(unsigned)n_grid, 256, 0, st>>>(d_fresh, n_paths, d_cnt);
-        k_tile_scan<<<1, 1024, 0, st>>>(d_cnt, (int)n_grid, slice, d_tstart[nxt], d_ustart[nxt], d_info);
+        k_tile_scan<<<1, 256, 0, st>>>(d_cnt, (int)n_grid, slice, d_tstart[nxt], d_ustart[nxt], d_info);
         k_tile_emit<<<(unsigned)n_grid, 256, 0, st>>>(d_fresh, n_paths, d_tstart[nxt], d_path[nxt], d_tile[nxt],
                                                        d_okr[nxt]);
         HIPCHK(hipGetLastError());

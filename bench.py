@@ -56,10 +56,10 @@ LDS_PEAK_GBS = N_CU * 128 * PEAK_CLOCK_HZ / 1e9
 # inside the timed evaluation events): counter sums over the profiled run's
 # dispatches of both kernels divided by its evaluation launches.
 MEASURED_TRAFFIC = {
-    ("room_boxes_ms", "k_discover"): {"bytes": (280.2 + 256.7) * 1e6, "l2_hit": 0.934},
-    ("room_boxes_ms", "k_evaluate"): {"bytes": (6.428e5 * 2048 + 1.6705e6 * 1024) / 26, "l2_hit": 0.749},
-    ("office", "k_discover"): {"bytes": (486.1 + 154.8) * 1e6, "l2_hit": 0.916},
-    ("office", "k_evaluate"): {"bytes": (4.215e7 * 2048 + 8.315e7 * 1024) / 180, "l2_hit": 0.753},
+    ("room_boxes_ms", "k_discover"): {"bytes": (1.7303e6 * 2048 + 3.0772e6 * 1024) / 6, "l2_hit": 0.890},
+    ("room_boxes_ms", "k_evaluate"): {"bytes": (6.3085e5 * 2048 + 1.5605e6 * 1024) / 26, "l2_hit": 0.734},
+    ("office", "k_discover"): {"bytes": (2.3729e6 * 2048 + 1.5978e6 * 1024) / 10, "l2_hit": 0.916},
+    ("office", "k_evaluate"): {"bytes": (4.4055e7 * 2048 + 9.4723e7 * 1024) / 180, "l2_hit": 0.820},
     ("office_batched", "k_cgemm"): {"bytes": 7.554e5 * 2048 + 1.311e5 * 1024, "l2_hit": 0.782, "mfma_util": 0.589},
 }
 
@@ -68,20 +68,20 @@ MEASURED_TRAFFIC = {
 # SQ_INSTS_VALU over a profiled run divided by that run's queries (the
 # tracer's counters of the same run); profiles/tracer_pmc_*.txt.
 MEASURED_VALU = {
-    ("room_boxes_ms", "k_evaluate"): (6.745e8 + 1.236e10 + 5.186e8) / 1517460974,   # 8.93, k_beam + k_evaluate
-    ("room_boxes_ms", "k_discover"): 2.078e9 * 6 / 491237478,                       # 25.38, k_discover<true>
-    ("office", "k_evaluate"): (5.083e10 + 1.418e12 + 8.954e10) / 113655616856,      # 13.71, k_beam + k_evaluate
-    ("office", "k_discover"): 6.68e9 * 10 / 1352819735,                             # 49.38, k_discover_wide<false>
+    ("room_boxes_ms", "k_evaluate"): 1.35665e10 / 1517460974,                       # 8.94, k_beam + k_evaluate
+    ("room_boxes_ms", "k_discover"): 1.24663e10 / 491237478,                        # 25.38, k_discover<true>
+    ("office", "k_evaluate"): 1.55694e12 / 113655676056,                            # 13.70, k_beam + k_evaluate
+    ("office", "k_discover"): 6.67732e10 / 1352819735,                              # 49.36, k_discover_wide<false>
 }
 # VALUBusy (SQ_ACTIVE_INST_VALU per CU over the GUI-active cycles) of the
 # same runs: the share of cycles the vector ALUs are busy, whatever each
 # instruction's issue cost (FP64, transcendental and 64-bit operations take
 # several cycles).
 MEASURED_VALU_BUSY = {
-    ("room_boxes_ms", "k_evaluate"): 0.761,
-    ("room_boxes_ms", "k_discover"): 0.710,
-    ("office", "k_evaluate"): 0.912,
-    ("office", "k_discover"): 0.621,
+    ("room_boxes_ms", "k_evaluate"): 0.853,
+    ("room_boxes_ms", "k_discover"): 0.758,
+    ("office", "k_evaluate"): 1.005,
+    ("office", "k_discover"): 0.670,
 }
 
 
@@ -185,6 +185,51 @@ def pmc_segments(parent: str) -> str:
                    f"lanes {c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.3f}  "
                    f"wait {c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:.3f}  fetch {fe:.1f} MB  write {wr:.1f} MB  "
                    f"L2 hit {h['TCC_HIT'] / (h['TCC_HIT'] + h['TCC_MISS']):.3f}")
+    return "\n".join(out)
+
+
+def kt_segments(parent: str, warmup: int, steps: int, traces: int = 3) -> str:
+    """Tracer kernels of a rocprofv3 --kernel-trace run of this script, split
+    by the first workload's discovery launches (`traces` per step) into the
+    timed steps (after `warmup` warm-up steps) and the isolated step that
+    follows them: per kernel the dispatches, mean duration and time per
+    step, and the evaluation (k_beam + k_evaluate) per k_evaluate dispatch,
+    the unit of the bench line's live evaluation figure."""
+    import collections
+    import csv
+    import glob
+    import re
+
+    recs = []
+    for fn in glob.glob(os.path.join(parent, "**", "*kernel_trace.csv"), recursive=True):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                name = re.sub(r"\(anonymous namespace\)::", "", r["Kernel_Name"])
+                name = re.sub(r"^void ", "", re.sub(r"\((?!anonymous).*$", "", name).strip())
+                if name.startswith(("k_beam", "k_evaluate", "k_discover")):
+                    recs.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), name))
+    recs.sort()
+    disc_name = next(n for _, _, n in recs if n.startswith("k_discover"))
+    disc = [t for t, _, n in recs if n == disc_name]
+    k0, k1, k2 = traces * warmup, traces * (warmup + steps), traces * (warmup + steps + 1)
+    if len(disc) < k2:
+        raise SystemExit(f"{len(disc)} {disc_name} dispatches: fewer than {k2} (warm-up, timed and isolated steps)")
+    segs = {"timed": (disc[k0], disc[k1], steps), "isolated": (disc[k1], disc[k2] if len(disc) > k2 else 1 << 62, 1)}
+    out = []
+    for seg, (lo, hi, n_steps) in segs.items():
+        agg = collections.defaultdict(list)
+        for t0, t1, n in recs:
+            if lo <= t0 < hi:
+                agg[n].append((t1 - t0) * 1e-6)
+        for n in sorted(agg):
+            d = agg[n]
+            out.append(f"{seg:8s} {n:40s} dispatches {len(d):4d}  avg {sum(d) / len(d):8.3f} ms  "
+                       f"per step {sum(d) / n_steps:8.3f} ms")
+        ev = [x for n in agg if n.startswith("k_evaluate") for x in agg[n]]
+        bm = [x for n in agg if n.startswith("k_beam") for x in agg[n]]
+        if ev:
+            out.append(f"{seg:8s} {'evaluation (k_beam + k_evaluate)':40s} dispatches {len(ev):4d}  "
+                       f"avg {(sum(ev) + sum(bm)) / len(ev):8.3f} ms  per step {(sum(ev) + sum(bm)) / n_steps:8.3f} ms")
     return "\n".join(out)
 
 
@@ -770,6 +815,11 @@ def main():
     ap.add_argument("--pmc-by-grid", action="store_true", help="--pmc-summary: one row per kernel and grid size")
     ap.add_argument("--pmc-segments", metavar="DIR",
                     help="tracer kernels of a --profile-pass run from DIR/pmc_{valu,fetch,write,l2}, room and office")
+    ap.add_argument("--kt-segments", metavar="DIR",
+                    help="tracer kernels of a --kernel-trace run from DIR, split into the timed steps (--warmup, "
+                         "--steps) and the isolated step by the discovery launches, --kt-traces per step")
+    ap.add_argument("--kt-traces", type=int, default=3,
+                    help="--kt-segments: traces per step (3 for room_boxes_ms, 10 for the office)")
     ap.add_argument("--only-office", action="store_true",
                     help="run only the office workload at its stated size and print its object")
     ap.add_argument("--valu-probe", action="store_true",
@@ -779,6 +829,9 @@ def main():
     args = ap.parse_args()
     if args.pmc_segments:
         print(pmc_segments(args.pmc_segments))
+        return
+    if args.kt_segments:
+        print(kt_segments(args.kt_segments, args.warmup, args.steps, args.kt_traces))
         return
     if args.pmc_summary:
         print(pmc_summary(args.pmc_summary, args.pmc_by_grid))
