@@ -106,13 +106,22 @@ class ChannelOperand:
 
 
 def split_count(Ra: int, Rb: int, Kd: int, cap: int = 64) -> int:
-    """Reduction splits that give a short-and-wide product about 512 waves
-    of 32 x 64 output tiles (a few rows of codes against thousands of Rx
-    leave the GPU's 256 CUs idle otherwise); 1 when the tiles alone do.
-    Measured at 30 x 4096 x 1040: forward 49 us unsplit, 16 us at 8 or 16
-    splits; backward (30 x 1040 x 4096) 176 us unsplit, 17-18 us at 24-48."""
+    """Reduction splits of a short-and-wide product on the per-wave kernel
+    (a few rows of codes against thousands of columns leave most of the
+    GPU's 1024 SIMDs idle otherwise), in waves of 32 x 64 output tiles:
+    about 512 waves for short reductions, and for long ones (200 and more
+    32-deep blocks) the most splits that keep the waves at or below 1024,
+    one per SIMD (just past that, the last waves run as a tail).  Measured:
+    30 x 4096 x 1040 forward 49 us unsplit, 19 us at 8 splits; backward
+    (30 x 1040 x 4096) 176 us unsplit, 19-21 us at 24-30.  30 x 18900 x
+    32036 forward 1.72 ms unsplit, 0.81 ms at 3 splits (1.10 at 4);
+    backward 1.12 ms, 0.81 at 2 (0.93 at 3).  30 x 6976 x 65536 backward
+    0.98 ms at 4 splits, 0.65 at 9 (0.79 at 12).  30 x 6912 x 6912: 0.35
+    ms unsplit, 0.079 at 8 (0.110 at 10)."""
     tiles = ((Ra + 31) // 32) * ((Rb + 63) // 64)
-    return max(1, min(cap, (Kd + 31) // 32, 512 // max(1, tiles)))
+    n_kb = (Kd + 31) // 32
+    target = 1024 if n_kb >= 200 else 512
+    return max(1, min(cap, n_kb, target // max(1, tiles)))
 
 
 class Contract(torch.autograd.Function):
