@@ -531,6 +531,7 @@ class PhaseMapModel(nn.Module):
         self.op = ChannelOperand(F_list, channels["src_to_dst"], keep_plain=plain)
         self.loss_list: List[float] = []
         self.rowmask_fn = None
+        self._eff_op, self._eff_key = None, None
 
     def stacked_operand(self) -> torch.Tensor:
         """A [n_maps * C, Kc] = [ A_0 | A_1 | ... | W ] per map, where A_b
@@ -566,7 +567,31 @@ class PhaseMapModel(nn.Module):
             self.ms_theta[i].copy_(torch.remainder(self.ms_theta[i], ms.max_phase_rad))
         return self.forward()
 
+    def frozen_operand(self) -> Optional[ChannelOperand]:
+        """With one metasurface whose phases are not optimised (one map; the
+        reference freezes them for its "MS not optimized" model,
+        physical_exp_nni_model.py:136), the metasurface adds a constant to
+        the channel: H_eff = H_d + G diag(u) F, formed on the same GEMM once
+        per value of the phases (their tensor's version counter), and the
+        codebook is optimised against it.  None otherwise."""
+        if not (self.fused_operand and self.n_maps == 1 and not self.ms_theta[0].requires_grad):
+            return None
+        th = self.ms_theta[0]
+        key = (th.data_ptr(), th._version)
+        if self._eff_key != key:
+            with torch.no_grad():
+                u = self.ms_A[0] * torch.exp(1j * torch.remainder(th[0], self.ms_list[0].max_phase_rad))
+                M = self.G_list[0].shape[0]
+                A = torch.cat([self.G_list[0] * u, torch.eye(M, dtype=torch.complex64, device=th.device)], 1)
+                self._eff_op = ChannelOperand([], Contract.apply(A.to(torch.complex64), self.op))
+            self._eff_key = key
+        return self._eff_op
+
     def received(self) -> torch.Tensor:
+        eff = self.frozen_operand()
+        if eff is not None:
+            W = self.antenna_A * torch.exp(-1j * self.antenna_theta)  # go.py:163
+            return Contract.apply(W.to(torch.complex64), eff)
         if self.fused_operand:
             A = PhaseOperand.apply(self.antenna_theta, self.ms_theta[0], self.antenna_A, self.ms_A[0],
                                    self.G_list[0], self.ms_list[0].max_phase_rad)
@@ -585,7 +610,7 @@ class PhaseMapModel(nn.Module):
             rowmask = (torch.rand(self.code_num * self.n_maps) < 0.1) if self.rowmask_fn is None \
                 else self.rowmask_fn()
         # last_cov: mean over Rx of the best code's power (dB), per map
-        if self.fused_step:
+        if self.fused_step and self.frozen_operand() is None:
             loss, self.last_cov = PhaseStep.apply(self.antenna_theta, self.ms_theta[0], self.antenna_A,
                                                   self.ms_A[0], self.G_list[0], self.ms_list[0].max_phase_rad,
                                                   self.op, self.mode, self.n0, self.w, rowmask)

@@ -398,6 +398,49 @@ def test_objective_gradient_fragments_equal_pack(gpu, mode):
     assert nz > 0 and torch.equal(fg, ref)
 
 
+@pytest.mark.parametrize("objective", ["logsnr", "capacity"])
+def test_frozen_metasurface_channel(gpu, objective):
+    """Metasurface phases not optimised (requires_grad False, one map): the
+    model forms H_eff = H_d + G diag(u) F once and optimises the codebook
+    against it.  Loss within 1e-4 relative and the codebook gradient within
+    2e-3 in norm of the complex128 oracle; the operand is reused while the
+    phases keep their value and formed again after an in-place change."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel
+    from oracle import contract_ref as R
+    _flush(f"case: frozen metasurface channel objective={objective}")
+    Hd, G, F = _channels(M=36, K=256, N=1024, seed=31)
+    ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+          "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+    ms = [Metasurface(None, (16, 16), None, None, [], max_phase=300.0)]
+    th_ms = torch.rand(1, 256, generator=torch.Generator().manual_seed(9)) * 2 * math.pi
+    model = PhaseMapModel("t", ch, ms, code_num=30, w=0.5, objective=objective, ms_theta=[th_ms])
+    model.ms_theta[0].requires_grad_(False)
+    mask = torch.rand(30, generator=torch.Generator().manual_seed(4)) < 0.1
+    model.rowmask_fn = lambda: mask
+    for rnd in range(2):
+        if rnd == 1:
+            with torch.no_grad():
+                model.ms_theta[0].add_(0.7)  # an in-place change: the operand is formed again
+        model.antenna_theta.grad = None
+        loss = model()
+        op = model.frozen_operand()
+        assert op is not None and model.frozen_operand() is op
+        loss.backward()
+        g_ap = model.antenna_theta.grad.cpu().double()
+        ta = model.antenna_theta.detach().cpu().double().requires_grad_(True)
+        tm = model.ms_theta[0].detach().cpu().double().reshape(-1)
+        Y = R.received(model.antenna_A.cpu(), ta, Hd, [G], [F], [tm], [300 / 180 * math.pi])
+        lo = R.loss_logsnr(Y, 1, model.n0) if objective == "logsnr" else R.loss_capacity(Y, model.n0, 0.5, mask)
+        lo.backward()
+        rel_l = abs(loss.item() - lo.item()) / abs(lo.item())
+        rel_ga = (g_ap - ta.grad).norm() / ta.grad.norm()
+        _flush(f"  phases {rnd}: loss gpu={loss.item():.6f} oracle={lo.item():.6f} rel={rel_l:.2e} "
+               f"grad_ap rel={rel_ga:.2e}")
+        assert rel_l < 1e-4
+        assert rel_ga < 2e-3
+        assert model.ms_theta[0].grad is None
+
+
 @pytest.mark.parametrize("objective,n_maps,codebook", [("logsnr", 1, True), ("capacity", 2, True),
                                                        ("logsnr", 3, False)])
 def test_phase_step_equals_separate_functions(gpu, objective, n_maps, codebook):
