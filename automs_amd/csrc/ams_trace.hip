@@ -40,7 +40,11 @@ __device__ __forceinline__ float3 operator+(float3 a, float3 b) { return f3(a.x 
 __device__ __forceinline__ float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ float3 operator*(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float3 norm3(float3 a) { return a * rsqrtf(dot3(a, a)); }
+// v_rsq_f32 directly: rsqrtf adds a rescaling for denormal arguments, which
+// squared lengths of direction vectors (about 1) never are; for every normal
+// argument the result is the same.
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float3 norm3(float3 a) { return a * rsq(dot3(a, a)); }
 
 // Per-ray state of the watertight triangle test (Woop, Benthin, Wald,
 // "Watertight Ray/Triangle Intersection", JCGT 2013).
@@ -409,7 +413,7 @@ __device__ __forceinline__ int closest_sphere_lattice(const RxLattice &L, const 
 
 __device__ __forceinline__ int theta_bin(float3 d, float3 N) {
   // optixPathTracer.cu:304-306
-  float c = dot3(d, N) * rsqrtf(dot3(N, N) * dot3(d, d));
+  float c = dot3(d, N) * rsq(dot3(N, N) * dot3(d, d));
   float th = acosf(fminf(1.0f, fabsf(c)));
   int id = (int)((th / (float)(M_PI / 2) * 36.0f + 1.0f) / 2.0f);
   return id > 18 ? 18 : id;
@@ -418,7 +422,7 @@ __device__ __forceinline__ int theta_bin(float3 d, float3 N) {
 // optixPathTracer.cu:107-136
 __device__ __forceinline__ float beam_pattern(float3 a, float3 b, int type) {
   if (type == 0) return 1.0f;
-  float c = dot3(a, b) * rsqrtf(dot3(a, a) * dot3(b, b));
+  float c = dot3(a, b) * rsq(dot3(a, a) * dot3(b, b));
   switch (type) {
   case 1:
     return c < 0.0f ? 0.0f : c * 2.4495f;
@@ -1267,7 +1271,22 @@ __device__ __forceinline__ size_t beam_index(const EvalArgs &a, int tl, int q) {
 // kernels that read them).
 typedef const __attribute__((address_space(4))) int cint4_t;
 typedef const __attribute__((address_space(4))) float cfloat4_t;
+typedef const __attribute__((address_space(4))) double cdouble4_t;
+typedef const __attribute__((address_space(4))) uint64_t cu64_4_t;
 __device__ __forceinline__ int uni_load(const int *p) { return *(cint4_t *)p; }
+__device__ __forceinline__ float4 uni_f4(const float4 *p) {
+  cfloat4_t *q = (cfloat4_t *)p;
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ AmsPlane uni_plane(const AmsPlane *p) {
+  cdouble4_t *q = (cdouble4_t *)p;
+  AmsPlane P;
+  P.n[0] = q[0];
+  P.n[1] = q[1];
+  P.n[2] = q[2];
+  P.d = q[3];
+  return P;
+}
 __device__ __forceinline__ AmsTri uni_tri(const AmsTri *p) {
   cfloat4_t *q = (cfloat4_t *)p;
   AmsTri t;
@@ -1392,14 +1411,15 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
     int x_hits = 0;
     int x_idx[AMS_MAX_HIT] = {0, 0, 0, 0, 0};
     if (active) {
-      int p = a.task_path[t];
-      uint64_t khi = a.path_hi[p], klo = a.path_lo[p];
+      const int p = UNI ? uni_load(a.task_path + t) : a.task_path[t];
+      const uint64_t khi = UNI ? *(cu64_4_t *)(a.path_hi + p) : a.path_hi[p];
+      const uint64_t klo = UNI ? *(cu64_4_t *)(a.path_lo + p) : a.path_lo[p];
       int h = (int)(khi >> 61), bm = (int)((khi >> 56) & 31);
       int nref = __popc(bm & ((1 << h) - 1));
       // image of the Tx element across the reflection planes (FP64)
       double3 I = make_double3(s3.x, s3.y, s3.z);
       for (int k = 0; k < nref; k++) {
-        const AmsPlane P = a.planes[key_get_plane(khi, klo, k)];
+        const AmsPlane P = UNI ? uni_plane(a.planes + key_get_plane(khi, klo, k)) : a.planes[key_get_plane(khi, klo, k)];
         double s = 2.0 * (I.x * P.n[0] + I.y * P.n[1] + I.z * P.n[2] - P.d);
         I.x -= s * P.n[0];
         I.y -= s * P.n[1];
@@ -1432,7 +1452,7 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
         bool geo = true;
         for (int q = 0; q < nref && geo; q++) {
           const uint32_t pid = key_get_plane(khi, klo, nref - 1 - q);
-          const float4 P = a.planes_f[pid];
+          const float4 P = UNI ? uni_f4(a.planes_f + pid) : a.planes_f[pid];
           float3 pn = f3(P.x, P.y, P.z);
           float nd = dot3(pn, pd);
           // 1-ulp reciprocal: far inside the check's 0.1 mm / 1 mm slack
@@ -1442,7 +1462,13 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
             break;
           }
           float3 X = po + pd * tp;
-          const AmsPlaneBox bx = a.pbox[pid];
+          AmsPlaneBox bx;
+          if (UNI) {
+            bx.lo = uni_f4(&a.pbox[pid].lo);
+            bx.hi = uni_f4(&a.pbox[pid].hi);
+          } else {
+            bx = a.pbox[pid];
+          }
           float mg = 1e-3f + 1e-6f * fmaxf(fmaxf(fabsf(X.x), fabsf(X.y)), fabsf(X.z));
           if (X.x < bx.lo.x - mg || X.x > bx.hi.x + mg || X.y < bx.lo.y - mg || X.y > bx.hi.y + mg ||
               X.z < bx.lo.z - mg || X.z > bx.hi.z + mg) {
@@ -1471,7 +1497,8 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
           tlim = bs;
         } else if ((bm >> (h + 1 - ttl)) & 1) {
           if (ri >= nref) break;
-          const float4 P = a.planes_f[key_get_plane(khi, klo, nref - 1 - ri)];
+          const uint32_t pid = key_get_plane(khi, klo, nref - 1 - ri);
+          const float4 P = UNI ? uni_f4(a.planes_f + pid) : a.planes_f[pid];
           float3 pn = f3(P.x, P.y, P.z);
           float nd = dot3(pn, d);
           float tp = nd != 0.0f ? (P.w - dot3(pn, o)) * __builtin_amdgcn_rcpf(nd) : -1.0f;
