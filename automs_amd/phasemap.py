@@ -573,7 +573,9 @@ class PhaseMapModel(nn.Module):
         physical_exp_nni_model.py:136), the metasurface adds a constant to
         the channel: H_eff = H_d + G diag(u) F, formed on the same GEMM once
         per value of the phases (their tensor's version counter), and the
-        codebook is optimised against it.  None otherwise."""
+        codebook is optimised against it.  A step captured in a HIP graph
+        replays the channel formed at capture: set the phases before
+        optimize_model(graph=True).  None otherwise."""
         if not (self.fused_operand and self.n_maps == 1 and not self.ms_theta[0].requires_grad):
             return None
         th = self.ms_theta[0]
