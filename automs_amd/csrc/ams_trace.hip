@@ -707,15 +707,27 @@ struct SceneView {
   const AmsNode4 *tnodes4;
   const AmsTri *tris;
 };
-template <bool LDS>
+// WIDE: 1 the four-wide walk (a.tnodes4 set), 0 the binary walk (a.tnodes4
+// null), -1 either, by a.tnodes4.  With the walk known at compile time the
+// staged nodes are known to lie in LDS (LDS loads, not flat ones).
+template <bool LDS, int WIDE = -1>
 __device__ __forceinline__ SceneView discovery_scene(const DiscoverArgs &a, float4 *smem) {
   SceneView v{a.tnodes, a.tnodes4, a.tris};
   if (LDS) {
-    const int n4 = a.tnodes4 ? 8 * a.n_tnodes4 : 2 * a.n_tnodes;
-    stage_scene(a.tnodes4 ? reinterpret_cast<const float4 *>(a.tnodes4) : reinterpret_cast<const float4 *>(a.tnodes),
-                n4, a.tris, a.n_tris, smem);
-    if (a.tnodes4) v.tnodes4 = reinterpret_cast<const AmsNode4 *>(smem);
-    else v.tnodes = reinterpret_cast<const AmsNode *>(smem);
+    const bool four = WIDE < 0 ? a.tnodes4 != nullptr : WIDE == 1;
+    const int n4 = four ? 8 * a.n_tnodes4 : 2 * a.n_tnodes;
+    stage_scene(four ? reinterpret_cast<const float4 *>(a.tnodes4) : reinterpret_cast<const float4 *>(a.tnodes), n4,
+                a.tris, a.n_tris, smem);
+    if (WIDE == 1) {
+      v.tnodes4 = reinterpret_cast<const AmsNode4 *>(smem);
+    } else if (WIDE == 0) {
+      v.tnodes = reinterpret_cast<const AmsNode *>(smem);
+      v.tnodes4 = nullptr;
+    } else if (four) {
+      v.tnodes4 = reinterpret_cast<const AmsNode4 *>(smem);
+    } else {
+      v.tnodes = reinterpret_cast<const AmsNode *>(smem);
+    }
     v.tris = reinterpret_cast<const AmsTri *>(smem + n4);
   }
   return v;
@@ -751,7 +763,7 @@ struct Branch {
 template <bool LDS, bool WIDE>
 __device__ __forceinline__ void discover_rays(const DiscoverArgs &a, const int rays_per_thread) {
   extern __shared__ float4 ams_smem[];
-  const SceneView sv = discovery_scene<LDS>(a, ams_smem);
+  const SceneView sv = discovery_scene<LDS, WIDE ? 1 : 0>(a, ams_smem);
   const AmsNode *tnodes = sv.tnodes;
   const AmsNode4 *tnodes4 = sv.tnodes4;
   const AmsTri *tris = sv.tris;
@@ -1070,11 +1082,11 @@ struct RefArgs {
 // order (a reflection child lands in a higher slot than its parent), so the
 // order of Rx hits -- and with it the per-tile min_hc gating and the
 // reserve cap -- follows the reference.
-template <bool LDS>
+template <bool LDS, bool WIDE>
 __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
   const DiscoverArgs &a = ra.d;
   extern __shared__ float4 ams_smem[];
-  const SceneView sv = discovery_scene<LDS>(a, ams_smem);
+  const SceneView sv = discovery_scene<LDS, WIDE ? 1 : 0>(a, ams_smem);
   const AmsNode *tnodes = sv.tnodes;
   const AmsNode4 *tnodes4 = sv.tnodes4;
   const AmsTri *tris = sv.tris;
@@ -1123,7 +1135,8 @@ __global__ void __launch_bounds__(256) k_discover_ref(RefArgs ra) {
             torig = tri_orig;
           } else {
             tt = INFINITY;
-            sl = scene_closest(tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl, torig, ct);
+            sl = scene_closest(WIDE ? tnodes4 : nullptr, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, cur.last_pl,
+                               torig, ct);
           }
           float ts = tt;
           int rx = rx_query(a, ray, ts, ct);
@@ -2732,8 +2745,11 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     ra.n_grid = (int)n_grid;
     ra.min_hc = d_minhc;
     HIPCHK(hipEventRecord(e0, st));
-    if (use_lds) k_discover_ref<true><<<(unsigned)((nthr + 255) / 256), 256, lds_bytes, st>>>(ra);
-    else k_discover_ref<false><<<(unsigned)((nthr + 255) / 256), 256, 0, st>>>(ra);
+    const unsigned rblocks = (unsigned)((nthr + 255) / 256);
+    if (use_lds && wide) k_discover_ref<true, true><<<rblocks, 256, lds_bytes, st>>>(ra);
+    else if (use_lds) k_discover_ref<true, false><<<rblocks, 256, lds_bytes, st>>>(ra);
+    else if (wide) k_discover_ref<false, true><<<rblocks, 256, 0, st>>>(ra);
+    else k_discover_ref<false, false><<<rblocks, 256, 0, st>>>(ra);
     HIPCHK(hipGetLastError());
     S.n_discover_launches++;
     HIPCHK(hipEventRecord(e1, st));
