@@ -56,10 +56,10 @@ LDS_PEAK_GBS = N_CU * 128 * PEAK_CLOCK_HZ / 1e9
 # inside the timed evaluation events): counter sums over the profiled run's
 # dispatches of both kernels divided by its evaluation launches.
 MEASURED_TRAFFIC = {
-    ("room_boxes_ms", "k_discover"): {"bytes": (1.7303e6 * 2048 + 3.0772e6 * 1024) / 6, "l2_hit": 0.890},
-    ("room_boxes_ms", "k_evaluate"): {"bytes": (6.3085e5 * 2048 + 1.5605e6 * 1024) / 26, "l2_hit": 0.734},
-    ("office", "k_discover"): {"bytes": (2.3729e6 * 2048 + 1.5978e6 * 1024) / 10, "l2_hit": 0.916},
-    ("office", "k_evaluate"): {"bytes": (4.4055e7 * 2048 + 9.4723e7 * 1024) / 180, "l2_hit": 0.820},
+    ("room_boxes_ms", "k_discover"): {"bytes": (1.7195e6 * 2048 + 3.0741e6 * 1024) / 6, "l2_hit": 0.891},
+    ("room_boxes_ms", "k_evaluate"): {"bytes": (6.3148e5 * 2048 + 1.5575e6 * 1024) / 26, "l2_hit": 0.719},
+    ("office", "k_discover"): {"bytes": (2.3763e6 * 2048 + 1.6631e6 * 1024) / 10, "l2_hit": 0.914},
+    ("office", "k_evaluate"): {"bytes": (4.4175e7 * 2048 + 9.5146e7 * 1024) / 180, "l2_hit": 0.821},
     ("office_batched", "k_cgemm"): {"bytes": 7.554e5 * 2048 + 1.311e5 * 1024, "l2_hit": 0.782, "mfma_util": 0.589},
 }
 
@@ -68,20 +68,20 @@ MEASURED_TRAFFIC = {
 # SQ_INSTS_VALU over a profiled run divided by that run's queries (the
 # tracer's counters of the same run); profiles/tracer_pmc_*.txt.
 MEASURED_VALU = {
-    ("room_boxes_ms", "k_evaluate"): 1.35665e10 / 1517460974,                       # 8.94, k_beam + k_evaluate
-    ("room_boxes_ms", "k_discover"): 1.24663e10 / 491237478,                        # 25.38, k_discover<true>
-    ("office", "k_evaluate"): 1.55694e12 / 113655676056,                            # 13.70, k_beam + k_evaluate
-    ("office", "k_discover"): 6.67732e10 / 1352819735,                              # 49.36, k_discover_wide<false>
+    ("room_boxes_ms", "k_evaluate"): 1.30833e10 / 1517460974,                       # 8.62, k_beam + k_evaluate
+    ("room_boxes_ms", "k_discover"): 1.22221e10 / 491237478,                        # 24.88, k_discover<true>
+    ("office", "k_evaluate"): 1.5053e12 / 113655676056,                             # 13.24, k_beam + k_evaluate
+    ("office", "k_discover"): 6.65919e10 / 1352819735,                              # 49.22, k_discover_wide<false>
 }
 # VALUBusy (SQ_ACTIVE_INST_VALU per CU over the GUI-active cycles) of the
 # same runs: the share of cycles the vector ALUs are busy, whatever each
 # instruction's issue cost (FP64, transcendental and 64-bit operations take
 # several cycles).
 MEASURED_VALU_BUSY = {
-    ("room_boxes_ms", "k_evaluate"): 0.853,
-    ("room_boxes_ms", "k_discover"): 0.758,
-    ("office", "k_evaluate"): 1.005,
-    ("office", "k_discover"): 0.670,
+    ("room_boxes_ms", "k_evaluate"): 0.844,
+    ("room_boxes_ms", "k_discover"): 0.765,
+    ("office", "k_evaluate"): 0.994,
+    ("office", "k_discover"): 0.664,
 }
 
 
