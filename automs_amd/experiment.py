@@ -104,7 +104,9 @@ def run_trial(scene_folder: str, params: Optional[dict] = None, steps: int = 200
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     for m in models[1:]:
-        optimize_model(m, lr=0.3, step=steps)
+        # one HIP-graph launch per step; the capacity loss's row masks are the
+        # eager loop's (drawn in the same order from the CPU generator)
+        optimize_model(m, lr=0.3, step=steps, graph=True)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     out = {"params": dict(DEFAULT_PARAMS, **(params or {})), "steps": steps,

@@ -708,17 +708,20 @@ def optimize_model(model: PhaseMapModel, lr: float = 0.3, step: int = 200, fused
     graph=True captures one whole step (forward, the hand-written backward
     and the fused Adam with its tracking) in a HIP graph and replays it:
     the step's small launches become one graph launch.  It needs the fused
-    Adam and an objective with no host input per step (the log-SNR
-    objective; the capacity objective draws its row mask on the host each
-    step).  graph_dot names a file for the captured graph's nodes (Graphviz
-    DOT, from the runtime's graph dump)."""
+    Adam.  The capacity objective draws a row mask per step from torch's CPU
+    generator (go.py:220): with graph=True all `step` masks are drawn first,
+    in the same order and so with the same values, kept on the device, and
+    each step picks its own by the Adam step count on the device; a
+    rowmask_fn of the caller cannot be replayed and is refused.  graph_dot
+    names a file for the captured graph's nodes (Graphviz DOT, from the
+    runtime's graph dump)."""
     params = [p for p in model.parameters() if p.requires_grad]
     if graph:
         if not fused:
             raise ValueError("optimize_model: graph=True needs the fused Adam")
-        if model.mode != 0:
-            raise ValueError("optimize_model: graph=True needs the log-SNR objective (the capacity objective's "
-                             "row mask is drawn on the host every step)")
+        if model.mode != 0 and model.rowmask_fn is not None:
+            raise ValueError("optimize_model: graph=True cannot replay a caller's rowmask_fn (draw the masks "
+                             "into the model's objective without one)")
     if not fused:
         return _optimize_torch_adam(model, params, lr, step, record, callback)
     opt = FusedAdam(params, lr=lr)
@@ -733,7 +736,23 @@ def optimize_model(model: PhaseMapModel, lr: float = 0.3, step: int = 200, fused
         opt.step(loss=loss, best=best, hist=hist)
         opt.zero_grad()
 
-    if graph:
+    if graph and model.mode != 0 and step > 0:
+        # the eager loop's masks, drawn up front: row i for step i, chosen on
+        # the device by the step count (the fused Adam's state[0])
+        C = model.code_num * model.n_maps
+        masks = torch.stack([torch.rand(C) < 0.1 for _ in range(step)]).to(dev)
+        cur_mask = torch.empty(C, dtype=torch.bool, device=dev)
+
+        def mask_of_step():
+            cur_mask.copy_(masks.index_select(0, opt.state[:1].long()).reshape(-1))
+            return cur_mask
+
+        model.rowmask_fn = mask_of_step
+        try:
+            _replay_steps(model, body, step, callback, graph_dot)
+        finally:
+            model.rowmask_fn = None
+    elif graph:
         _replay_steps(model, body, step, callback, graph_dot)
     else:
         for it in range(step):

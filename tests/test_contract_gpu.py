@@ -543,8 +543,8 @@ def test_graphed_optimiser_matches_eager(gpu, tmp_path):
     (best-loss) parameters equal the eager loop's within 1e-6 relative: the
     same kernels run in the same order.  Between replays the callback runs
     eager work (a clone, a device sync every third step).  The captured
-    graph holds no memset node.  The capacity objective, whose row mask is
-    drawn on the host each step, is refused."""
+    graph holds no memset node.  A caller's rowmask_fn, which a replay
+    cannot call, is refused."""
     from automs_amd.phasemap import Metasurface, PhaseMapModel, optimize_model
     # the room workload's shapes: 16 AP elements, a 32 x 32 metasurface, 4096 Rx
     Hd, G, F = _channels(K=1024, N=4096, seed=31)
@@ -581,8 +581,46 @@ def test_graphed_optimiser_matches_eager(gpu, tmp_path):
     else:
         _flush("  no graph dump written by this runtime")
     cap = PhaseMapModel("c", ch, ms, code_num=30, objective="capacity", ms_theta=[th_ms], antenna_theta=th_ap)
+    cap.rowmask_fn = lambda: torch.zeros(30, dtype=torch.bool)
     with pytest.raises(ValueError):
         optimize_model(cap, step=2, graph=True)
+
+
+@pytest.mark.parametrize("kind", ["metasurface", "frozen", "codebook"])
+def test_graphed_optimiser_capacity(gpu, kind):
+    """The capacity objective under optimize_model(graph=True): the row
+    masks are drawn up front from the CPU generator and picked per step on
+    the device.  Against the eager loop from the same generator state, for
+    the reference run's three kinds of model (metasurface optimised,
+    metasurface frozen, codebook alone): the same loss history and final
+    parameters (1e-6 relative) and the same generator state afterwards."""
+    from automs_amd.phasemap import Metasurface, PhaseMapModel, optimize_model
+    _flush(f"case: graphed optimiser, capacity objective, {kind}")
+    Hd, G, F = _channels(M=36, K=200, N=900, seed=77)
+    if kind == "codebook":
+        ch, ms = {"src_to_dst": Hd.to(torch.complex64).to(gpu)}, []
+    else:
+        ch = {"src_to_dst": Hd.to(torch.complex64).to(gpu), "src_to_ms{0}": G.to(torch.complex64).to(gpu),
+              "ms{0}_to_dst": F.to(torch.complex64).to(gpu)}
+        ms = [Metasurface(None, (10, 20), None, None, [], max_phase=360.0)]
+    res = {}
+    for graph in (False, True):
+        model = PhaseMapModel("t", ch, ms, code_num=30, w=0.5, objective="capacity", seed=3)
+        if kind == "frozen":
+            model.ms_theta[0].requires_grad_(False)
+        torch.manual_seed(5)
+        optimize_model(model, lr=0.3, step=12, graph=graph)
+        torch.cuda.synchronize()
+        after = torch.rand(4)
+        res[graph] = (torch.tensor(model.loss_list, dtype=torch.float64), model.antenna_theta.detach().cpu().double(),
+                      (model.ms_theta[0].detach().cpu().double() if ms else torch.zeros(1, dtype=torch.float64)),
+                      after.double())
+    for name, a, b in zip(("loss", "ap", "ms", "generator"), res[False], res[True]):
+        rel = ((a - b).norm() / max(a.norm(), 1e-30)).item()
+        _flush(f"  {name}: rel {rel:.2e} (bitwise equal: {torch.equal(a, b)})")
+        assert a.shape == b.shape and rel <= 1e-6
+    assert torch.equal(res[False][3], res[True][3])
+    assert res[True][0][-1] < res[True][0][0]
 
 
 @pytest.mark.parametrize("steps", [0, 1, 2, 3])
