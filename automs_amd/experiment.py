@@ -71,10 +71,14 @@ def coverage_stats(model: PhaseMapModel, floor_db: float = -70.0) -> dict:
 
 
 def build_trial_channels(scene_folder: str, src, dst, ms, ap_dir, max_reflection: int = 4) -> Dict[str, dict]:
-    """The two channel sets of physical_exp_nni_model.py:76-90."""
-    noms = channel.build_channels(src, dst, [], ap_dir, scene_folder, WAVE_LEN, max_reflection)
-    with_ms = channel.build_channels(src, dst, [ms], ap_dir, scene_folder, WAVE_LEN, max_reflection)
-    return {"noms": noms, "ms": with_ms}
+    """The two channel sets of physical_exp_nni_model.py:76-90, traced at the
+    same time (independent traces; the same channels as one after the
+    other)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        noms = ex.submit(channel.build_channels, src, dst, [], ap_dir, scene_folder, WAVE_LEN, max_reflection)
+        with_ms = ex.submit(channel.build_channels, src, dst, [ms], ap_dir, scene_folder, WAVE_LEN, max_reflection)
+        return {"noms": noms.result(), "ms": with_ms.result()}
 
 
 def trial_models(channels: Dict[str, dict], ms: Metasurface, code_num: int = 30, w: float = 0.5):
