@@ -2721,6 +2721,9 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
   // outgrows its buffer.  A chunk whose records overflow is split and redone
   // (the kernel is deterministic, so the split gives the same records).
   size_t acc_n = 0, acc_cap = 1 << 20;
+  // the running set is one sorted, duplicate-free run (one launch's unique
+  // records, or a compacted set): its final sort + unique is then skipped
+  bool acc_sorted = true;
   uint64_t *a_hi = (uint64_t *)sc.get("acc_hi", acc_cap * 8), *a_lo = (uint64_t *)sc.get("acc_lo", acc_cap * 8);
   if (!a_hi || !a_lo) return AMS_ERR_ALLOC;
   long long cur_chunk = chunk;
@@ -2851,6 +2854,7 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     }
     HIPCHK(hipMemcpyAsync(a_hi + acc_n, c_hi, nu * 8, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(a_lo + acc_n, c_lo, nu * 8, hipMemcpyDeviceToDevice, st));
+    acc_sorted = acc_n == 0;  // a second run appended: no longer sorted as a whole
     acc_n += nu;
     rb = re;
     cur_chunk = chunk;
@@ -2873,9 +2877,11 @@ int trace_channel(ams_scene *s, const ams_array *tx, const ams_array *rx, const 
     acc_hi.swap(h2);
     acc_lo.swap(l2);
   } else {
-    size_t m = 0;
-    int rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m, kbits);
-    if (rc) return rc;
+    size_t m = acc_n;
+    if (!acc_sorted) {
+      int rc = sort_unique(sc, a_hi, a_lo, acc_n, st, &m, kbits);
+      if (rc) return rc;
+    }
     acc_hi.resize(m);
     acc_lo.resize(m);
     if (m) {  // through page-locked memory, as read_back
