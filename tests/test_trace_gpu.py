@@ -433,3 +433,34 @@ def test_slab_bounded_last_leg(small_room, trace_scene, which):
         ts.close()
         assert len(ka ^ kb) <= max(2, len(kb) // 10000)
         assert rel < 1e-6
+
+
+def test_discovery_in_two_launches(gpu):
+    """2^25 Fibonacci rays (16 AP elements x 2^21) are discovered in two
+    launches of 2^24 rays whose unique records are merged on the device.
+    Records depend on each ray alone, so the set equals the union of the
+    sets of the AP's two halves (rows 0-1 and 2-3, 2^24 rays each, one
+    launch), and it comes back sorted and free of duplicates."""
+    from automs_amd import scenes
+    from automs_amd.scenes import ArraySpec
+    from automs_amd.tracer import TraceScene, trace
+    w = scenes.room_boxes(fib_n=1 << 21)
+    _flush(f"case: discovery in two launches, room_boxes, {w.tx.n} x 2^21 rays")
+    ts = TraceScene(w.scene)
+
+    def run(tx):
+        _, st, keys = trace(ts, tx, w.rx, w.wave_len, w.max_reflection, w.fib_n, w.per_element, w.gridsize,
+                            flags=2, keys_cap=1 << 22)
+        return st, np.asarray(keys, dtype=np.uint64)
+
+    st, full = run(w.tx)
+    halves = [run(ArraySpec(np.ascontiguousarray(w.tx.pos[r0:r0 + 2]), w.tx.dir, w.tx.bptype, w.tx.radius))
+              for r0 in (0, 2)]
+    ts.close()
+    _flush(f"  launches {st['n_discover_launches']}, records {len(full)}; halves "
+           f"{[h[0]['n_discover_launches'] for h in halves]} launches, {[len(h[1]) for h in halves]} records")
+    assert st["n_discover_launches"] == 2 and all(h[0]["n_discover_launches"] == 1 for h in halves)
+    assert len(full) == st["records_unique"]
+    pairs = [tuple(k) for k in full.tolist()]
+    assert pairs == sorted(set(pairs))
+    assert set(pairs) == set(map(tuple, halves[0][1].tolist())) | set(map(tuple, halves[1][1].tolist()))
