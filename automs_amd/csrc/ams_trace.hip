@@ -1280,8 +1280,8 @@ __device__ __forceinline__ size_t beam_index(const EvalArgs &a, int tl, int q) {
 }
 
 // Wave-uniform reads through the constant address space, so the compiler
-// issues scalar loads (the lists and the scene are not written by the
-// kernels that read them).
+// issues scalar loads (the lists, the scene, the path keys, the tasks'
+// paths and the planes are not written by the kernels that read them).
 typedef const __attribute__((address_space(4))) int cint4_t;
 typedef const __attribute__((address_space(4))) float cfloat4_t;
 typedef const __attribute__((address_space(4))) double cdouble4_t;
@@ -1432,7 +1432,8 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
       // image of the Tx element across the reflection planes (FP64)
       double3 I = make_double3(s3.x, s3.y, s3.z);
       for (int k = 0; k < nref; k++) {
-        const AmsPlane P = UNI ? uni_plane(a.planes + key_get_plane(khi, klo, k)) : a.planes[key_get_plane(khi, klo, k)];
+        const uint32_t pid = key_get_plane(khi, klo, k);
+        const AmsPlane P = UNI ? uni_plane(a.planes + pid) : a.planes[pid];
         double s = 2.0 * (I.x * P.n[0] + I.y * P.n[1] + I.z * P.n[2] - P.d);
         I.x -= s * P.n[0];
         I.y -= s * P.n[1];
