@@ -720,8 +720,8 @@ def optimize_model(model: PhaseMapModel, lr: float = 0.3, step: int = 200, fused
         if not fused:
             raise ValueError("optimize_model: graph=True needs the fused Adam")
         if model.mode != 0 and model.rowmask_fn is not None:
-            raise ValueError("optimize_model: graph=True cannot replay a caller's rowmask_fn (draw the masks "
-                             "into the model's objective without one)")
+            raise ValueError("optimize_model: graph=True cannot replay a caller's rowmask_fn; leave it unset and "
+                             "the row masks are drawn as go.py:220 draws them")
     if not fused:
         return _optimize_torch_adam(model, params, lr, step, record, callback)
     opt = FusedAdam(params, lr=lr)
