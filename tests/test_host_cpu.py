@@ -448,3 +448,25 @@ def test_fused_adam_rejects_a_wrong_best_buffer():
     for bad in (torch.zeros(4), torch.zeros(5, dtype=torch.float64), torch.zeros(10)[::2]):
         with pytest.raises(ValueError):
             opt.step(loss=loss, best=bad)
+
+
+def test_split_count_rule():
+    """Reduction splits of the per-wave contraction kernel: about 512 waves
+    of 32 x 64 output tiles for short reductions; for long ones (200 and
+    more 32-deep blocks) the most splits that keep the waves at or below
+    1024, one per SIMD.  Values at the shapes they were measured on."""
+    from automs_amd.phasemap import split_count
+
+    def waves(Ra, Rb, Kd):
+        return ((Ra + 31) // 32) * ((Rb + 63) // 64) * split_count(Ra, Rb, Kd)
+
+    # the reference run's 30 x 18900 x 32036 product and its backward
+    assert split_count(30, 18900, 32036) == 3 and split_count(30, 32036, 18900) == 2
+    # the office three-surface backward and chain products; its forward fills the GPU unsplit
+    assert split_count(30, 6976, 65536) == 9 and split_count(30, 6912, 6912) == 9
+    assert split_count(30, 65536, 6976) == 1
+    # config 1's short reductions keep the 512-wave target
+    assert split_count(30, 4096, 1040) == 8 and split_count(30, 1040, 4096) == 30
+    for shape in [(30, 18900, 32036), (30, 32036, 18900), (30, 6976, 65536), (30, 6912, 6912), (64, 9000, 20000)]:
+        assert waves(*shape) <= 1024
+        assert split_count(*shape) <= (shape[2] + 31) // 32
