@@ -1338,6 +1338,52 @@ __device__ __forceinline__ int cand_closest(const AmsTri *__restrict__ tris, con
   return hit;
 }
 
+// closest_tri for a wave whose active lanes walk together (their rays leave
+// one Rx in nearly one direction): the wave visits a node when any lane's
+// box test passes, so the node index is wave-uniform and the node and its
+// triangles are read with scalar loads.  A lane's own walk visits a subset
+// of these nodes and tests every triangle it could hit, so its closest hit
+// (ties to the smaller original index) is closest_tri's.
+__device__ __forceinline__ int closest_tri_wave(const AmsNode *__restrict__ nodes, int n_nodes,
+                                                const AmsTri *__restrict__ tris, const Ray &r, float tmin,
+                                                float &tbest, int skip_pl, int &orig_out, Counters &ct) {
+  int hit = -1, horig = 0x7fffffff;
+  int i = 0;
+  while (i < n_nodes) {
+    const float4 lo = uni_f4(reinterpret_cast<const float4 *>(nodes + i));
+    const float4 hi = uni_f4(reinterpret_cast<const float4 *>(nodes + i) + 1);
+    ct.nodes++;
+    const int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
+    const bool h = box_test(r, lo, hi, tmin, tbest);
+    if (__ballot(h)) {
+      if (info == 0) {
+        i++;
+        continue;
+      }
+      if (h) {
+        const int first = info >> 3, cnt = info & 7;
+        for (int k = 0; k < cnt; k++) {
+          const AmsTri tr = uni_tri(tris + first + k);
+          ct.tris++;
+          if (__float_as_int(tr.b.w) == skip_pl) continue;
+          float t;
+          if (tri_test(r, tr.a, tr.b, tr.c, tmin, tbest, t)) {
+            int orig = __float_as_int(tr.a.w);
+            if (t < tbest || orig < horig) {
+              tbest = t;
+              hit = first + k;
+              horig = orig;
+            }
+          }
+        }
+      }
+    }
+    i = miss;
+  }
+  orig_out = horig;
+  return hit;
+}
+
 // Path length of a successful chain along its reflection triangles' own
 // planes, in FP64: the length the chain traced (optixPathTracer.cu:344 sums
 // the traced segments), for a chain whose reflection triangles are only
@@ -1533,7 +1579,8 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs &a) {
           slot = cand_closest<UNI && !LDS>(tris, cand, (int)nc, ray, AMS_TMIN, tt, last_pl, torig, ct);
         } else {
           slot = WIDE ? scene_closest(a.tnodes4, tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct)
-                      : closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
+                 : (UNI && !LDS) ? closest_tri_wave(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct)
+                                 : closest_tri(tnodes, a.n_tnodes, tris, ray, AMS_TMIN, tt, last_pl, torig, ct);
           if (BEAM) ct.sph++;  // a walk of the hierarchy
         }
         ct.queries++;
