@@ -157,6 +157,11 @@ def test_office_reduced(gpu, name):
     beam, walk = _run(ts, w, flags), _run(ts, w, OFF)
     ts.close()
     _same(beam, walk, f"office {name}")
+    if name == "F0":
+        # 2304 Tx: every wave shares one Rx, and the legs whose lists overflow
+        # walk the hierarchy as one wave (scalar loads); H above is bitwise the
+        # per-lane walk's
+        assert beam[2]["beam_walk_legs"] > 0 and beam[2]["bwd_walks"] > 0
 
 
 def test_office_device_rounds(gpu):
