@@ -106,6 +106,72 @@ def test_threaded_bvh_closest_hit_equals_brute_force():
         assert bi == bbi
 
 
+def test_wave_walk_equals_lane_walks():
+    """closest_tri_wave's rule on the threaded hierarchy: the wave enters a
+    node when any lane's box test passes and a lane tests a leaf's triangles
+    only when its own test passes.  Each lane's closest hit equals its own
+    stackless walk's and brute force, for coherent waves (one origin, nearly
+    one direction) and incoherent ones."""
+    rng = np.random.default_rng(7)
+    T = 160
+    tri = (rng.uniform(-4, 4, (T, 1, 3)) + rng.normal(0, 0.5, (T, 3, 3)))
+    blo, miss, bhi, info, order = _bvh(tri.min(1), tri.max(1))
+    n_nodes = len(miss)
+
+    def box(i, o, inv, best):
+        t0 = np.minimum((blo[i] - o) * inv, (bhi[i] - o) * inv).max()
+        t1 = np.maximum((blo[i] - o) * inv, (bhi[i] - o) * inv).min()
+        return max(t0, 1e-5) <= min(t1, best) * (1 + 4e-7)
+
+    def leaf(i, o, d, best, bi):
+        for p in order[(info[i] >> 3):(info[i] >> 3) + (info[i] & 7)]:
+            t = _mt(o, d, *tri[p])
+            if t is not None and 1e-5 <= t < best:
+                best, bi = t, p
+        return best, bi
+
+    visits = [0, 0]
+    for coherent in (True, False):
+        for _ in range(3):
+            o0 = rng.uniform(-5, 5, 3)
+            d0 = rng.normal(size=3)
+            if coherent:
+                O = o0 + rng.normal(0, 0.05, (64, 3))
+                D = d0 / np.linalg.norm(d0) + rng.normal(0, 0.05, (64, 3))
+            else:
+                O = rng.uniform(-5, 5, (64, 3))
+                D = rng.normal(size=(64, 3))
+            D /= np.linalg.norm(D, axis=1, keepdims=True)
+            inv = 1 / np.where(D == 0, 1e-30, D)
+            best, bi = np.full(64, np.inf), np.full(64, -1)
+            i = 0
+            while i < n_nodes:  # one node index for the wave
+                visits[coherent] += 1
+                h = [box(i, O[k], inv[k], best[k]) for k in range(64)]
+                if any(h):
+                    if info[i] == 0:
+                        i += 1
+                        continue
+                    for k in range(64):
+                        if h[k]:
+                            best[k], bi[k] = leaf(i, O[k], D[k], best[k], bi[k])
+                i = miss[i]
+            for k in range(64):
+                lb, lbi, j = np.inf, -1, 0
+                while j < n_nodes:  # the lane's own walk
+                    if box(j, O[k], inv[k], lb):
+                        if info[j] == 0:
+                            j += 1
+                            continue
+                        lb, lbi = leaf(j, O[k], D[k], lb, lbi)
+                    j = miss[j]
+                hits = [(_mt(O[k], D[k], *tri[p]), p) for p in range(T)]
+                hits = [(t, p) for t, p in hits if t is not None and t >= 1e-5]
+                assert bi[k] == lbi == (min(hits)[1] if hits else -1)
+                assert best[k] == lb
+    assert visits[True] < visits[False]  # coherent waves visit fewer nodes
+
+
 def _bvh4(lo, hi, leaf_max=4):
     from automs_amd import _lib
     L = _lib.lib()
